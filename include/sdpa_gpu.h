@@ -1,0 +1,119 @@
+/*
+ * sdpa_gpu.h -- C ABI of the MI355X (gfx950) Newton-direction hot path of
+ * SDPA 7.4.4.  This is the drop-in boundary: the host interior-point loop
+ * (SDPA::solve, sdpa_solve.cpp:59-233) calls only these entry points, at the
+ * seams where the reference calls Newton::Mehrotra, StepLength::
+ * computeStepLength and Solutions::update (sdpa_solve.cpp:96-197).
+ *
+ * Conventions
+ *  - Plain C types only; every entry returns int: 0 = success, > 0 = a
+ *    LAPACK-style info (the 1-based failing pivot, or 1 for "the X/Z
+ *    factorization failed"), < 0 = an error (see sdpa_gpu_last_error).
+ *    Nothing here calls exit().
+ *  - Problem data follow the .dat-s records (sdpa_io.cpp:473-606): k = 0..m
+ *    (0 is F0), l = 1..nBlock, i, j 1-based, LP blocks have negative size.
+ *    Internally C = -F0, A_k = F_k, b = c (the reference's reversed
+ *    primal/dual, sdpa_tool.h:95).
+ *  - Matrices in the "packed block space" are the SDP blocks column-major,
+ *    concatenated in block order, followed by the LP coordinates.  All
+ *    iterate arrays (X, Z, Dx, Dz, ...) stay resident on the device; host
+ *    copies move only through set/get calls.
+ *  - One context per host thread; it owns one HIP stream and all buffers.
+ */
+#ifndef SDPA_GPU_H
+#define SDPA_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdpa_gpu_ctx sdpa_gpu_ctx;
+
+/* Context lifetime.  No reference counterpart (the reference keeps this state
+ * inside class SDPA, sdpa_call.h:91-267). */
+int sdpa_gpu_create(int device, sdpa_gpu_ctx** ctx);
+int sdpa_gpu_destroy(sdpa_gpu_ctx* ctx);
+const char* sdpa_gpu_last_error(sdpa_gpu_ctx* ctx);
+
+/* Load a problem.  Replaces IO::read/SDPA::inputElement + initializeUpperTriangle
+ * (sdpa_io.cpp:253-606, sdpa_call.cpp:285-341, 650-666) and the structural part
+ * of SDPA::initializeSolve (sdpa_solve.cpp:10-29): changeToDense
+ * (sdpa_struct.cpp:510-555), InputData::initialize_index (sdpa_dataset.cpp:268-400),
+ * Chordal::ordering_bMat's dense decision (sdpa_chordal.cpp:365-410) and
+ * Newton::computeFormula_SDP with kappa (sdpa_newton.cpp:515-695). */
+int sdpa_gpu_load_problem(sdpa_gpu_ctx* ctx, int m, int nBlock, const int* blockStruct,
+                          const double* c, long nnz, const int* k, const int* l, const int* i,
+                          const int* j, const double* v, double kappa);
+
+/* out[0..11] = m, nBlock, nSDP, nLP, nDim, packed length, #F1, #F2, #F3,
+ * schur_dense (1 if the reference would choose the dense Schur path),
+ * bmat_store_first, number of SDP blocks.  Returns the count written. */
+int sdpa_gpu_problem_info(sdpa_gpu_ctx* ctx, long* out, int cap);
+/* Size of SDP block q (0-based) or -1. */
+int sdpa_gpu_block_size(sdpa_gpu_ctx* ctx, int q);
+
+/* Iterate in / out (internal convention, packed block space; y length m).
+ * Any pointer may be NULL.  Replaces the Solutions members (sdpa_dataset.h). */
+int sdpa_gpu_set_iterate(sdpa_gpu_ctx* ctx, const double* X, const double* y, const double* Z);
+int sdpa_gpu_get_iterate(sdpa_gpu_ctx* ctx, double* X, double* y, double* Z);
+/* X = Z = lambda I, y = 0, invChol = I/sqrt(lambda), Zinv = I/lambda
+ * (Solutions::initialize, sdpa_dataset.cpp:47-74). */
+int sdpa_gpu_identity_start(sdpa_gpu_ctx* ctx, double lambda);
+/* Remember the current X, Z as the initial point X0, Z0 (init-point runs,
+ * sdpa_solve.cpp:61-72). */
+int sdpa_gpu_keep_initial_point(sdpa_gpu_ctx* ctx);
+
+/* Solutions::computeInverse (sdpa_dataset.cpp:117-142): Cholesky of every X
+ * and Z block (dpotrf semantics), invCholX, invCholZ and Zinv.  Returns 1 if
+ * a factorization fails. */
+int sdpa_gpu_inv_chol(sdpa_gpu_ctx* ctx);
+
+/* Newton::Make_bMat (sdpa_newton.cpp:2634-2670) into the device bMat.
+ * precision: 0 = FP64. */
+int sdpa_gpu_make_bmat(sdpa_gpu_ctx* ctx, int precision);
+/* Lal::choleskyFactorWithAdjust -> rATL_dpotrfL (sdpa_linear.cpp:762-790,
+ * sdpa_dpotrf.cpp:218-265): in-place lower Cholesky of bMat with SDPA's
+ * pivot rule.  Returns 0 or the 1-based failing column. */
+int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* ctx, int precision);
+/* Lal::solveSystems (sdpa_linear.cpp:843-862): dy = bMat^-1 gVec with the
+ * factor left by sdpa_gpu_factor_bmat. */
+int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* ctx, int refine);
+
+/* StepLength::computeStepLength's eigenvalue estimates (sdpa_parts.cpp:363-432,
+ * Jal::getMinEigen sdpa_jordan.cpp:58-109): lamX = lambda_min(L_X^-1 Dx L_X^-T),
+ * lamZ likewise for Z, each the minimum over blocks (LP included). */
+int sdpa_gpu_min_eig(sdpa_gpu_ctx* ctx, double* lamX, double* lamZ);
+
+/* Residuals::compute (sdpa_dataset.cpp:585-608): primal p = b - A(X), dual
+ * D = C - sum y_i A_i - Z, and their max norms. */
+int sdpa_gpu_residuals(sdpa_gpu_ctx* ctx, double* normPrimalVec, double* normDualMat);
+/* Newton::Make_gVec (sdpa_newton.cpp:817-883): r_zinv = target Zinv - X
+ * (- Dx Dz Zinv if corrector), minus X D Zinv when `infeasible` (phase noINFO
+ * or pFEAS), g = -A(.) + p. */
+int sdpa_gpu_make_gvec(sdpa_gpu_ctx* ctx, int corrector, double target, int infeasible);
+/* compute_DzMat / compute_DxMat (sdpa_newton.cpp:2718-2752). */
+int sdpa_gpu_make_direction(sdpa_gpu_ctx* ctx, int infeasible);
+/* Inner products for the control scalars; codes:
+ *  0 C.X  1 b.y  2 C.Dx  3 b.dy  4 X.Dz  5 Dx.Z  6 Dx.Dz  7 X.Z
+ *  8 trace X  9 trace Z  10 X0.Z  11 X.Z0 */
+int sdpa_gpu_dots(sdpa_gpu_ctx* ctx, int nq, const int* codes, double* out);
+/* Solutions::update's step (sdpa_dataset.cpp:151-158). */
+int sdpa_gpu_update(sdpa_gpu_ctx* ctx, double alphaP, double alphaD);
+
+/* Buffer access for tests and callers that keep their own copies.
+ * which: 0 X, 1 Z, 2 y, 3 Zinv, 4 invCholX, 5 invCholZ, 6 Dx, 7 Dz, 8 dy,
+ *        9 gVec, 10 bMat (m*m), 11 primal residual, 12 dual residual.
+ * len must equal the buffer's length. */
+int sdpa_gpu_get_buffer(sdpa_gpu_ctx* ctx, int which, double* dst, long len);
+int sdpa_gpu_set_buffer(sdpa_gpu_ctx* ctx, int which, const double* src, long len);
+
+/* Per-phase device timing (HIP events on the context's stream).  When
+ * enabled each timed phase synchronizes.  out[0..9] = ms and call counts for
+ * bMat, Cholesky, solve, eigen, inverse. */
+int sdpa_gpu_timing(sdpa_gpu_ctx* ctx, int enable, double* out10);
+int sdpa_gpu_synchronize(sdpa_gpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDPA_GPU_H */

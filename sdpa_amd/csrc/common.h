@@ -1,0 +1,79 @@
+// common.h -- shared definitions of the sdpa_amd HIP library (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+namespace sdpa_amd {
+
+#define SDPA_HIP_CHECK(expr)                                                           \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      std::fprintf(stderr, "sdpa_amd: HIP error %s at %s:%d: %s\n", hipGetErrorString(_e), \
+                   __FILE__, __LINE__, #expr);                                         \
+      return -1000 - (int)_e;                                                          \
+    }                                                                                  \
+  } while (0)
+
+constexpr int kWave = 64;
+
+// Descriptor of one column-major matrix inside a batch.
+struct MatDesc {
+  double* ptr;
+  int n;    // rows == cols for the square factorizations
+  int ld;
+};
+
+// Batched GEMM: C = alpha * op(A) * op(B) + beta * C for every item.
+struct GemmItem {
+  const double* A;
+  const double* B;
+  double* C;
+  int M, N, K;
+  int lda, ldb, ldc;
+};
+
+// ---- dense kernels (dense.hip) ----
+// transA/transB: 0 = N, 1 = T.  lower_only: compute only tiles/elements with
+// row >= col of C (C square).  Items may have different shapes.
+int gemm_batched(hipStream_t s, const GemmItem* d_items, int nitems, int maxM, int maxN,
+                 int transA, int transB, double alpha, double beta, int lower_only);
+// Convenience single GEMM (copies one item to a device scratch slot).
+struct DenseWork;  // per-context scratch for descriptor arrays
+int gemm(hipStream_t s, DenseWork* w, const double* A, int lda, const double* B, int ldb,
+         double* C, int ldc, int M, int N, int K, int transA, int transB, double alpha,
+         double beta, int lower_only);
+
+// Blocked lower Cholesky of a batch of matrices.  mode 0: LAPACK dpotrf
+// semantics (fail at the first pivot that is not > 0); mode 1: SDPA's adjust
+// rule (pivot < -1e-6 fails, pivot < 1e-14 becomes 1e100).  d_info[b]
+// receives 0 or the 1-based failing column of item b.
+int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitems, int mode,
+                  int* d_info);
+// W = L^-1 for a batch of lower factors (upper parts of W zeroed).
+int trtri_batched(hipStream_t s, DenseWork* w, const MatDesc* h_L, const MatDesc* h_W,
+                  int nitems);
+// x := L^-T L^-1 x (two triangular solves with the lower factor L, m x m).
+int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, double* x);
+
+struct DenseWork {
+  GemmItem* d_items = nullptr;  // device array of descriptors
+  int cap_items = 0;
+  MatDesc* d_mats = nullptr;
+  int cap_mats = 0;
+  double* d_dinv = nullptr;  // inverses of 64x64 diagonal blocks
+  size_t cap_dinv = 0;
+  double* d_vec = nullptr;   // m-vector scratch of the Schur solves
+  size_t cap_vec = 0;
+  int ensure_items(int n);
+  int ensure_vec(size_t n);
+  int ensure_mats(int n);
+  int ensure_dinv(size_t n);
+  void release();
+};
+
+}  // namespace sdpa_amd

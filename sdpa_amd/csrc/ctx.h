@@ -1,0 +1,120 @@
+// ctx.h -- the device-resident state behind one sdpa_gpu_ctx (internal).
+#pragma once
+#include "common.h"
+
+#include <string>
+
+namespace sdpa_amd {
+
+// Constraint table of one SDP block (InputData::SDP_constraint and the
+// SparseMatrix data of every A_k that has this block, sdpa_dataset.cpp:268-309).
+struct BlockTable {
+  int n = 0;          // block size
+  long off = 0;       // packed offset
+  int ncon = 0;       // constraints with data in this block
+  int nF1 = 0, nF3 = 0;
+  int nheavy = 0;
+  int ndense = 0;     // constraints whose block is dense (changeToDense)
+  // device arrays, length ncon (or as noted)
+  int* d_con = nullptr;    // global constraint index (increasing)
+  int* d_estart = nullptr; // first entry in the entry pool
+  int* d_ecnt = nullptr;   // stored entries (NonZeroCount)
+  int* d_nzeff = nullptr;  // NonZeroEffect
+  int* d_form = nullptr;   // 1, 2, 3 (sdpa_newton.cpp:515-695)
+  int* d_dpos = nullptr;   // column in the dense pool of this block, -1 if sparse
+  int* d_heavy = nullptr;  // local indices of heavy constraints (nheavy)
+  int* d_f1own = nullptr;  // local indices of F1/F2 owners (nF1)
+  int* d_dcon = nullptr;   // global constraint of each dense pool column (ndense)
+  double* d_dense = nullptr;  // n*n x ndense dense pool (column per dense A_k)
+  std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DenseWork dw;
+  bool loaded = false;
+  // problem structure
+  int m = 0, nBlock = 0, nSDP = 0, nLP = 0, nDim = 0;
+  long total = 0, lpOff = 0;
+  std::vector<int> sdpN;
+  std::vector<long> sdpOff;
+  int* d_sdpN = nullptr;
+  long* d_sdpOff = nullptr;
+  std::vector<int> blockType, blockNumber, blockStruct;
+  int nF[3] = {0, 0, 0};
+  int schurDense = 1;
+  bool bmatStoreFirst = false;  // single block covering all constraints, all F3
+  std::vector<BlockTable> blk;
+  // SDP sparse entry pool (local row/col, value)
+  int* d_er = nullptr;
+  int* d_ec = nullptr;
+  double* d_ev = nullptr;
+  long nent = 0;
+  // LP coordinates: CSR over coordinates of (constraint, value)
+  int* d_lpptr = nullptr;
+  int* d_lpcon = nullptr;
+  double* d_lpval = nullptr;
+  // constraint-major sparse terms for A_i . D: (p1, p2, v), CSR over constraints
+  int* d_cptr = nullptr;
+  long* d_cp1 = nullptr;
+  long* d_cp2 = nullptr;
+  double* d_cv = nullptr;
+  // position-major terms for sum_i y_i A_i: CSR over distinct packed positions
+  long npos = 0;
+  long* d_ppos = nullptr;
+  long* d_pptr = nullptr;
+  int* d_pcon = nullptr;
+  double* d_pval = nullptr;
+  // dense A blocks: (constraint, block) list for dots and GEMV
+  int ndenseTot = 0;
+  double* d_b = nullptr;   // m
+  double* d_C = nullptr;   // packed dense C (internal, = -F0)
+  // iterate and work (packed block space unless noted)
+  double *d_X = nullptr, *d_Z = nullptr, *d_y = nullptr;
+  double *d_invCholX = nullptr, *d_invCholZ = nullptr, *d_Zinv = nullptr;
+  double *d_LX = nullptr, *d_LZ = nullptr;
+  double *d_Dx = nullptr, *d_Dz = nullptr, *d_dy = nullptr;
+  double *d_rzinv = nullptr, *d_xrdzinv = nullptr, *d_g = nullptr;
+  double *d_w1 = nullptr, *d_w2 = nullptr, *d_w3 = nullptr;
+  double *d_pvec = nullptr, *d_dmat = nullptr;  // residuals
+  double *d_X0 = nullptr, *d_Z0 = nullptr;      // initial point (init-point runs)
+  double* d_B = nullptr;   // m x m Schur matrix, column-major, lower triangle used
+  double* d_G = nullptr;   // F1 scratch: n*n per owner in a chunk
+  size_t capG = 0;
+  double* d_F1C = nullptr;  // F1 dense-pair products
+  size_t capF1C = 0;
+  int* d_info = nullptr;   // factorization info (one per batch item)
+  double* d_red = nullptr;  // reduction partials
+  double* h_red = nullptr;  // pinned host mirror of reduction results
+  int* h_info = nullptr;    // pinned
+  double* d_eig = nullptr;  // per-item eigen results
+  // timing (ms), accumulated since the last reset
+  hipEvent_t ev[8];
+  double t_bmat = 0, t_chol = 0, t_solve = 0, t_eig = 0, t_inv = 0;
+  int n_bmat = 0, n_chol = 0, n_solve = 0;
+  bool timing = false;
+  std::string last_error;
+};
+
+// schur.hip
+int make_bmat(Ctx* c);
+// ops.hip
+enum DotCode {
+  DOT_CX = 0, DOT_BY, DOT_CDX, DOT_BDY, DOT_XDZ, DOT_DXZ, DOT_DXDZ, DOT_XZ,
+  DOT_TRX, DOT_TRZ, DOT_X0Z, DOT_XZ0, DOT_COUNT
+};
+int compute_dots(Ctx* c, int nq, const int* codes, double* out);
+int residuals(Ctx* c, double* normP, double* normD);
+int make_gvec(Ctx* c, int corrector, double target, int infeasible);
+int make_direction(Ctx* c, int infeasible);
+int update_point(Ctx* c, double alphaP, double alphaD);
+int inverse_chol(Ctx* c);
+int sum_yA(Ctx* c, const double* d_yv, double* d_out);  // out = sum_i y_i A_i (packed)
+int a_dot(Ctx* c, const double* d_D, double* d_out);    // out_i = A_i . D
+int triple_product(Ctx* c, const double* A, const double* B, const double* C, double* R);
+int lp_min_step(Ctx* c, const double* D, const double* L, double* out);
+// eig.hip
+int min_eig(Ctx* c, double* lamX, double* lamZ);
+
+}  // namespace sdpa_amd

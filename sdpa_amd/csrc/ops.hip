@@ -1,0 +1,450 @@
+// ops.hip -- the device side of the Newton-direction assembly and the
+// iterate bookkeeping around the Schur solve, on packed block spaces:
+//   A_i . D for every i          InputData::multi_InnerProductToA  (sdpa_dataset.cpp:414-424)
+//   sum_i y_i A_i                InputData::multi_plusToA          (sdpa_dataset.cpp:427-433)
+//   residuals                    Residuals::compute                (sdpa_dataset.cpp:585-608)
+//   r_zinv, gVec                 Newton::compute_rMat/Make_gVec    (sdpa_newton.cpp:793-883)
+//   Dz, Dx                       compute_DzMat/compute_DxMat       (sdpa_newton.cpp:2718-2752)
+//   step update                  Solutions::update                 (sdpa_dataset.cpp:145-171)
+//   X/Z inverse Cholesky         Solutions::computeInverse         (sdpa_dataset.cpp:117-142)
+//   inner products for the control scalars (sdpa_parts.cpp).
+// Reductions are two-pass with a fixed order, so results are reproducible.
+#include "ctx.h"
+
+#include <cmath>
+
+namespace sdpa_amd {
+
+constexpr int kRedChunk = 4096;   // elements per partial sum
+constexpr int kMaxChunks = 4096;  // partial slots per dot
+
+// out_i = sum over the sparse terms of constraint i of v*(D[p1] + D[p2]) (p2 != p1)
+__global__ __launch_bounds__(256) void k_adot_sparse(const int* __restrict__ cptr,
+                                                     const long* __restrict__ p1,
+                                                     const long* __restrict__ p2,
+                                                     const double* __restrict__ v,
+                                                     const double* __restrict__ D, int m,
+                                                     double* __restrict__ out) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int e = cptr[i] + lane; e < cptr[i + 1]; e += 64) {
+    double d = D[p1[e]];
+    if (p2[e] != p1[e]) d += D[p2[e]];
+    s += v[e] * d;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) out[i] = s;
+}
+
+// out[con] += dot(pool column, D_q) for every dense member of block q.
+__global__ __launch_bounds__(256) void k_adot_dense(const double* __restrict__ pool,
+                                                    const int* __restrict__ dlist_con, int n,
+                                                    const double* __restrict__ Dq,
+                                                    double* __restrict__ out) {
+  const size_t nn = (size_t)n * n;
+  const double* a = pool + (size_t)blockIdx.x * nn;
+  double s = 0.0;
+  for (size_t e = threadIdx.x; e < nn; e += 256) s += a[e] * Dq[e];
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out + dlist_con[blockIdx.x], red[0] + red[1] + red[2] + red[3]);
+}
+
+// out[pos] = sum_e val_e * y[con_e] over the terms of each distinct packed position.
+__global__ void k_yA_sparse(const long* __restrict__ ppos, const long* __restrict__ pptr,
+                            const int* __restrict__ pcon, const double* __restrict__ pval,
+                            long npos, const double* __restrict__ y, double* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npos) return;
+  double s = 0.0;
+  for (long e = pptr[t]; e < pptr[t + 1]; ++e) s += pval[e] * y[pcon[e]];
+  out[ppos[t]] = s;
+}
+
+// Dq[p] += sum_d pool[p, d] * y[con_d]  (dense members of one block)
+__global__ void k_yA_dense(const double* __restrict__ pool, const int* __restrict__ dlist_con,
+                           int nd, long nn, const double* __restrict__ y, double* __restrict__ Dq) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nn) return;
+  double s = 0.0;
+  for (int d = 0; d < nd; ++d) s += pool[p + nn * d] * y[dlist_con[d]];
+  Dq[p] += s;
+}
+
+// generic elementwise: out = a*A + b*B (+ c*Cv)
+__global__ void k_axpbypcz(long len, double a, const double* __restrict__ A, double b,
+                           const double* __restrict__ B, double cc, const double* __restrict__ Cv,
+                           double* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= len) return;
+  double r = a * A[t];
+  if (B) r += B[t] * b;
+  if (Cv) r += Cv[t] * cc;
+  out[t] = r;
+}
+
+// Lal::getSymmetrize per SDP block: a_ij = a_ji = (a_ij + a_ji) * 0.5
+__global__ void k_symmetrize(double* __restrict__ X, const long* __restrict__ off,
+                             const int* __restrict__ nn, int nblk) {
+  const int q = blockIdx.y;
+  if (q >= nblk) return;
+  const int n = nn[q];
+  double* a = X + off[q];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n * n) return;
+  const int i = t % n, j = t / n;
+  if (i <= j) return;
+  const double s = (a[i + (size_t)n * j] + a[j + (size_t)n * i]) * 0.5;
+  a[i + (size_t)n * j] = s;
+  a[j + (size_t)n * i] = s;
+}
+
+// partial sums of x.y (op 0), max|x| (op 1), min x*y*y (op 2, LP step check)
+__global__ __launch_bounds__(256) void k_reduce_partial(const double* __restrict__ x,
+                                                        const double* __restrict__ y, long len,
+                                                        int op, double* __restrict__ part) {
+  const long base = (long)blockIdx.x * kRedChunk;
+  double s = (op == 2) ? 1.0e50 : 0.0;
+  for (long t = base + threadIdx.x; t < base + kRedChunk && t < len; t += 256) {
+    if (op == 0) s += x[t] * y[t];
+    else if (op == 1) s = fmax(s, fabs(x[t]));
+    else s = fmin(s, x[t] * y[t] * y[t]);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double o = red[threadIdx.x + w];
+      if (op == 0) red[threadIdx.x] += o;
+      else if (op == 1) red[threadIdx.x] = fmax(red[threadIdx.x], o);
+      else red[threadIdx.x] = fmin(red[threadIdx.x], o);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_reduce_final(const double* __restrict__ part, int nparts, int op,
+                               double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double s = (op == 2) ? 1.0e50 : 0.0;
+  for (int i = 0; i < nparts; ++i) {
+    if (op == 0) s += part[i];
+    else if (op == 1) s = fmax(s, part[i]);
+    else s = fmin(s, part[i]);
+  }
+  *out = s;
+}
+
+// sum of the diagonals of the SDP blocks plus the LP entries
+__global__ void k_trace(const double* __restrict__ X, const long* __restrict__ off,
+                        const int* __restrict__ nn, int nblk, long lpOff, int nLP,
+                        double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int q = 0; q < nblk; ++q) {
+    const int n = nn[q];
+    for (int k = threadIdx.x; k < n; k += 256) s += X[off[q] + (long)k * n + k];
+  }
+  for (int t = threadIdx.x; t < nLP; t += 256) s += X[lpOff + t];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// LP part of computeInverse: 1/sqrt(x), 1/sqrt(z), 1/z; flag on a negative value
+__global__ void k_lp_inverse(const double* __restrict__ X, const double* __restrict__ Z,
+                             double* __restrict__ icx, double* __restrict__ icz,
+                             double* __restrict__ zi, long lpOff, int nLP, int* __restrict__ bad) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nLP) return;
+  const double x = X[lpOff + t], z = Z[lpOff + t];
+  if (x < 0.0 || z < 0.0) atomicOr(bad, 1);
+  icx[lpOff + t] = 1.0 / sqrt(x);
+  icz[lpOff + t] = 1.0 / sqrt(z);
+  zi[lpOff + t] = 1.0 / z;
+}
+
+// ---- host wrappers ---------------------------------------------------------
+
+static inline dim3 grid1(long len, int bs = 256) { return dim3((unsigned)((len + bs - 1) / bs)); }
+
+
+int a_dot(Ctx* c, const double* D, double* out) {
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_adot_sparse, dim3((c->m + 3) / 4), dim3(256), 0, s, c->d_cptr, c->d_cp1,
+                     c->d_cp2, c->d_cv, D, c->m, out);
+  for (int q = 0; q < c->nSDP; ++q) {
+    BlockTable& T = c->blk[q];
+    if (T.ndense == 0) continue;
+    hipLaunchKernelGGL(k_adot_dense, dim3(T.ndense), dim3(256), 0, s, T.d_dense, T.d_dcon, T.n,
+                       D + T.off, out);
+  }
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int sum_yA(Ctx* c, const double* y, double* out) {
+  hipStream_t s = c->stream;
+  SDPA_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(double) * c->total, s));
+  if (c->npos > 0)
+    hipLaunchKernelGGL(k_yA_sparse, grid1(c->npos), dim3(256), 0, s, c->d_ppos, c->d_pptr,
+                       c->d_pcon, c->d_pval, c->npos, y, out);
+  for (int q = 0; q < c->nSDP; ++q) {
+    BlockTable& T = c->blk[q];
+    if (T.ndense == 0) continue;
+    long nn = (long)T.n * T.n;
+    hipLaunchKernelGGL(k_yA_dense, grid1(nn), dim3(256), 0, s, T.d_dense, T.d_dcon, T.ndense,
+                       nn, y, out + T.off);
+  }
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+static int axpbypcz(Ctx* c, long len, double a, const double* A, double b, const double* B,
+                    double cc, const double* Cv, double* out) {
+  if (len <= 0) return 0;
+  hipLaunchKernelGGL(k_axpbypcz, grid1(len), dim3(256), 0, c->stream, len, a, A, b, B, cc, Cv,
+                     out);
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int triple_product(Ctx* c, const double* A, const double* B, const double* Cm, double* R) {
+  // R = A*B*C per SDP block (two batched GEMMs through d_w3), LP entries multiply
+  const int nb = c->nSDP;
+  if (nb > 0) {
+    std::vector<GemmItem> it(2 * nb);
+    int maxn = 0;
+    for (int q = 0; q < nb; ++q) {
+      const int n = c->sdpN[q];
+      const long o = c->sdpOff[q];
+      it[q] = GemmItem{A + o, B + o, c->d_w3 + o, n, n, n, n, n, n};
+      it[nb + q] = GemmItem{c->d_w3 + o, Cm + o, R + o, n, n, n, n, n, n};
+      maxn = n > maxn ? n : maxn;
+    }
+    if (int e = c->dw.ensure_items(2 * nb)) return e;
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * 2 * nb,
+                                  hipMemcpyHostToDevice, c->stream));
+    if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0)) return e;
+    if (int e = gemm_batched(c->stream, c->dw.d_items + nb, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
+      return e;
+  }
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+__global__ void k_mul3(long len, const double* __restrict__ A, const double* __restrict__ B,
+                       const double* __restrict__ C, double* __restrict__ R) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < len) R[t] = A[t] * B[t] * C[t];
+}
+
+static int triple(Ctx* c, const double* A, const double* B, const double* Cm, double* R) {
+  if (int e = triple_product(c, A, B, Cm, R)) return e;
+  if (c->nLP > 0) {
+    const long o = c->lpOff;
+    hipLaunchKernelGGL(k_mul3, grid1(c->nLP), dim3(256), 0, c->stream, (long)c->nLP, A + o, B + o,
+                       Cm + o, R + o);
+  }
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int residuals(Ctx* c, double* normP, double* normD) {
+  // p = b - A(X) ; D = C - sum y A - Z
+  if (int e = a_dot(c, c->d_X, c->d_pvec)) return e;
+  if (int e = axpbypcz(c, c->m, -1.0, c->d_pvec, 1.0, c->d_b, 0.0, nullptr, c->d_pvec)) return e;
+  if (int e = sum_yA(c, c->d_y, c->d_dmat)) return e;
+  if (int e = axpbypcz(c, c->total, -1.0, c->d_dmat, 1.0, c->d_C, -1.0, c->d_Z, c->d_dmat))
+    return e;
+  // max norms
+  const int codes_dummy = 0;
+  (void)codes_dummy;
+  double* part = c->d_red;
+  int np1 = (int)((c->m + kRedChunk - 1) / kRedChunk);
+  hipLaunchKernelGGL(k_reduce_partial, dim3(np1), dim3(256), 0, c->stream, c->d_pvec,
+                     (const double*)nullptr, (long)c->m, 1, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, part, np1, 1,
+                     part + 2 * kMaxChunks);
+  int np2 = (int)((c->total + kRedChunk - 1) / kRedChunk);
+  hipLaunchKernelGGL(k_reduce_partial, dim3(np2), dim3(256), 0, c->stream, c->d_dmat,
+                     (const double*)nullptr, c->total, 1, part + kMaxChunks);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, part + kMaxChunks, np2, 1,
+                     part + 2 * kMaxChunks + 1);
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, part + 2 * kMaxChunks, 2 * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  *normP = c->h_red[0];
+  *normD = c->h_red[1];
+  return 0;
+}
+
+int compute_dots(Ctx* c, int nq, const int* codes, double* out) {
+  hipStream_t s = c->stream;
+  double* part = c->d_red;
+  double* res = c->d_red + 2 * kMaxChunks;
+  for (int k = 0; k < nq; ++k) {
+    const double *x = nullptr, *y = nullptr;
+    long len = c->total;
+    switch (codes[k]) {
+      case DOT_CX: x = c->d_C; y = c->d_X; break;
+      case DOT_BY: x = c->d_b; y = c->d_y; len = c->m; break;
+      case DOT_CDX: x = c->d_C; y = c->d_Dx; break;
+      case DOT_BDY: x = c->d_b; y = c->d_dy; len = c->m; break;
+      case DOT_XDZ: x = c->d_X; y = c->d_Dz; break;
+      case DOT_DXZ: x = c->d_Dx; y = c->d_Z; break;
+      case DOT_DXDZ: x = c->d_Dx; y = c->d_Dz; break;
+      case DOT_XZ: x = c->d_X; y = c->d_Z; break;
+      case DOT_X0Z: x = c->d_X0; y = c->d_Z; break;
+      case DOT_XZ0: x = c->d_X; y = c->d_Z0; break;
+      case DOT_TRX:
+      case DOT_TRZ: {
+        hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, codes[k] == DOT_TRX ? c->d_X : c->d_Z,
+                           c->d_sdpOff, c->d_sdpN, c->nSDP, c->lpOff, c->nLP, res + k);
+        continue;
+      }
+      default:
+        return -2;
+    }
+    if (!x || !y) return -3;
+    int np = (int)((len + kRedChunk - 1) / kRedChunk);
+    if (np > kMaxChunks) return -4;
+    hipLaunchKernelGGL(k_reduce_partial, dim3(np > 0 ? np : 1), dim3(256), 0, s, x, y, len, 0,
+                       part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, part, np, 0, res + k);
+  }
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, res, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+  SDPA_HIP_CHECK(hipStreamSynchronize(s));
+  for (int k = 0; k < nq; ++k) out[k] = c->h_red[k];
+  return 0;
+}
+
+int make_gvec(Ctx* c, int corrector, double target, int infeasible) {
+  // r_zinv = target * Zinv - X  (- Dx Dz Zinv in the corrector)
+  if (int e = axpbypcz(c, c->total, target, c->d_Zinv, -1.0, c->d_X, 0.0, nullptr, c->d_rzinv))
+    return e;
+  if (corrector) {
+    if (int e = triple(c, c->d_Dx, c->d_Dz, c->d_Zinv, c->d_w1)) return e;
+    if (int e = axpbypcz(c, c->total, 1.0, c->d_rzinv, -1.0, c->d_w1, 0.0, nullptr, c->d_rzinv))
+      return e;
+  }
+  const double* D = c->d_rzinv;
+  if (infeasible) {
+    if (!corrector)
+      if (int e = triple(c, c->d_X, c->d_dmat, c->d_Zinv, c->d_xrdzinv)) return e;
+    if (int e = axpbypcz(c, c->total, 1.0, c->d_rzinv, -1.0, c->d_xrdzinv, 0.0, nullptr, c->d_w1))
+      return e;
+    D = c->d_w1;
+  }
+  // g = -A(D) + p
+  if (int e = a_dot(c, D, c->d_g)) return e;
+  return axpbypcz(c, c->m, -1.0, c->d_g, 1.0, c->d_pvec, 0.0, nullptr, c->d_g);
+}
+
+int make_direction(Ctx* c, int infeasible) {
+  // Dz = -sum dy A (+ D)
+  if (int e = sum_yA(c, c->d_dy, c->d_Dz)) return e;
+  if (int e = axpbypcz(c, c->total, -1.0, c->d_Dz, infeasible ? 1.0 : 0.0,
+                       infeasible ? c->d_dmat : nullptr, 0.0, nullptr, c->d_Dz))
+    return e;
+  // Dx = sym(r_zinv - X Dz Zinv)
+  if (int e = triple(c, c->d_X, c->d_Dz, c->d_Zinv, c->d_w1)) return e;
+  if (int e = axpbypcz(c, c->total, 1.0, c->d_rzinv, -1.0, c->d_w1, 0.0, nullptr, c->d_Dx))
+    return e;
+  if (c->nSDP > 0) {
+    int maxn = 0;
+    for (int q = 0; q < c->nSDP; ++q) maxn = c->sdpN[q] > maxn ? c->sdpN[q] : maxn;
+    dim3 g((unsigned)(((long)maxn * maxn + 255) / 256), c->nSDP);
+    hipLaunchKernelGGL(k_symmetrize, g, dim3(256), 0, c->stream, c->d_Dx, c->d_sdpOff, c->d_sdpN,
+                       c->nSDP);
+  }
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int update_point(Ctx* c, double alphaP, double alphaD) {
+  if (int e = axpbypcz(c, c->total, 1.0, c->d_X, alphaP, c->d_Dx, 0.0, nullptr, c->d_X)) return e;
+  if (int e = axpbypcz(c, c->m, 1.0, c->d_y, alphaD, c->d_dy, 0.0, nullptr, c->d_y)) return e;
+  return axpbypcz(c, c->total, 1.0, c->d_Z, alphaD, c->d_Dz, 0.0, nullptr, c->d_Z);
+}
+
+int inverse_chol(Ctx* c) {
+  // L = chol(X), invCholX = L^-1 ; L = chol(Z), invCholZ = L^-1, Zinv = L^-T L^-1
+  const int nb = c->nSDP;
+  int fail = 0;
+  if (nb > 0) {
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->d_LX, c->d_X, sizeof(double) * c->lpOff,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->d_LZ, c->d_Z, sizeof(double) * c->lpOff,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    std::vector<MatDesc> L(2 * nb), W(2 * nb);
+    for (int q = 0; q < nb; ++q) {
+      const int n = c->sdpN[q];
+      const long o = c->sdpOff[q];
+      L[q] = MatDesc{c->d_LX + o, n, n};
+      L[nb + q] = MatDesc{c->d_LZ + o, n, n};
+      W[q] = MatDesc{c->d_invCholX + o, n, n};
+      W[nb + q] = MatDesc{c->d_invCholZ + o, n, n};
+    }
+    if (int e = potrf_batched(c->stream, &c->dw, L.data(), 2 * nb, 0, c->d_info)) return e;
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, c->d_info, sizeof(int) * 2 * nb,
+                                  hipMemcpyDeviceToHost, c->stream));
+    SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < 2 * nb; ++b)
+      if (c->h_info[b] != 0) fail = 1;
+    if (fail) return 1;
+    if (int e = trtri_batched(c->stream, &c->dw, L.data(), W.data(), 2 * nb)) return e;
+    // Zinv = invCholZ^T * invCholZ
+    std::vector<GemmItem> it(nb);
+    int maxn = 0;
+    for (int q = 0; q < nb; ++q) {
+      const int n = c->sdpN[q];
+      const long o = c->sdpOff[q];
+      it[q] = GemmItem{c->d_invCholZ + o, c->d_invCholZ + o, c->d_Zinv + o, n, n, n, n, n, n};
+      maxn = n > maxn ? n : maxn;
+    }
+    if (int e = c->dw.ensure_items(nb)) return e;
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * nb,
+                                  hipMemcpyHostToDevice, c->stream));
+    if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 1, 0, 1.0, 0.0, 0)) return e;
+  }
+  if (c->nLP > 0) {
+    int* bad = c->d_info + 2 * nb;
+    SDPA_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_lp_inverse, grid1(c->nLP), dim3(256), 0, c->stream, c->d_X, c->d_Z,
+                       c->d_invCholX, c->d_invCholZ, c->d_Zinv, c->lpOff, c->nLP, bad);
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (c->h_info[0]) return 1;
+  }
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int lp_min_step(Ctx* c, const double* D, const double* L, double* out) {
+  // min over LP coordinates of D * L * L (Jal::getMinEigen LP part, sdpa_jordan.cpp:101-106)
+  if (c->nLP <= 0) { *out = 1.0e50; return 0; }
+  double* part = c->d_red;
+  int np = (c->nLP + kRedChunk - 1) / kRedChunk;
+  hipLaunchKernelGGL(k_reduce_partial, dim3(np), dim3(256), 0, c->stream, D + c->lpOff,
+                     L + c->lpOff, (long)c->nLP, 2, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, part, np, 2,
+                     part + 2 * kMaxChunks);
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, part + 2 * kMaxChunks, sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  *out = c->h_red[0];
+  return 0;
+}
+
+}  // namespace sdpa_amd

@@ -1,0 +1,313 @@
+// schur.hip -- Schur complement matrix bMat on the device
+// (Newton::Make_bMat dense path, sdpa_newton.cpp:2634-2670).
+//
+//   bMat[i,j] = sum over SDP blocks of A_i . (X A_j Z^-1)  +  LP term,
+//
+// written into the lower triangle (i >= j) of the m x m column-major buffer;
+// the Cholesky and the solves read only that triangle.  Every unordered pair
+// is computed once, by its owner j (more NonZeroEffect; ties go to the smaller
+// index, sdpa_newton.cpp:2079-2081) with the owner's formula:
+//   F3 (sparse owner, calF3_thread_2, 1319-1403): gather over the two nonzero
+//      lists.  Light pairs (both lists <= kLight entries) are one thread each;
+//      pairs that involve a heavy list are one wave each with a wave reduction.
+//   F1/F2 (calF1/calF2, 885-936): G_j = X A_j Z^-1 by batched MFMA GEMMs, then
+//      A_i . G_j -- one GEMM (K = n^2) for dense partners, a gather for
+//      sparse ones.  F2 is evaluated through G_j as well (same value).
+//   LP (compute_bMat_dense_LP, 2562-2602): x_l z_l^-1 a_il a_jl.
+#include "ctx.h"
+
+namespace sdpa_amd {
+
+constexpr int kLight = 8;
+
+__device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int* __restrict__ ec,
+                                          const double* __restrict__ ev, int so, int co, int sp,
+                                          int cp, int n, const double* __restrict__ X,
+                                          const double* __restrict__ Zi, int e1_begin,
+                                          int e1_step) {
+  double ret = 0.0;
+  for (int e1 = e1_begin; e1 < co; e1 += e1_step) {
+    const int a = er[so + e1], b = ec[so + e1];
+    const double v1 = ev[so + e1];
+    const double* Xa = X + (size_t)n * a;
+    const double* Xb = X + (size_t)n * b;
+    const double* Za = Zi + (size_t)n * a;
+    const double* Zb = Zi + (size_t)n * b;
+    double sum = 0.0;
+    if (a != b) {
+      for (int e2 = 0; e2 < cp; ++e2) {
+        const int g = er[sp + e2], d = ec[sp + e2];
+        double plu;
+        if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d] + Za[d] * Xb[g] + Za[g] * Xb[d];
+        else plu = Zb[d] * Xa[g] + Za[d] * Xb[g];
+        sum += ev[sp + e2] * plu;
+      }
+    } else {
+      for (int e2 = 0; e2 < cp; ++e2) {
+        const int g = er[sp + e2], d = ec[sp + e2];
+        double plu;
+        if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d];
+        else plu = Zb[d] * Xa[g];
+        sum += ev[sp + e2] * plu;
+      }
+    }
+    ret += v1 * sum;
+  }
+  return ret;
+}
+
+// One thread per pair (ki >= kj) of the block's constraint list.
+// grid = (ceil(ncon/64), ceil(ncon/4)), block = 256 (64 rows x 4 columns).
+__global__ __launch_bounds__(256) void k_f3_light(
+    const int* __restrict__ con, const int* __restrict__ estart, const int* __restrict__ ecnt,
+    const int* __restrict__ nzeff, const int* __restrict__ form, const int* __restrict__ er,
+    const int* __restrict__ ec, const double* __restrict__ ev, int ncon, int n,
+    const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ B, int m,
+    int store) {
+  if ((int)(blockIdx.x * 64 + 63) < (int)(blockIdx.y * 4)) return;
+  const int ki = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int kj = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (ki >= ncon || kj >= ncon || ki < kj) return;
+  const int nzi = nzeff[ki], nzj = nzeff[kj];
+  const int o = (nzi > nzj) ? ki : kj;  // tie -> smaller index kj
+  const int p = (o == ki) ? kj : ki;
+  const int co = ecnt[o], cp = ecnt[p];
+  double val = 0.0;
+  bool mine = (form[o] == 3) && co <= kLight && cp <= kLight;
+  if (mine) val = f3_pair(er, ec, ev, estart[o], co, estart[p], cp, n, X, Zi, 0, 1);
+  double* dst = B + (size_t)con[ki] + (size_t)m * con[kj];
+  if (store) *dst = val;
+  else if (mine) *dst += val;
+}
+
+// One wave per pair that involves a heavy list.  grid = (ceil(ncon/4), nheavy).
+__global__ __launch_bounds__(256) void k_f3_heavy(
+    const int* __restrict__ con, const int* __restrict__ estart, const int* __restrict__ ecnt,
+    const int* __restrict__ nzeff, const int* __restrict__ form, const int* __restrict__ heavy,
+    const int* __restrict__ er, const int* __restrict__ ec, const double* __restrict__ ev, int ncon,
+    int n, const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ B,
+    int m) {
+  const int h = heavy[blockIdx.y];
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= ncon) return;
+  if (ecnt[k] > kLight && k > h) return;  // heavy-heavy pairs once, from the larger index
+  const int ki = k > h ? k : h, kj = k > h ? h : k;
+  const int nzi = nzeff[ki], nzj = nzeff[kj];
+  const int o = (nzi > nzj) ? ki : kj;
+  const int p = (o == ki) ? kj : ki;
+  if (form[o] != 3) return;
+  double v = f3_pair(er, ec, ev, estart[o], ecnt[o], estart[p], ecnt[p], n, X, Zi, lane, 64);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0) B[(size_t)con[ki] + (size_t)m * con[kj]] += v;
+}
+
+// ---- F1/F2 ----------------------------------------------------------------
+// Dense copy of the owners' A_j (sparse owners are expanded, dense ones copied
+// from the pool) for one chunk.  grid = (chunk), block = 256.
+__global__ __launch_bounds__(256) void k_densify(const int* __restrict__ own, int first,
+                                                 const int* __restrict__ estart,
+                                                 const int* __restrict__ ecnt,
+                                                 const int* __restrict__ dpos,
+                                                 const int* __restrict__ er,
+                                                 const int* __restrict__ ec,
+                                                 const double* __restrict__ ev,
+                                                 const double* __restrict__ pool, int n,
+                                                 double* __restrict__ out) {
+  const int j = own[first + blockIdx.x];
+  double* A = out + (size_t)blockIdx.x * n * n;
+  const size_t nn = (size_t)n * n;
+  if (dpos[j] >= 0) {
+    const double* src = pool + (size_t)dpos[j] * nn;
+    for (size_t e = threadIdx.x; e < nn; e += 256) A[e] = src[e];
+    return;
+  }
+  for (size_t e = threadIdx.x; e < nn; e += 256) A[e] = 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int e = 0; e < ecnt[j]; ++e) {
+      int r = er[estart[j] + e], c = ec[estart[j] + e];
+      A[r + (size_t)n * c] += ev[estart[j] + e];
+      if (r != c) A[c + (size_t)n * r] += ev[estart[j] + e];
+    }
+  }
+}
+
+__device__ __forceinline__ bool owns(int jloc, int iloc, const int* nzeff) {
+  const int jnz = nzeff[jloc], inz = nzeff[iloc];
+  return !((jnz < inz) || (inz == jnz && iloc < jloc));
+}
+
+// Sparse partners of F1 owners: one wave per (owner in chunk, partner).
+// grid = (ceil(ncon/4), chunk).
+__global__ __launch_bounds__(256) void k_f1_sparse(
+    const int* __restrict__ own, int first, const int* __restrict__ con,
+    const int* __restrict__ estart, const int* __restrict__ ecnt, const int* __restrict__ nzeff,
+    const int* __restrict__ dpos, const int* __restrict__ er, const int* __restrict__ ec,
+    const double* __restrict__ ev, int ncon, int n, const double* __restrict__ G,
+    double* __restrict__ B, int m) {
+  const int jc = blockIdx.y;
+  const int j = own[first + jc];
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= ncon || dpos[i] >= 0 || !owns(j, i, nzeff)) return;
+  const double* Gj = G + (size_t)jc * n * n;
+  double v = 0.0;
+  for (int e = lane; e < ecnt[i]; e += 64) {
+    int r = er[estart[i] + e], c = ec[estart[i] + e];
+    double g = (r == c) ? Gj[r + (size_t)n * c] : Gj[r + (size_t)n * c] + Gj[c + (size_t)n * r];
+    v += ev[estart[i] + e] * g;
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0) {
+    int gi = con[i], gj = con[j];
+    int a = gi > gj ? gi : gj, b = gi > gj ? gj : gi;
+    B[(size_t)a + (size_t)m * b] += v;
+  }
+}
+
+// Dense partners: P = Abar^T Gbar (ndense x chunk) was formed by a GEMM;
+// add the owned entries.  grid = (ceil(ndense/256), chunk).
+__global__ __launch_bounds__(256) void k_f1_scatter(const int* __restrict__ own, int first,
+                                                    const int* __restrict__ dlist,
+                                                    const int* __restrict__ con,
+                                                    const int* __restrict__ nzeff, int ndense,
+                                                    const double* __restrict__ P,
+                                                    double* __restrict__ B, int m) {
+  const int jc = blockIdx.y;
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= ndense) return;
+  const int j = own[first + jc];
+  const int i = dlist[id];
+  if (!owns(j, i, nzeff)) return;
+  int gi = con[i], gj = con[j];
+  int a = gi > gj ? gi : gj, b = gi > gj ? gj : gi;
+  B[(size_t)a + (size_t)m * b] += P[id + (size_t)ndense * jc];
+}
+
+// ---- LP ---------------------------------------------------------------------
+// One thread per LP coordinate; pairs k1 <= k2 of its constraint list.
+__global__ void k_lp_bmat(const int* __restrict__ ptr, const int* __restrict__ lcon,
+                          const double* __restrict__ lval, int nLP, const double* __restrict__ X,
+                          const double* __restrict__ Zi, long lpOff, double* __restrict__ B,
+                          int m) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nLP) return;
+  const double xz = X[lpOff + t] * Zi[lpOff + t];
+  for (int k1 = ptr[t]; k1 < ptr[t + 1]; ++k1) {
+    const int j = lcon[k1];
+    const double aj = lval[k1];
+    for (int k2 = k1; k2 < ptr[t + 1]; ++k2) {
+      const int i = lcon[k2];
+      const double v = xz * lval[k2] * aj;
+      const int a = i > j ? i : j, b = i > j ? j : i;
+      atomicAdd(B + (size_t)a + (size_t)m * b, v);
+    }
+  }
+}
+
+struct F1Scratch;
+
+static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
+  const int n = T.n;
+  const size_t nn = (size_t)n * n;
+  // chunk so that densified A, X*A and G each fit in a 3 x 512 MiB budget
+  size_t budget = (size_t)512 << 20;
+  int chunk = (int)std::max<size_t>(1, budget / (nn * sizeof(double)));
+  if (chunk > T.nF1) chunk = T.nF1;
+  size_t need = 3 * (size_t)chunk * nn;
+  if (need > c->capG) {
+    if (c->d_G) SDPA_HIP_CHECK(hipFree(c->d_G));
+    SDPA_HIP_CHECK(hipMalloc(&c->d_G, need * sizeof(double)));
+    c->capG = need;
+  }
+  double* Ad = c->d_G;
+  double* XA = c->d_G + (size_t)chunk * nn;
+  double* G = c->d_G + 2 * (size_t)chunk * nn;
+  if (T.ndense > 0) {
+    size_t needP = (size_t)T.ndense * chunk;
+    if (needP > c->capF1C) {
+      if (c->d_F1C) SDPA_HIP_CHECK(hipFree(c->d_F1C));
+      SDPA_HIP_CHECK(hipMalloc(&c->d_F1C, needP * sizeof(double)));
+      c->capF1C = needP;
+    }
+  }
+  // dense-member list (local indices of dense constraints, in pool order)
+  static thread_local std::vector<int> dl;
+  int* d_dlist = nullptr;
+  if (T.ndense > 0) {
+    dl.assign(T.ndense, 0);
+    for (int k = 0; k < T.ncon; ++k)
+      if (T.h_dpos[k] >= 0) dl[T.h_dpos[k]] = k;
+    SDPA_HIP_CHECK(hipMallocAsync((void**)&d_dlist, sizeof(int) * T.ndense, c->stream));
+    SDPA_HIP_CHECK(hipMemcpyAsync(d_dlist, dl.data(), sizeof(int) * T.ndense,
+                                  hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<GemmItem> items(2 * chunk);
+  if (int e = c->dw.ensure_items(2 * chunk)) return e;
+  for (int first = 0; first < T.nF1; first += chunk) {
+    const int cnt = std::min(chunk, T.nF1 - first);
+    hipLaunchKernelGGL(k_densify, dim3(cnt), dim3(256), 0, c->stream, T.d_f1own, first,
+                       T.d_estart, T.d_ecnt, T.d_dpos, c->d_er, c->d_ec, c->d_ev, T.d_dense, n, Ad);
+    for (int t = 0; t < cnt; ++t) {
+      items[t] = GemmItem{X, Ad + t * nn, XA + t * nn, n, n, n, n, n, n};
+      items[cnt + t] = GemmItem{XA + t * nn, Zi, G + t * nn, n, n, n, n, n, n};
+    }
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, items.data(), sizeof(GemmItem) * 2 * cnt,
+                                  hipMemcpyHostToDevice, c->stream));
+    if (int e = gemm_batched(c->stream, c->dw.d_items, cnt, n, n, 0, 0, 1.0, 0.0, 0)) return e;
+    if (int e = gemm_batched(c->stream, c->dw.d_items + cnt, cnt, n, n, 0, 0, 1.0, 0.0, 0))
+      return e;
+    if (T.ndense > 0) {
+      // P = Abar^T * Gbar : (ndense x n^2) * (n^2 x cnt)
+      if (int e = gemm(c->stream, &c->dw, T.d_dense, (int)nn, G, (int)nn, c->d_F1C, T.ndense,
+                       T.ndense, cnt, (int)nn, 1, 0, 1.0, 0.0, 0))
+        return e;
+      hipLaunchKernelGGL(k_f1_scatter, dim3((T.ndense + 255) / 256, cnt), dim3(256), 0, c->stream,
+                         T.d_f1own, first, d_dlist, T.d_con, T.d_nzeff, T.ndense, c->d_F1C,
+                         c->d_B, c->m);
+    }
+    if (T.ndense < T.ncon)
+      hipLaunchKernelGGL(k_f1_sparse, dim3((T.ncon + 3) / 4, cnt), dim3(256), 0, c->stream,
+                         T.d_f1own, first, T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff, T.d_dpos,
+                         c->d_er, c->d_ec, c->d_ev, T.ncon, n, G, c->d_B, c->m);
+    // the next chunk rewrites the descriptor slots and scratch: stream order keeps it safe
+  }
+  if (d_dlist) SDPA_HIP_CHECK(hipFreeAsync(d_dlist, c->stream));
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int make_bmat(Ctx* c) {
+  hipStream_t s = c->stream;
+  const int m = c->m;
+  if (!c->bmatStoreFirst)
+    SDPA_HIP_CHECK(hipMemsetAsync(c->d_B, 0, sizeof(double) * (size_t)m * m, s));
+  for (int q = 0; q < c->nSDP; ++q) {
+    BlockTable& T = c->blk[q];
+    if (T.ncon == 0) continue;
+    const double* X = c->d_X + T.off;
+    const double* Zi = c->d_Zinv + T.off;
+    if (T.nF1 > 0)
+      if (int e = f1_block(c, T, X, Zi)) return e;
+    if (T.nF3 > 0) {
+      dim3 g((T.ncon + 63) / 64, (T.ncon + 3) / 4);
+      hipLaunchKernelGGL(k_f3_light, g, dim3(256), 0, s, T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff,
+                         T.d_form, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n, X, Zi, c->d_B, m,
+                         c->bmatStoreFirst ? 1 : 0);
+      if (T.nheavy > 0) {
+        dim3 gh((T.ncon + 3) / 4, T.nheavy);
+        hipLaunchKernelGGL(k_f3_heavy, gh, dim3(256), 0, s, T.d_con, T.d_estart, T.d_ecnt,
+                           T.d_nzeff, T.d_form, T.d_heavy, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n,
+                           X, Zi, c->d_B, m);
+      }
+    }
+  }
+  if (c->nLP > 0)
+    hipLaunchKernelGGL(k_lp_bmat, dim3((c->nLP + 255) / 256), dim3(256), 0, s, c->d_lpptr,
+                       c->d_lpcon, c->d_lpval, c->nLP, c->d_X, c->d_Zinv, c->lpOff, c->d_B, m);
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace sdpa_amd

@@ -1,0 +1,648 @@
+// sdpa_gpu.hip -- implementation of the C ABI in include/sdpa_gpu.h:
+// context lifetime, problem upload (with SDPA's structural preprocessing),
+// and the thin wrappers around the kernels in dense/schur/ops/eig.hip.
+#include "../../include/sdpa_gpu.h"
+#include "ctx.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+using namespace sdpa_amd;
+
+struct sdpa_gpu_ctx {
+  Ctx c;
+};
+
+namespace {
+
+constexpr int kLightHost = 8;  // must match schur.hip's kLight
+
+int fail(Ctx* c, int code, const char* msg) {
+  c->last_error = msg;
+  return code;
+}
+
+template <class T>
+int upload(Ctx* c, T** dst, const std::vector<T>& src) {
+  if (*dst) SDPA_HIP_CHECK(hipFree(*dst));
+  *dst = nullptr;
+  size_t bytes = sizeof(T) * (src.empty() ? 1 : src.size());
+  SDPA_HIP_CHECK(hipMalloc((void**)dst, bytes));
+  if (!src.empty())
+    SDPA_HIP_CHECK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int dalloc(double** p, size_t n) {
+  if (*p) SDPA_HIP_CHECK(hipFree(*p));
+  *p = nullptr;
+  SDPA_HIP_CHECK(hipMalloc((void**)p, sizeof(double) * (n ? n : 1)));
+  SDPA_HIP_CHECK(hipMemset(*p, 0, sizeof(double) * (n ? n : 1)));
+  return 0;
+}
+
+void free_problem(Ctx* c) {
+  for (auto& T : c->blk) {
+    for (void* p : {(void*)T.d_con, (void*)T.d_estart, (void*)T.d_ecnt, (void*)T.d_nzeff,
+                    (void*)T.d_form, (void*)T.d_dpos, (void*)T.d_heavy, (void*)T.d_f1own,
+                    (void*)T.d_dcon, (void*)T.d_dense})
+      if (p) (void)hipFree(p);
+  }
+  c->blk.clear();
+  void** ptrs[] = {(void**)&c->d_er, (void**)&c->d_ec, (void**)&c->d_ev, (void**)&c->d_lpptr,
+                   (void**)&c->d_lpcon, (void**)&c->d_lpval, (void**)&c->d_cptr, (void**)&c->d_cp1,
+                   (void**)&c->d_cp2, (void**)&c->d_cv, (void**)&c->d_ppos, (void**)&c->d_pptr,
+                   (void**)&c->d_pcon, (void**)&c->d_pval, (void**)&c->d_b, (void**)&c->d_C,
+                   (void**)&c->d_X, (void**)&c->d_Z, (void**)&c->d_y, (void**)&c->d_invCholX,
+                   (void**)&c->d_invCholZ, (void**)&c->d_Zinv, (void**)&c->d_LX, (void**)&c->d_LZ,
+                   (void**)&c->d_Dx, (void**)&c->d_Dz, (void**)&c->d_dy, (void**)&c->d_rzinv,
+                   (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
+                   (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
+                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_G, (void**)&c->d_F1C,
+                   (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
+                   (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
+  for (void** p : ptrs) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  c->capG = c->capF1C = 0;
+  c->loaded = false;
+}
+
+struct Ent {
+  int r, c;
+  double v;
+};
+
+}  // namespace
+
+extern "C" {
+
+int sdpa_gpu_create(int device, sdpa_gpu_ctx** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return -2;
+  if (device < 0 || device >= ndev) return -3;
+  SDPA_HIP_CHECK(hipSetDevice(device));
+  sdpa_gpu_ctx* h = new sdpa_gpu_ctx();
+  Ctx* c = &h->c;
+  c->device = device;
+  SDPA_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) SDPA_HIP_CHECK(hipEventCreate(&e));
+  SDPA_HIP_CHECK(hipHostMalloc((void**)&c->h_red, sizeof(double) * 64, hipHostMallocDefault));
+  SDPA_HIP_CHECK(hipHostMalloc((void**)&c->h_info, sizeof(int) * 64, hipHostMallocDefault));
+  *out = h;
+  return 0;
+}
+
+int sdpa_gpu_destroy(sdpa_gpu_ctx* h) {
+  if (!h) return 0;
+  Ctx* c = &h->c;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_problem(c);
+  c->dw.release();
+  for (auto& e : c->ev) (void)hipEventDestroy(e);
+  if (c->h_red) (void)hipHostFree(c->h_red);
+  if (c->h_info) (void)hipHostFree(c->h_info);
+  (void)hipStreamDestroy(c->stream);
+  delete h;
+  return 0;
+}
+
+const char* sdpa_gpu_last_error(sdpa_gpu_ctx* h) { return h ? h->c.last_error.c_str() : "no context"; }
+
+int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockStruct,
+                          const double* cvec, long nnz, const int* K, const int* L, const int* I,
+                          const int* J, const double* V, double kappa) {
+  if (!h) return -1;
+  Ctx* c = &h->c;
+  SDPA_HIP_CHECK(hipSetDevice(c->device));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  free_problem(c);
+  if (m <= 0 || nBlock <= 0) return fail(c, -10, "m and nBlock must be positive");
+  c->m = m;
+  c->nBlock = nBlock;
+  // BlockStruct::makeInternalStructure (sdpa_block.cpp:66-108)
+  c->blockStruct.assign(blockStruct, blockStruct + nBlock);
+  c->blockType.assign(nBlock, 0);
+  c->blockNumber.assign(nBlock, 0);
+  c->nSDP = 0;
+  c->nLP = 0;
+  c->sdpN.clear();
+  for (int l = 0; l < nBlock; ++l) {
+    int s = blockStruct[l];
+    if (s >= 2) {
+      c->blockType[l] = 1;
+      c->blockNumber[l] = c->nSDP++;
+      c->sdpN.push_back(s);
+    } else if (s < 0 || s == 1) {
+      c->blockType[l] = 2;
+      c->blockNumber[l] = c->nLP;
+      c->nLP += (s < 0 ? -s : 1);
+    } else {
+      return fail(c, -11, "block size 0");
+    }
+  }
+  c->sdpOff.assign(c->nSDP, 0);
+  long off = 0;
+  int nd = 0;
+  for (int q = 0; q < c->nSDP; ++q) {
+    c->sdpOff[q] = off;
+    off += (long)c->sdpN[q] * c->sdpN[q];
+    nd += c->sdpN[q];
+  }
+  c->lpOff = off;
+  c->total = off + c->nLP;
+  c->nDim = nd + c->nLP;
+
+  // group the records: per (k, SDP block) entry lists in input order; C dense
+  const int nS = c->nSDP;
+  std::vector<double> hC(c->total, 0.0);
+  std::vector<std::vector<std::vector<Ent>>> ent(nS, std::vector<std::vector<Ent>>(m));
+  std::vector<std::vector<std::pair<int, double>>> lpent(m);  // (coordinate, value)
+  for (long e = 0; e < nnz; ++e) {
+    const int k = K[e], l = L[e] - 1;
+    if (k < 0 || k > m || l < 0 || l >= nBlock) return fail(c, -12, "record out of range");
+    if (c->blockType[l] == 1) {
+      const int q = c->blockNumber[l], n = c->sdpN[q];
+      const int i = I[e] - 1, j = J[e] - 1;
+      if (i < 0 || j < 0 || i >= n || j >= n) return fail(c, -13, "index out of range");
+      if (k == 0) {
+        double* Cq = hC.data() + c->sdpOff[q];
+        Cq[i + (size_t)n * j] += -V[e];
+        if (i != j) Cq[j + (size_t)n * i] += -V[e];
+      } else {
+        ent[q][k - 1].push_back(Ent{i, j, V[e]});
+      }
+    } else {
+      if (I[e] != J[e]) return fail(c, -14, "LP entry must be diagonal");
+      const int t = c->blockNumber[l] + I[e] - 1;
+      if (I[e] < 1 || t >= c->nLP) return fail(c, -13, "LP index out of range");
+      if (k == 0) hC[c->lpOff + t] += -V[e];
+      else {
+        auto& le = lpent[k - 1];
+        bool found = false;
+        for (auto& pr : le)
+          if (pr.first == t) { pr.second += V[e]; found = true; }
+        if (!found) le.push_back({t, V[e]});
+      }
+    }
+  }
+
+  // per-block tables: changeToDense, index, formula (sdpa_newton.cpp:515-695)
+  c->blk.assign(nS, BlockTable());
+  std::vector<int> h_er, h_ec;
+  std::vector<double> h_ev;
+  c->nF[0] = c->nF[1] = c->nF[2] = 0;
+  std::vector<int> maxNCon(1, 0);
+  for (int q = 0; q < nS; ++q) {
+    BlockTable& T = c->blk[q];
+    T.n = c->sdpN[q];
+    T.off = c->sdpOff[q];
+    const int n = T.n;
+    std::vector<int> con, ecnt, nzeff, isdense;
+    for (int k = 0; k < m; ++k) {
+      if (ent[q][k].empty()) continue;
+      con.push_back(k);
+      int cnt = (int)ent[q][k].size();
+      bool dense = (double)cnt >= (double)n * n * 0.20;  // sdpa_struct.cpp:518
+      int ne = 0;
+      for (auto& x : ent[q][k]) ne += (x.r == x.c) ? 1 : 2;
+      ecnt.push_back(cnt);
+      nzeff.push_back(dense ? n * n : ne);
+      isdense.push_back(dense ? 1 : 0);
+    }
+    T.ncon = (int)con.size();
+    const int nc = T.ncon;
+    // upNonZeroCount (lines 571-596)
+    std::vector<double> up(nc);
+    {
+      // sort-based count: up_j = jnz + sum_{inz < jnz} inz + sum_{inz == jnz, i > j} inz
+      std::vector<int> order(nc);
+      std::iota(order.begin(), order.end(), 0);
+      std::sort(order.begin(), order.end(), [&](int a, int b) {
+        return nzeff[a] != nzeff[b] ? nzeff[a] < nzeff[b] : a < b;
+      });
+      // prefix sums over groups of equal nzeff
+      double below = 0.0;
+      size_t g0 = 0;
+      while (g0 < order.size()) {
+        size_t g1 = g0;
+        while (g1 < order.size() && nzeff[order[g1]] == nzeff[order[g0]]) ++g1;
+        const double z = nzeff[order[g0]];
+        // within the group (sorted by index), members after position p have larger index
+        for (size_t p = g0; p < g1; ++p) {
+          const double larger = (double)(g1 - p - 1);
+          up[order[p]] = z + below + larger * z;
+        }
+        below += z * (double)(g1 - g0);
+        g0 = g1;
+      }
+    }
+    std::vector<int> form(nc), dpos(nc, -1), heavy, f1own, dcon;
+    std::vector<int> estart(nc);
+    for (int k = 0; k < nc; ++k) {
+      const double jnz = nzeff[k], nn = n, u = up[k];
+      const double f1 = kappa * nn * jnz + nn * nn * nn + kappa * u;
+      const double f2 = kappa * nn * jnz + kappa * (nn + 1) * u;
+      const double f3 = kappa * (2 * kappa * jnz + 1) * u;
+      int f;
+      if (isdense[k]) f = (f1 < f2) ? 1 : 2;
+      else if (f1 < f2 && f1 < f3) f = (nn <= 200 && 2.0 * nn >= jnz) ? 3 : 1;
+      else if (f2 < f3) f = 2;
+      else f = 3;
+      form[k] = f;
+      c->nF[f - 1]++;
+      if (f == 3) T.nF3++;
+      else { T.nF1++; f1own.push_back(k); }
+      estart[k] = (int)h_er.size();
+      for (auto& x : ent[q][con[k]]) {
+        h_er.push_back(x.r);
+        h_ec.push_back(x.c);
+        h_ev.push_back(x.v);
+      }
+      if (isdense[k]) { dpos[k] = T.ndense++; dcon.push_back(con[k]); }
+      if (!isdense[k] && ecnt[k] > kLightHost) heavy.push_back(k);
+    }
+    T.nheavy = (int)heavy.size();
+    if (int e = upload(c, &T.d_con, con)) return e;
+    if (int e = upload(c, &T.d_estart, estart)) return e;
+    if (int e = upload(c, &T.d_ecnt, ecnt)) return e;
+    if (int e = upload(c, &T.d_nzeff, nzeff)) return e;
+    if (int e = upload(c, &T.d_form, form)) return e;
+    if (int e = upload(c, &T.d_dpos, dpos)) return e;
+    if (int e = upload(c, &T.d_heavy, heavy)) return e;
+    if (int e = upload(c, &T.d_f1own, f1own)) return e;
+    if (int e = upload(c, &T.d_dcon, dcon)) return e;
+    T.h_con = con;
+    T.h_form = form;
+    T.h_dpos = dpos;
+    T.h_ecnt = ecnt;
+    T.h_estart = estart;
+    if (T.ndense > 0) {
+      const size_t nn2 = (size_t)n * n;
+      std::vector<double> pool(nn2 * T.ndense, 0.0);
+      for (int k = 0; k < nc; ++k) {
+        if (dpos[k] < 0) continue;
+        double* A = pool.data() + nn2 * dpos[k];
+        for (auto& x : ent[q][con[k]]) {
+          if (x.r == x.c) A[x.r + (size_t)n * x.c] = x.v;
+          else A[x.r + (size_t)n * x.c] = A[x.c + (size_t)n * x.r] = x.v;
+        }
+      }
+      if (int e = upload(c, &T.d_dense, pool)) return e;
+    }
+  }
+  if (int e = upload(c, &c->d_er, h_er)) return e;
+  if (int e = upload(c, &c->d_ec, h_ec)) return e;
+  if (int e = upload(c, &c->d_ev, h_ev)) return e;
+  c->nent = (long)h_er.size();
+
+  // constraint-major sparse terms (dense blocks excluded) and position-major transpose
+  std::vector<int> cptr(m + 1, 0);
+  std::vector<long> cp1, cp2;
+  std::vector<double> cv;
+  for (int k = 0; k < m; ++k) {
+    for (int q = 0; q < nS; ++q) {
+      const auto& es = ent[q][k];
+      if (es.empty()) continue;
+      const int n = c->sdpN[q];
+      if ((double)es.size() >= (double)n * n * 0.20) continue;  // dense: handled by the pool
+      for (auto& x : es) {
+        cp1.push_back(c->sdpOff[q] + x.r + (long)n * x.c);
+        cp2.push_back(c->sdpOff[q] + x.c + (long)n * x.r);
+        cv.push_back(x.v);
+      }
+    }
+    for (auto& pr : lpent[k]) {
+      cp1.push_back(c->lpOff + pr.first);
+      cp2.push_back(c->lpOff + pr.first);
+      cv.push_back(pr.second);
+    }
+    cptr[k + 1] = (int)cp1.size();
+  }
+  {
+    struct PT { long p; int con; double v; };
+    std::vector<PT> pts;
+    pts.reserve(cp1.size() * 2);
+    for (int k = 0; k < m; ++k)
+      for (int e = cptr[k]; e < cptr[k + 1]; ++e) {
+        pts.push_back(PT{cp1[e], k, cv[e]});
+        if (cp2[e] != cp1[e]) pts.push_back(PT{cp2[e], k, cv[e]});
+      }
+    std::stable_sort(pts.begin(), pts.end(), [](const PT& a, const PT& b) { return a.p < b.p; });
+    std::vector<long> ppos, pptr(1, 0);
+    std::vector<int> pcon;
+    std::vector<double> pval;
+    for (size_t t = 0; t < pts.size(); ++t) {
+      if (t == 0 || pts[t].p != pts[t - 1].p) {
+        if (t > 0) pptr.push_back((long)t);
+        ppos.push_back(pts[t].p);
+      }
+      pcon.push_back(pts[t].con);
+      pval.push_back(pts[t].v);
+    }
+    if (!pts.empty()) pptr.push_back((long)pts.size());
+    c->npos = (long)ppos.size();
+    if (int e = upload(c, &c->d_ppos, ppos)) return e;
+    if (int e = upload(c, &c->d_pptr, pptr)) return e;
+    if (int e = upload(c, &c->d_pcon, pcon)) return e;
+    if (int e = upload(c, &c->d_pval, pval)) return e;
+  }
+  if (int e = upload(c, &c->d_cptr, cptr)) return e;
+  if (int e = upload(c, &c->d_cp1, cp1)) return e;
+  if (int e = upload(c, &c->d_cp2, cp2)) return e;
+  if (int e = upload(c, &c->d_cv, cv)) return e;
+
+  // LP coordinates (InputData::initialize_index_LP)
+  {
+    std::vector<int> lpptr(c->nLP + 1, 0), lpcon;
+    std::vector<double> lpval;
+    std::vector<std::vector<std::pair<int, double>>> byc(c->nLP);
+    for (int k = 0; k < m; ++k)
+      for (auto& pr : lpent[k]) byc[pr.first].push_back({k, pr.second});
+    for (int t = 0; t < c->nLP; ++t) {
+      for (auto& pr : byc[t]) { lpcon.push_back(pr.first); lpval.push_back(pr.second); }
+      lpptr[t + 1] = (int)lpcon.size();
+    }
+    if (int e = upload(c, &c->d_lpptr, lpptr)) return e;
+    if (int e = upload(c, &c->d_lpcon, lpcon)) return e;
+    if (int e = upload(c, &c->d_lpval, lpval)) return e;
+    // Chordal::ordering_bMat dense decision (sdpa_chordal.cpp:376-410)
+    c->schurDense = 0;
+    if (m <= 100) c->schurDense = 1;
+    const double thr = m * std::sqrt(0.70);
+    for (int q = 0; q < nS; ++q)
+      if (c->blk[q].ncon > thr) c->schurDense = 1;
+    for (int t = 0; t < c->nLP; ++t)
+      if ((double)byc[t].size() > thr) c->schurDense = 1;
+  }
+  c->bmatStoreFirst = (nS == 1 && c->nLP == 0 && c->blk[0].ncon == m && c->blk[0].nF1 == 0);
+
+  std::vector<double> hb(cvec, cvec + m);
+  if (int e = upload(c, &c->d_b, hb)) return e;
+  if (int e = upload(c, &c->d_C, hC)) return e;
+  if (int e = upload(c, &c->d_sdpN, c->sdpN)) return e;
+  if (int e = upload(c, &c->d_sdpOff, c->sdpOff)) return e;
+  const size_t T = c->total;
+  double** packed[] = {&c->d_X, &c->d_Z, &c->d_invCholX, &c->d_invCholZ, &c->d_Zinv, &c->d_LX,
+                       &c->d_LZ, &c->d_Dx, &c->d_Dz, &c->d_rzinv, &c->d_xrdzinv, &c->d_w2,
+                       &c->d_dmat, &c->d_X0, &c->d_Z0};
+  for (double** p : packed)
+    if (int e = dalloc(p, T)) return e;
+  if (int e = dalloc(&c->d_w1, 2 * T)) return e;
+  if (int e = dalloc(&c->d_w3, 2 * T)) return e;
+  double** mvec[] = {&c->d_y, &c->d_dy, &c->d_g, &c->d_pvec};
+  for (double** p : mvec)
+    if (int e = dalloc(p, m)) return e;
+  if (int e = dalloc(&c->d_B, (size_t)m * m)) return e;
+  if (int e = dalloc(&c->d_red, 3 * 4096 + 64)) return e;
+  if (int e = dalloc(&c->d_eig, 2 * (size_t)(nS + 1))) return e;
+  SDPA_HIP_CHECK(hipMalloc(&c->d_info, sizeof(int) * (2 * nS + 2)));
+  if (2 * nS + 2 > 64) {
+    (void)hipHostFree(c->h_info);
+    SDPA_HIP_CHECK(hipHostMalloc((void**)&c->h_info, sizeof(int) * (2 * nS + 2), hipHostMallocDefault));
+  }
+  c->loaded = true;
+  return 0;
+}
+
+int sdpa_gpu_problem_info(sdpa_gpu_ctx* h, long* out, int cap) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  long v[12] = {c->m, c->nBlock, c->nSDP, c->nLP, c->nDim, c->total, c->nF[0], c->nF[1],
+                c->nF[2], c->schurDense, c->bmatStoreFirst ? 1 : 0, c->nSDP};
+  int n = cap < 12 ? cap : 12;
+  for (int t = 0; t < n; ++t) out[t] = v[t];
+  return n;
+}
+
+int sdpa_gpu_block_size(sdpa_gpu_ctx* h, int q) {
+  if (!h || !h->c.loaded || q < 0 || q >= h->c.nSDP) return -1;
+  return h->c.sdpN[q];
+}
+
+static double* buffer_ptr(Ctx* c, int which, long* len) {
+  switch (which) {
+    case 0: *len = c->total; return c->d_X;
+    case 1: *len = c->total; return c->d_Z;
+    case 2: *len = c->m; return c->d_y;
+    case 3: *len = c->total; return c->d_Zinv;
+    case 4: *len = c->total; return c->d_invCholX;
+    case 5: *len = c->total; return c->d_invCholZ;
+    case 6: *len = c->total; return c->d_Dx;
+    case 7: *len = c->total; return c->d_Dz;
+    case 8: *len = c->m; return c->d_dy;
+    case 9: *len = c->m; return c->d_g;
+    case 10: *len = (long)c->m * c->m; return c->d_B;
+    case 11: *len = c->m; return c->d_pvec;
+    case 12: *len = c->total; return c->d_dmat;
+    default: *len = 0; return nullptr;
+  }
+}
+
+int sdpa_gpu_get_buffer(sdpa_gpu_ctx* h, int which, double* dst, long len) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  long L;
+  double* p = buffer_ptr(c, which, &L);
+  if (!p || len != L) return fail(c, -20, "buffer id or length mismatch");
+  SDPA_HIP_CHECK(hipMemcpyAsync(dst, p, sizeof(double) * L, hipMemcpyDeviceToHost, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int sdpa_gpu_set_buffer(sdpa_gpu_ctx* h, int which, const double* src, long len) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  long L;
+  double* p = buffer_ptr(c, which, &L);
+  if (!p || len != L) return fail(c, -20, "buffer id or length mismatch");
+  SDPA_HIP_CHECK(hipMemcpyAsync(p, src, sizeof(double) * L, hipMemcpyHostToDevice, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int sdpa_gpu_set_iterate(sdpa_gpu_ctx* h, const double* X, const double* y, const double* Z) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  if (X) SDPA_HIP_CHECK(hipMemcpyAsync(c->d_X, X, sizeof(double) * c->total, hipMemcpyHostToDevice, c->stream));
+  if (y) SDPA_HIP_CHECK(hipMemcpyAsync(c->d_y, y, sizeof(double) * c->m, hipMemcpyHostToDevice, c->stream));
+  if (Z) SDPA_HIP_CHECK(hipMemcpyAsync(c->d_Z, Z, sizeof(double) * c->total, hipMemcpyHostToDevice, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int sdpa_gpu_get_iterate(sdpa_gpu_ctx* h, double* X, double* y, double* Z) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  if (X) SDPA_HIP_CHECK(hipMemcpyAsync(X, c->d_X, sizeof(double) * c->total, hipMemcpyDeviceToHost, c->stream));
+  if (y) SDPA_HIP_CHECK(hipMemcpyAsync(y, c->d_y, sizeof(double) * c->m, hipMemcpyDeviceToHost, c->stream));
+  if (Z) SDPA_HIP_CHECK(hipMemcpyAsync(Z, c->d_Z, sizeof(double) * c->total, hipMemcpyDeviceToHost, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int sdpa_gpu_identity_start(sdpa_gpu_ctx* h, double lambda) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  std::vector<double> X(c->total, 0.0), IC(c->total, 0.0), ZI(c->total, 0.0);
+  for (int q = 0; q < c->nSDP; ++q)
+    for (int k = 0; k < c->sdpN[q]; ++k) {
+      long d = c->sdpOff[q] + (long)k * c->sdpN[q] + k;
+      X[d] = lambda;
+      IC[d] = 1.0 / std::sqrt(lambda);
+      ZI[d] = 1.0 / lambda;
+    }
+  for (int t = 0; t < c->nLP; ++t) {
+    long d = c->lpOff + t;
+    X[d] = lambda;
+    IC[d] = 1.0 / std::sqrt(lambda);
+    ZI[d] = 1.0 / lambda;
+  }
+  const size_t B = sizeof(double) * c->total;
+  SDPA_HIP_CHECK(hipMemcpy(c->d_X, X.data(), B, hipMemcpyHostToDevice));
+  SDPA_HIP_CHECK(hipMemcpy(c->d_Z, X.data(), B, hipMemcpyHostToDevice));
+  SDPA_HIP_CHECK(hipMemcpy(c->d_invCholX, IC.data(), B, hipMemcpyHostToDevice));
+  SDPA_HIP_CHECK(hipMemcpy(c->d_invCholZ, IC.data(), B, hipMemcpyHostToDevice));
+  SDPA_HIP_CHECK(hipMemcpy(c->d_Zinv, ZI.data(), B, hipMemcpyHostToDevice));
+  SDPA_HIP_CHECK(hipMemset(c->d_y, 0, sizeof(double) * c->m));
+  return 0;
+}
+
+int sdpa_gpu_keep_initial_point(sdpa_gpu_ctx* h) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  const size_t B = sizeof(double) * c->total;
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->d_X0, c->d_X, B, hipMemcpyDeviceToDevice, c->stream));
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->d_Z0, c->d_Z, B, hipMemcpyDeviceToDevice, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// -- timing helpers ----------------------------------------------------------
+static void tic(Ctx* c) {
+  if (c->timing) (void)hipEventRecord(c->ev[0], c->stream);
+}
+static double toc(Ctx* c) {
+  if (!c->timing) return 0.0;
+  (void)hipEventRecord(c->ev[1], c->stream);
+  (void)hipEventSynchronize(c->ev[1]);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+  return ms;
+}
+
+int sdpa_gpu_inv_chol(sdpa_gpu_ctx* h) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  tic(c);
+  int r = inverse_chol(c);
+  c->t_inv += toc(c);
+  return r;
+}
+
+int sdpa_gpu_make_bmat(sdpa_gpu_ctx* h, int precision) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  if (precision != 0) return fail(c, -30, "only FP64 (precision 0) is implemented");
+  tic(c);
+  int r = make_bmat(c);
+  c->t_bmat += toc(c);
+  c->n_bmat++;
+  return r;
+}
+
+int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* h, int precision) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  if (precision != 0) return fail(c, -30, "only FP64 (precision 0) is implemented");
+  tic(c);
+  MatDesc d{c->d_B, c->m, c->m};
+  int r = potrf_batched(c->stream, &c->dw, &d, 1, 1, c->d_info);
+  if (r) return r;
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->t_chol += toc(c);
+  c->n_chol++;
+  return c->h_info[0];
+}
+
+int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* h, int refine) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  (void)refine;
+  tic(c);
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->d_dy, c->d_g, sizeof(double) * c->m, hipMemcpyDeviceToDevice,
+                                c->stream));
+  int r = chol_solve(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_dy);
+  c->t_solve += toc(c);
+  c->n_solve++;
+  return r;
+}
+
+int sdpa_gpu_min_eig(sdpa_gpu_ctx* h, double* lamX, double* lamZ) {
+  if (!h || !h->c.loaded) return -1;
+  Ctx* c = &h->c;
+  tic(c);
+  int r = min_eig(c, lamX, lamZ);
+  c->t_eig += toc(c);
+  return r;
+}
+
+int sdpa_gpu_residuals(sdpa_gpu_ctx* h, double* nP, double* nD) {
+  if (!h || !h->c.loaded) return -1;
+  return residuals(&h->c, nP, nD);
+}
+
+int sdpa_gpu_make_gvec(sdpa_gpu_ctx* h, int corrector, double target, int infeasible) {
+  if (!h || !h->c.loaded) return -1;
+  return make_gvec(&h->c, corrector, target, infeasible);
+}
+
+int sdpa_gpu_make_direction(sdpa_gpu_ctx* h, int infeasible) {
+  if (!h || !h->c.loaded) return -1;
+  return make_direction(&h->c, infeasible);
+}
+
+int sdpa_gpu_dots(sdpa_gpu_ctx* h, int nq, const int* codes, double* out) {
+  if (!h || !h->c.loaded) return -1;
+  if (nq <= 0 || nq > 32) return -2;
+  return compute_dots(&h->c, nq, codes, out);
+}
+
+int sdpa_gpu_update(sdpa_gpu_ctx* h, double aP, double aD) {
+  if (!h || !h->c.loaded) return -1;
+  return update_point(&h->c, aP, aD);
+}
+
+int sdpa_gpu_timing(sdpa_gpu_ctx* h, int enable, double* out) {
+  if (!h) return -1;
+  Ctx* c = &h->c;
+  if (out) {
+    out[0] = c->t_bmat; out[1] = c->n_bmat;
+    out[2] = c->t_chol; out[3] = c->n_chol;
+    out[4] = c->t_solve; out[5] = c->n_solve;
+    out[6] = c->t_eig; out[7] = 0;
+    out[8] = c->t_inv; out[9] = 0;
+  }
+  if (enable >= 0) {
+    c->timing = enable != 0;
+    c->t_bmat = c->t_chol = c->t_solve = c->t_eig = c->t_inv = 0;
+    c->n_bmat = c->n_chol = c->n_solve = 0;
+  }
+  return 0;
+}
+
+int sdpa_gpu_synchronize(sdpa_gpu_ctx* h) {
+  if (!h) return -1;
+  SDPA_HIP_CHECK(hipStreamSynchronize(h->c.stream));
+  return 0;
+}
+
+}  // extern "C"

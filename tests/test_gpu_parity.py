@@ -1,0 +1,222 @@
+"""HIP path (through the C ABI of include/sdpa_gpu.h and the solver wrapper)
+against the CPU oracle on the same seeded inputs.
+
+Tolerances: every kernel is FP64.  Per-element results that the oracle
+computes with a different summation order are compared at rtol 1e-11
+(bMat, factors, inverses); the solve at 1e-9 relative to the solution norm;
+the truncated Lanczos estimate at 1e-7 relative (the device applies
+L D L^T as one explicit matrix, the reference applies L, D, L^T in turn);
+whole solves: same phase, iteration counts within one, objectives within
+1e-6 relative and gap/feasibility at the 1e-7 stopping level.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sdpa_amd import problems as PR
+
+pytestmark = pytest.mark.gpu
+
+sa = pytest.importorskip("sdpa_amd")
+
+
+def spd(n, rng, cond=30.0):
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (q * np.geomspace(1, cond, n)) @ q.T
+
+
+def packed_random(d, rng, cond=30.0):
+    """Random SPD X, Z per SDP block and positive LP entries, packed."""
+    Xs, Zs, Zis = [], [], []
+    for s in d.block_struct:
+        s = int(s)
+        if s >= 2:
+            X, Z = spd(s, rng, cond), spd(s, rng, cond)
+            Xs.append(X.reshape(-1, order="F")); Zs.append(Z.reshape(-1, order="F"))
+            Zis.append(np.linalg.inv(Z).reshape(-1, order="F"))
+    nlp = sum(-int(s) if s < 0 else (1 if s == 1 else 0) for s in d.block_struct)
+    xl, zl = rng.uniform(0.5, 2, nlp), rng.uniform(0.5, 2, nlp)
+    X = np.concatenate(Xs + [xl]); Z = np.concatenate(Zs + [zl]); Zi = np.concatenate(Zis + [1 / zl])
+    return X, Z, Zi
+
+
+def ctx_for(d):
+    return sa.GpuContext(0).load(d)
+
+
+CASES = {
+    "maxcut100": lambda: PR.maxcut(100, 0.1, 1),
+    "lovasz40": lambda: PR.lovasz(40, 0.3, 5),
+    "dense_n20_m30": lambda: PR.dense_random(n=20, m=30, seed=2),
+    "dense_n70_m40": lambda: PR.dense_random(n=70, m=40, seed=6),
+    "multiblock_lp": lambda: PR.multi_block(n_sdp_blocks=6, nmin=3, nmax=30, lp=12, m=40,
+                                            density=0.05, seed=9),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_bmat_matches_oracle(name):
+    d = CASES[name]()
+    rng = np.random.default_rng(1)
+    X, Z, Zi = packed_random(d, rng)
+    g = ctx_for(d)
+    g.set("X", X); g.set("Zinv", Zi)
+    g.make_bmat()
+    B = g.bmat()
+    want = O.Problem(d).make_bmat(X, Zi)
+    tril = np.tril_indices(d.m)
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(B[tril], want[tril], rtol=1e-11, atol=1e-13 * scale)
+    print(name, "formulas", g.formula_counts())
+
+
+@pytest.mark.parametrize("m", [2, 5, 64, 65, 130, 300])
+def test_factor_and_solve_match_oracle(m):
+    rng = np.random.default_rng(m)
+    d = PR.maxcut(m, 0.5, seed=3)  # any problem with this m; only bMat is used
+    g = ctx_for(d)
+    A = spd(d.m, rng, 1e4)
+    g.set("B", A.reshape(-1, order="F"))
+    info = g.factor_bmat()
+    assert info == 0
+    L = g.bmat()
+    info_o, Lo = O.chol_adjust(A)
+    tril = np.tril_indices(d.m)
+    np.testing.assert_allclose(L[tril], Lo[tril], rtol=1e-10, atol=1e-12)
+    rhs = rng.standard_normal(d.m)
+    g.set("g", rhs)
+    g.solve_bmat()
+    x = g.get("dy")
+    want = np.linalg.solve(A, rhs)
+    assert np.linalg.norm(x - want) <= 1e-9 * np.linalg.norm(want)
+
+
+def test_factor_adjust_and_failure_rules():
+    rng = np.random.default_rng(7)
+    d = PR.maxcut(150, 0.5, seed=3)
+    g = ctx_for(d)
+    A = spd(150, rng, 1e3)
+    zero = np.array([0, 63, 64, 100, 149])
+    A[zero, :] = 0.0
+    A[:, zero] = 0.0
+    g.set("B", A.reshape(-1, order="F"))
+    assert g.factor_bmat() == 0
+    L = g.bmat()
+    _, Lo = O.chol_adjust(A)
+    big = np.diag(L) > 1e40
+    assert (np.flatnonzero(big) == zero).all()
+    tril = np.tril_indices(150)
+    np.testing.assert_allclose(L[tril], Lo[tril], rtol=1e-10, atol=1e-12)
+    B = spd(150, rng, 1e3) - 2e3 * np.eye(150)
+    g.set("B", B.reshape(-1, order="F"))
+    info_o, _ = O.chol_adjust(B)
+    assert g.factor_bmat() == info_o > 0
+
+
+def test_inverse_cholesky_matches_oracle():
+    d = PR.multi_block(n_sdp_blocks=5, nmin=2, nmax=140, lp=7, m=20, density=0.05, seed=4)
+    rng = np.random.default_rng(2)
+    X, Z, _ = packed_random(d, rng, cond=1e3)
+    g = ctx_for(d)
+    g.set("X", X); g.set("Z", Z)
+    assert g.inv_chol() == 0
+    icx, icz, zi = g.get("invCholX"), g.get("invCholZ"), g.get("Zinv")
+    off = 0
+    for s in d.block_struct:
+        s = int(s)
+        if s < 2:
+            continue
+        Xb = X[off:off + s * s].reshape(s, s, order="F")
+        Zb = Z[off:off + s * s].reshape(s, s, order="F")
+        _, Lx, _ = O.inv_chol(Xb)
+        _, Lz, Zinv = O.inv_chol(Zb)
+        np.testing.assert_allclose(icx[off:off + s * s].reshape(s, s, order="F"), Lx, rtol=1e-10, atol=1e-13)
+        np.testing.assert_allclose(icz[off:off + s * s].reshape(s, s, order="F"), Lz, rtol=1e-10, atol=1e-13)
+        np.testing.assert_allclose(zi[off:off + s * s].reshape(s, s, order="F"), Zinv, rtol=1e-9,
+                                   atol=1e-12 * np.abs(Zinv).max())
+        off += s * s
+    np.testing.assert_allclose(icx[off:], 1 / np.sqrt(X[off:]), rtol=1e-14)
+    np.testing.assert_allclose(zi[off:], 1 / Z[off:], rtol=1e-14)
+    # a non-PD block is reported, as dpotrf's info != 0 is
+    Xbad = X.copy()
+    Xbad[0] = -1.0
+    g.set("X", Xbad)
+    assert g.inv_chol() == 1
+
+
+def test_min_eig_matches_oracle():
+    d = PR.multi_block(n_sdp_blocks=6, nmin=2, nmax=200, lp=5, m=20, density=0.05, seed=8)
+    rng = np.random.default_rng(3)
+    X, Z, _ = packed_random(d, rng, cond=50.0)
+    g = ctx_for(d)
+    g.set("X", X); g.set("Z", Z)
+    assert g.inv_chol() == 0
+    Dx = rng.standard_normal(len(X)); Dz = rng.standard_normal(len(X))
+    # symmetric SDP blocks
+    off = 0
+    for s in d.block_struct:
+        s = int(s)
+        if s >= 2:
+            for D in (Dx, Dz):
+                b = D[off:off + s * s].reshape(s, s, order="F")
+                D[off:off + s * s] = ((b + b.T) / 2).reshape(-1, order="F")
+            off += s * s
+    g.set("Dx", Dx); g.set("Dz", Dz)
+    lx, lz = g.min_eig()
+    P = O.Problem(d)
+    ox = P.min_eig(g.get("invCholX"), Dx)
+    oz = P.min_eig(g.get("invCholZ"), Dz)
+    assert lx == pytest.approx(ox, rel=1e-7, abs=1e-10)
+    assert lz == pytest.approx(oz, rel=1e-7, abs=1e-10)
+
+
+@pytest.mark.parametrize("name", ["example1", "example2"])
+def test_solver_reproduces_manual(golden_dir, name):
+    import json, os
+    from sdpa_amd import dats
+    gd = json.load(open(os.path.join(golden_dir, "manual_examples.json")))[name]
+    s = sa.Solver(0).read_input(os.path.join(golden_dir, gd["file"]))
+    r = s.solve()
+    assert r.phase == gd["phase"]
+    assert r.iterations == gd["iterations"]
+    assert r.primal_obj == pytest.approx(gd["objValPrimal"], rel=1e-9)
+    assert r.dual_obj == pytest.approx(gd["objValDual"], rel=1e-9)
+    assert r.relative_gap == pytest.approx(gd["relative_gap"], rel=1e-5)
+    o = O.Problem(dats.read(os.path.join(golden_dir, gd["file"]))).solve()
+    for a, b in zip(r.trace, o.trace):
+        for key, idx in (("mu", 0), ("objP", 3), ("objD", 4), ("beta", 7)):
+            assert a[idx] == pytest.approx(b[key], rel=1e-7, abs=1e-9), (name, key)
+
+
+SOLVE_CASES = {
+    "maxcut100": lambda: PR.maxcut(100, 0.1, 1),
+    "lovasz30": lambda: PR.lovasz(30, 0.3, 2),
+    "dense_n25_m40": lambda: PR.dense_random(n=25, m=40, seed=3),
+    "multiblock_lp": lambda: PR.multi_block(n_sdp_blocks=8, nmin=3, nmax=25, lp=20, m=60,
+                                            density=0.03, seed=5),
+}
+
+
+@pytest.mark.parametrize("name", list(SOLVE_CASES))
+def test_full_solve_matches_oracle(name):
+    d = SOLVE_CASES[name]()
+    r = sa.Solver(0).load(d).solve()
+    o = O.Problem(d).solve()
+    print(name, r.phase, r.iterations, o.iterations, r.primal_obj, o.primal_obj)
+    assert r.phase == o.phase
+    assert abs(r.iterations - o.iterations) <= 1
+    assert r.primal_obj == pytest.approx(o.primal_obj, rel=1e-6, abs=1e-8)
+    assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)
+    if o.phase == "pdOPT":
+        assert r.relative_gap <= 1e-7
+        assert r.primal_error <= 1e-7 and r.dual_error <= 1e-7
+
+
+def test_init_point_solve(golden_dir):
+    import os
+    s = sa.Solver(0).read_input(os.path.join(golden_dir, "example1.dat-s"))
+    s.set_init(np.array([5.9, -1.375, -1.375, 1.0]), np.array([11.0, 0.0, 0.0, 9.0]),
+               -np.array([0.0, -4.0, 0.0]))
+    r = s.solve()
+    assert r.phase == "pdOPT"
+    assert r.primal_obj == pytest.approx(-41.9, rel=1e-6)
