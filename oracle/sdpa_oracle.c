@@ -521,28 +521,78 @@ static int potrf_leaf(int N, double *A, int lda) {
   return 0;
 }
 
-/* dtrsm('R','L','T','N', M, N, 1, L, B): B := B * L^-T  (L is N x N lower) */
+/* dtrsm('R','L','T','N', M, N, 1, L, B): B := B * L^-T  (L is N x N lower).
+ * Rows are independent: blocks of 64 rows in parallel, columns swept left to
+ * right (B[:,c] -= L[c,k] B[:,k] for k < c, then / L[c,c]). */
 static void trsm_rltn(int M, int N, const double *L, int lda, double *B, int ldb) {
-#pragma omp parallel for schedule(static) if ((long)M * N > 4096)
-  for (int r = 0; r < M; r++)
+  const int RB = 64;
+  int nrb = (M + RB - 1) / RB;
+#pragma omp parallel for schedule(dynamic, 1) if ((long)M * N > 4096)
+  for (int rb = 0; rb < nrb; rb++) {
+    int r0 = rb * RB, r1 = r0 + RB < M ? r0 + RB : M;
     for (int c = 0; c < N; c++) {
-      double s = B[r + (size_t)ldb * c];
-      for (int k = 0; k < c; k++) s -= B[r + (size_t)ldb * k] * L[c + (size_t)lda * k];
-      B[r + (size_t)ldb * c] = s / L[c + (size_t)lda * c];
+      double *bc = B + (size_t)ldb * c;
+      int k = 0;
+      for (; k + 4 <= c; k += 4) {
+        const double l0 = L[c + (size_t)lda * k], l1 = L[c + (size_t)lda * (k + 1)];
+        const double l2 = L[c + (size_t)lda * (k + 2)], l3 = L[c + (size_t)lda * (k + 3)];
+        const double *b0 = B + (size_t)ldb * k, *b1 = b0 + ldb, *b2 = b1 + ldb, *b3 = b2 + ldb;
+        for (int r = r0; r < r1; r++) bc[r] -= l0 * b0[r] + l1 * b1[r] + l2 * b2[r] + l3 * b3[r];
+      }
+      for (; k < c; k++) {
+        const double lk = L[c + (size_t)lda * k];
+        const double *bk = B + (size_t)ldb * k;
+        for (int r = r0; r < r1; r++) bc[r] -= lk * bk[r];
+      }
+      const double d = L[c + (size_t)lda * c];
+      for (int r = r0; r < r1; r++) bc[r] /= d;
     }
+  }
 }
 
-/* dsyrk('L','N', N, K, -1, A, 1, C): lower C -= A A^T */
+/* dsyrk('L','N', N, K, -1, A, 1, C): lower C -= A A^T, by 64 x 64 tiles of C
+ * in parallel, k in blocks of 256, four columns of C per sweep. */
 static void syrk_ln(int N, int K, const double *A, int lda, double *C, int ldc) {
-#pragma omp parallel for schedule(dynamic, 8) if ((long)N * K > 4096)
-  for (int j = 0; j < N; j++)
-    for (int k = 0; k < K; k++) {
-      double ajk = A[j + (size_t)lda * k];
-      if (ajk == 0.0) continue;
-      const double *ak = A + (size_t)lda * k;
-      double *cj = C + (size_t)ldc * j;
-      for (int i = j; i < N; i++) cj[i] -= ak[i] * ajk;
+  const int TBL = 64, KB = 256;
+  int nt = (N + TBL - 1) / TBL;
+  long ntiles = (long)nt * (nt + 1) / 2;
+#pragma omp parallel for schedule(dynamic, 1) if ((long)N * K > 4096)
+  for (long t = 0; t < ntiles; t++) {
+    int I = 0;
+    long rem = t;
+    while (rem >= I + 1) { rem -= I + 1; I++; }
+    int J = (int)rem;
+    int i0 = I * TBL, i1 = i0 + TBL < N ? i0 + TBL : N;
+    int j0 = J * TBL, j1 = j0 + TBL < N ? j0 + TBL : N;
+    for (int k0 = 0; k0 < K; k0 += KB) {
+      int k1 = k0 + KB < K ? k0 + KB : K;
+      int j = j0;
+      for (; j + 4 <= j1; j += 4) {
+        double *c0 = C + (size_t)ldc * j, *c1 = c0 + ldc, *c2 = c1 + ldc, *c3 = c2 + ldc;
+        for (int k = k0; k < k1; k++) {
+          const double *ak = A + (size_t)lda * k;
+          const double a0 = ak[j], a1 = ak[j + 1], a2 = ak[j + 2], a3 = ak[j + 3];
+          int ib = i0 > j ? i0 : j;
+          for (int i = ib; i < i1; i++) {
+            const double x = ak[i];
+            c0[i] -= x * a0;
+            if (i >= j + 1) c1[i] -= x * a1;
+            if (i >= j + 2) c2[i] -= x * a2;
+            if (i >= j + 3) c3[i] -= x * a3;
+          }
+        }
+      }
+      for (; j < j1; j++) {
+        double *cj = C + (size_t)ldc * j;
+        for (int k = k0; k < k1; k++) {
+          const double *ak = A + (size_t)lda * k;
+          const double ajk = ak[j];
+          int ib = i0 > j ? i0 : j;
+          for (int i = ib; i < i1; i++) cj[i] -= ak[i] * ajk;
+        }
+      }
     }
+  }
 }
 
 /* rATL_dpotrfL, sdpa_dpotrf.cpp:218-265: recursive halving. */

@@ -50,13 +50,15 @@ __device__ __forceinline__ void mma_tile(v4d acc[2][2], const double* As, int ld
   }
 }
 
-// Position of accumulator element v of tile (ti,tj) for this lane:
-// row = r0 + 16*ti + lane%16, col = c0 + 16*tj + 4*(lane/16) + v.
+// Position of accumulator element v of tile (ti,tj) for this lane.  The
+// f64 16x16x4 MFMA leaves D[i][j] in lane j + 16*(i%4), register i/4
+// (measured on gfx950), and D = C^T here, so
+// row = r0 + 16*ti + lane%16, col = c0 + 16*tj + 4*v + lane/16.
 __device__ __forceinline__ void acc_pos(int ti, int tj, int v, int& r, int& c) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   r = 32 * (w & 1) + 16 * ti + (lane & 15);
-  c = 32 * (w >> 1) + 16 * tj + 4 * (lane >> 4) + v;
+  c = 32 * (w >> 1) + 16 * tj + 4 * v + (lane >> 4);
 }
 
 __device__ __forceinline__ void acc_zero(v4d acc[2][2]) {
