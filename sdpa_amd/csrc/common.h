@@ -52,13 +52,21 @@ int gemm(hipStream_t s, DenseWork* w, const double* A, int lda, const double* B,
 // semantics (fail at the first pivot that is not > 0); mode 1: SDPA's adjust
 // rule (pivot < -1e-6 fails, pivot < 1e-14 becomes 1e100).  d_info[b]
 // receives 0 or the 1-based failing column of item b.
+// dinv_out (optional): receives inv(L_kk) of every 64x64 diagonal block,
+// slot (item, k0/64), 64*64 doubles each, ceil(maxn/64) slots per item.
 int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitems, int mode,
-                  int* d_info);
+                  int* d_info, double* dinv_out = nullptr);
 // W = L^-1 for a batch of lower factors (upper parts of W zeroed).
 int trtri_batched(hipStream_t s, DenseWork* w, const MatDesc* h_L, const MatDesc* h_W,
                   int nitems);
-// x := L^-T L^-1 x (two triangular solves with the lower factor L, m x m).
-int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, double* x);
+// Inverses of the 256x256 diagonal blocks of the m x m lower factor L into
+// Winv (ceil(m/256) blocks of 256*256 doubles, ld 256).
+int diag_block_inverses(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
+                        double* Winv);
+// x := L^-T L^-1 x (two triangular solves with the lower factor L, m x m),
+// using the block inverses from diag_block_inverses.
+int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, const double* Winv,
+               double* x);
 
 struct DenseWork {
   GemmItem* d_items = nullptr;  // device array of descriptors

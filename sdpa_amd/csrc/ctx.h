@@ -80,6 +80,8 @@ struct Ctx {
   double *d_pvec = nullptr, *d_dmat = nullptr;  // residuals
   double *d_X0 = nullptr, *d_Z0 = nullptr;      // initial point (init-point runs)
   double* d_B = nullptr;   // m x m Schur matrix, column-major, lower triangle used
+  double* d_Binv = nullptr;  // inverses of the 64x64 diagonal blocks of its factor
+  double* d_Winv = nullptr;  // inverses of its 256x256 diagonal blocks (for the solves)
   double* d_G = nullptr;   // F1 scratch: n*n per owner in a chunk
   size_t capG = 0;
   double* d_F1C = nullptr;  // F1 dense-pair products

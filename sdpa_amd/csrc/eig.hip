@@ -26,82 +26,109 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-// Implicit QL on a (N <= 64) tridiagonal held one entry per lane (dreg = d[lane],
-// ereg = e[lane]); Z is N x N in LDS (ld 65), row = lane.  Eigenvalues end
-// ascending in dreg.  Returns nonzero if an eigenvalue did not converge.
-__device__ int ql_wave(int N, double& dreg, double& ereg, double* Z) {
-  const int lane = threadIdx.x;
-  for (int i = 0; i < N; ++i) Z[lane + 65 * i] = (lane == i) ? 1.0 : 0.0;
-  if (lane == N - 1) ereg = 0.0;
-  int fail = 0;
-  for (int l = 0; l < N && !fail; ++l) {
-    int iter = 0;
-    for (;;) {
-      int mm;
-      for (mm = l; mm < N - 1; ++mm) {
-        double dm = __shfl(dreg, mm, 64), dm1 = __shfl(dreg, mm + 1, 64);
-        double em = __shfl(ereg, mm, 64);
-        if (fabs(em) <= 2.220446049250313e-16 * (fabs(dm) + fabs(dm1))) break;
-      }
-      if (mm == l) break;
-      if (++iter > 30 * N) { fail = 1; break; }
-      double dl = __shfl(dreg, l, 64), dl1 = __shfl(dreg, l + 1, 64), el = __shfl(ereg, l, 64);
-      double g = (dl1 - dl) / (2.0 * el);
-      double r = hypot(g, 1.0);
-      g = __shfl(dreg, mm, 64) - dl + el / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-      double s = 1.0, c = 1.0, p = 0.0;
-      int i;
-      bool brk = false;
-      for (i = mm - 1; i >= l; --i) {
-        double ei = __shfl(ereg, i, 64);
-        double f = s * ei, b = c * ei;
-        r = hypot(f, g);
-        if (lane == i + 1) ereg = r;
-        if (r == 0.0) {
-          if (lane == i + 1) dreg -= p;
-          if (lane == mm) ereg = 0.0;
-          brk = true;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = __shfl(dreg, i + 1, 64) - p;
-        r = (__shfl(dreg, i, 64) - g) * s + 2.0 * c * b;
-        p = s * r;
-        if (lane == i + 1) dreg = g + p;
-        g = c * r - b;
-        if (lane < N) {
-          double zf = Z[lane + 65 * (i + 1)];
-          double zi = Z[lane + 65 * i];
-          Z[lane + 65 * (i + 1)] = s * zi + c * zf;
-          Z[lane + 65 * i] = c * zi - s * zf;
-        }
-      }
-      if (brk && i >= l) continue;
-      if (lane == l) { dreg -= p; ereg = g; }
-      if (lane == mm) ereg = 0.0;
-    }
+// Smallest eigenvalue theta of the symmetric tridiagonal T (diagonal d[0..N),
+// off-diagonal e[0..N-1), in LDS) and the last component of its unit
+// eigenvector -- what the reference reads from dsteqr's output (the first
+// eigenvalue in ascending order and Q[k][0], sdpa_linear.cpp:98-127).  One
+// wave: multisection with Sturm counts (64 shifts per round, the interval of
+// the smallest eigenvalue shrinks 65-fold per round), then two steps of
+// inverse iteration from a shift just below theta, where T - sigma I is
+// positive definite and its LDL^T needs no pivoting.
+__device__ void tridiag_min_wave(int N, const double* d, const double* e, double& theta,
+                                 double& zlast) {
+  const int lane = threadIdx.x & 63;
+  const double eps = 2.220446049250313e-16;
+  double lo = 1.0e300, hi = -1.0e300;
+  for (int i = lane; i < N; i += 64) {
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < N - 1 ? fabs(e[i]) : 0.0);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
   }
-  // selection sort, ascending, with the eigenvector columns
-  for (int i = 0; i < N - 1; ++i) {
-    int kmin = i;
-    double dmin = __shfl(dreg, i, 64);
-    for (int j = i + 1; j < N; ++j) {
-      double dj = __shfl(dreg, j, 64);
-      if (dj < dmin) { dmin = dj; kmin = j; }
-    }
-    if (kmin != i) {
-      double di = __shfl(dreg, i, 64);
-      if (lane == i) dreg = dmin;
-      if (lane == kmin) dreg = di;
-      if (lane < N) {
-        double z = Z[lane + 65 * i];
-        Z[lane + 65 * i] = Z[lane + 65 * kmin];
-        Z[lane + 65 * kmin] = z;
-      }
-    }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
   }
-  return fail;
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  const double pivmin = 1.0e-290 + eps * eps * tnorm * tnorm;
+  lo -= 4.0 * eps * tnorm + pivmin;
+  hi += 4.0 * eps * tnorm + pivmin;
+  for (int round = 0; round < 20; ++round) {
+    const double x = lo + (hi - lo) * ((double)(lane + 1) / 65.0);
+    // Sturm count: eigenvalues below x
+    int cnt = 0;
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+    for (int i = 1; i < N; ++i) {
+      q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += q < 0.0;
+    }
+    const unsigned long long mask = __ballot(cnt >= 1);
+    const int first = mask ? (__ffsll((long long)mask) - 1) : 64;
+    const double nhi = (first < 64) ? __shfl(x, first, 64) : hi;
+    const double nlo = (first == 0) ? lo : __shfl(x, first > 0 ? first - 1 : 0, 64);
+    lo = (first == 64) ? __shfl(x, 63, 64) : nlo;
+    hi = nhi;
+    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+  }
+  theta = 0.5 * (lo + hi);
+  // inverse iteration: LDL^T of T - sigma I (pivots piv_i), two solves from
+  // b = ones with a normalization between them.  The sequential sweeps run on
+  // every lane with the same scalars; lane i keeps entry i.
+  const double sigma = theta - (8.0 * eps * tnorm + pivmin);
+  double last = 1.0, nrm2 = 0.0;
+  {
+    double piv = 0.0, y = 0.0, zi = 0.0;
+    double pprev = 1.0, yprev = 0.0;
+    for (int i = 0; i < N; ++i) {  // pivots and forward sweep L y = 1
+      const double ei = (i > 0) ? e[i - 1] : 0.0;
+      double p = (d[i] - sigma) - ((i > 0) ? ei * ei / pprev : 0.0);
+      if (fabs(p) < pivmin) p = pivmin;
+      const double l = (i > 0) ? ei / pprev : 0.0;
+      const double yi = 1.0 - l * yprev;
+      if (lane == i) { piv = p; y = yi; }
+      pprev = p;
+      yprev = yi;
+    }
+    // backward: z_i = y_i / piv_i - (e_i / piv_i) z_{i+1}
+    double znext = 0.0;
+    for (int i = N - 1; i >= 0; --i) {
+      const double pi = __shfl(piv, i, 64), yi = __shfl(y, i, 64);
+      const double ei = (i < N - 1) ? e[i] : 0.0;
+      const double zi2 = yi / pi - (ei / pi) * znext;
+      if (lane == i) zi = zi2;
+      znext = zi2;
+    }
+    // normalize and second solve with b = z
+    double s2 = (lane < N) ? zi * zi : 0.0;
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
+    const double inv = 1.0 / sqrt(s2);
+    const double b = zi * inv;
+    pprev = 1.0;
+    yprev = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const double ei = (i > 0) ? e[i - 1] : 0.0;
+      const double p = __shfl(piv, i, 64);
+      const double l = (i > 0) ? ei / pprev : 0.0;
+      const double yi = __shfl(b, i, 64) - l * yprev;
+      if (lane == i) y = yi;
+      pprev = p;
+      yprev = yi;
+    }
+    znext = 0.0;
+    for (int i = N - 1; i >= 0; --i) {
+      const double pi = __shfl(piv, i, 64), yi = __shfl(y, i, 64);
+      const double ei = (i < N - 1) ? e[i] : 0.0;
+      const double zi2 = yi / pi - (ei / pi) * znext;
+      if (lane == i) zi = zi2;
+      znext = zi2;
+    }
+    nrm2 = (lane < N) ? zi * zi : 0.0;
+    for (int o = 32; o > 0; o >>= 1) nrm2 += __shfl_xor(nrm2, o, 64);
+    last = __shfl(zi, N - 1, 64);
+  }
+  zlast = last / sqrt(nrm2);
 }
 
 // Smallest eigenvalue of a symmetric n x n (n <= 64) matrix in LDS (ld 65),
@@ -147,78 +174,124 @@ __device__ double jacobi_min_wave(int n, double* A) {
   return dv;
 }
 
-// One wave per (block, which).  Ms: packed scaled directions for X (which 0)
-// and Z (which 1), stacked at offsets 0 and lpOff.
-__global__ __launch_bounds__(64) void k_min_eig(const double* __restrict__ Ms, long stride,
-                                                const long* __restrict__ off,
-                                                const int* __restrict__ nn, int nblk,
-                                                double* __restrict__ out) {
+// n <= 20: one wave per (block, which), cyclic Jacobi on M = L D L^T.
+// Ms: packed scaled directions for X (which 0) and Z (which 1), at offsets 0
+// and `stride`.
+__global__ __launch_bounds__(64) void k_min_eig_small(const double* __restrict__ Ms, long stride,
+                                                      const long* __restrict__ off,
+                                                      const int* __restrict__ nn, int nblk,
+                                                      double* __restrict__ out) {
   const int q = blockIdx.x, which = blockIdx.y;
   const int n = nn[q];
+  if (n > 20) return;
   const double* M = Ms + which * stride + off[q];
   const int lane = threadIdx.x;
   __shared__ double Zs[64 * 65];
-  __shared__ double qv[kMaxN], qold[kMaxN], rv[kMaxN], wv[kMaxN];
-  double result;
-  if (n <= 20) {
-    for (int j = 0; j < n; ++j)
-      if (lane < n) Zs[lane + 65 * j] = M[lane + (size_t)n * j];
-    // dsyev('L') reads the lower triangle: mirror it
-    __syncthreads();
-    for (int j = 0; j < n; ++j)
-      if (lane < n && lane < j) Zs[lane + 65 * j] = Zs[j + 65 * lane];
-    __syncthreads();
-    result = jacobi_min_wave(n, Zs);
-  } else {
-    double dreg = 1.0e50, ereg = 0.0;    // diag[lane], off[lane] of the Lanczos recurrence
-    double mn = 1.0e+51, mn_old = 1.0e+52, error = 1.0e+10;
-    int k = 0, kk = 0;
-    for (int i = lane; i < n; i += 64) { qv[i] = 0.0; rv[i] = 1.0; }
-    double beta = sqrt((double)n);
-    const double sqn = sqrt((double)n);
-    while (k < n && k < sqn + 10 && k < kMaxLanczos && beta > 1.0e-16 &&
-           (fabs(mn - mn_old) > (1.0e-5) * fabs(mn) + (1.0e-8) ||
-            fabs(error * beta) > (1.0e-2) * fabs(mn) + (1.0e-4))) {
-      const double inv = 1.0 / beta;
-      __syncthreads();
-      for (int i = lane; i < n; i += 64) { qold[i] = qv[i]; qv[i] = rv[i] * inv; }
-      __syncthreads();
-      // w = M q
-      for (int i = lane; i < n; i += 64) {
-        double s = 0.0;
-        for (int j = 0; j < n; ++j) s += M[i + (size_t)n * j] * qv[j];
-        wv[i] = s;
-      }
-      __syncthreads();
-      double alpha = 0.0;
-      for (int i = lane; i < n; i += 64) alpha += qv[i] * wv[i];
-      alpha = wsum(alpha);
-      // diag[k] = alpha
-      if (lane == k) dreg = alpha;
-      for (int i = lane; i < n; i += 64) rv[i] = (wv[i] - qv[i] * alpha) - qold[i] * beta;
-      if (kk >= sqrt((double)k) || k == n - 1 || k > sqrt((double)n + 9)) {
-        kk = 0;
-        double d = dreg, e = ereg;
-        if (lane > k) { d = 0.0; e = 0.0; }
-        __syncthreads();
-        int info = ql_wave(k + 1, d, e, Zs);
-        __syncthreads();
-        if (info > 0) break;
-        mn_old = mn;
-        mn = __shfl(d, 0, 64);
-        error = Zs[k + 65 * 0];
-      }
-      __syncthreads();
-      double v2 = 0.0;
-      for (int i = lane; i < n; i += 64) v2 += rv[i] * rv[i];
-      beta = sqrt(wsum(v2));
-      if (lane == k) ereg = beta;
-      ++k;
-      ++kk;
-    }
-    result = mn - fabs(error * beta);
-  }
+  for (int j = 0; j < n; ++j)
+    if (lane < n) Zs[lane + 65 * j] = M[lane + (size_t)n * j];
+  // dsyev('L') reads the lower triangle: mirror it
+  __syncthreads();
+  for (int j = 0; j < n; ++j)
+    if (lane < n && lane < j) Zs[lane + 65 * j] = Zs[j + 65 * lane];
+  __syncthreads();
+  const double result = jacobi_min_wave(n, Zs);
   if (lane == 0) out[which * nblk + q] = result;
+}
+
+constexpr int kLzThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wsum(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < kLzThreads / 64; ++i) s += red[i];
+  return s;
+}
+
+// n > 20: the truncated Lanczos of Lal::getMinEigen on M, 256 threads per
+// (block, which); the matrix-vector product uses all four waves, the
+// tridiagonal eigenproblem runs on wave 0.
+__global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict__ Ms, long stride,
+                                                        const long* __restrict__ off,
+                                                        const int* __restrict__ nn, int nblk,
+                                                        double* __restrict__ out) {
+  const int q = blockIdx.x, which = blockIdx.y;
+  const int n = nn[q];
+  if (n <= 20) return;
+  const double* M = Ms + which * stride + off[q];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const bool w0 = tid < 64;
+  __shared__ double dg[kMaxLanczos], od[kMaxLanczos];  // tridiagonal of the recurrence
+  __shared__ double qv[kMaxN], qold[kMaxN], rv[kMaxN], wv[kMaxN];
+  __shared__ double red[kLzThreads / 64];
+  __shared__ double sh[4];  // theta, last eigenvector component
+  double mn = 1.0e+51, mn_old = 1.0e+52, error = 1.0e+10;
+  int k = 0, kk = 0;
+  for (int i = tid; i < n; i += kLzThreads) { qv[i] = 0.0; rv[i] = 1.0; }
+  double beta = sqrt((double)n);
+  const double sqn = sqrt((double)n);
+  while (k < n && k < sqn + 10 && k < kMaxLanczos && beta > 1.0e-16 &&
+         (fabs(mn - mn_old) > (1.0e-5) * fabs(mn) + (1.0e-8) ||
+          fabs(error * beta) > (1.0e-2) * fabs(mn) + (1.0e-4))) {
+    const double inv = 1.0 / beta;
+    __syncthreads();
+    for (int i = tid; i < n; i += kLzThreads) { qold[i] = qv[i]; qv[i] = rv[i] * inv; }
+    __syncthreads();
+    // w = M q: one wave per row block of 4 rows; M is symmetric, so row i is
+    // read as column i (contiguous), lanes over j, four rows in flight.
+    {
+      const int wv_id = tid >> 6;
+      for (int i0 = 4 * wv_id; i0 < n; i0 += 4 * (kLzThreads / 64)) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const double* c0 = M + (size_t)n * i0;
+        const bool h1 = i0 + 1 < n, h2 = i0 + 2 < n, h3 = i0 + 3 < n;
+        for (int j = lane; j < n; j += 64) {
+          const double qj = qv[j];
+          s0 += c0[j] * qj;
+          if (h1) s1 += c0[n + j] * qj;
+          if (h2) s2 += c0[2 * (size_t)n + j] * qj;
+          if (h3) s3 += c0[3 * (size_t)n + j] * qj;
+        }
+        s0 = wsum(s0); s1 = wsum(s1); s2 = wsum(s2); s3 = wsum(s3);
+        if (lane == 0) {
+          wv[i0] = s0;
+          if (h1) wv[i0 + 1] = s1;
+          if (h2) wv[i0 + 2] = s2;
+          if (h3) wv[i0 + 3] = s3;
+        }
+      }
+    }
+    __syncthreads();
+    double a = 0.0;
+    for (int i = tid; i < n; i += kLzThreads) a += qv[i] * wv[i];
+    const double alpha = block_sum(a, red);
+    if (tid == 0) dg[k] = alpha;
+    for (int i = tid; i < n; i += kLzThreads) rv[i] = (wv[i] - qv[i] * alpha) - qold[i] * beta;
+    if (kk >= sqrt((double)k) || k == n - 1 || k > sqrt((double)n + 9)) {
+      kk = 0;
+      __syncthreads();
+      if (w0) {
+        double th, zl;
+        tridiag_min_wave(k + 1, dg, od, th, zl);
+        if (lane == 0) { sh[0] = th; sh[1] = zl; }
+      }
+      __syncthreads();
+      mn_old = mn;
+      mn = sh[0];
+      error = sh[1];
+    }
+    double v2 = 0.0;
+    for (int i = tid; i < n; i += kLzThreads) v2 += rv[i] * rv[i];
+    beta = sqrt(block_sum(v2, red));
+    if (tid == 0) od[k] = beta;
+    ++k;
+    ++kk;
+  }
+  if (tid == 0) out[which * nblk + q] = mn - fabs(error * beta);
 }
 
 int lp_min_step(Ctx* c, const double* D, const double* L, double* out);
@@ -249,7 +322,9 @@ int min_eig(Ctx* c, double* lamX, double* lamZ) {
       return e;
     if (int e = gemm_batched(c->stream, c->dw.d_items + 2 * nb, 2 * nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
       return e;
-    hipLaunchKernelGGL(k_min_eig, dim3(nb, 2), dim3(64), 0, c->stream, c->d_w1, c->lpOff,
+    hipLaunchKernelGGL(k_min_eig_small, dim3(nb, 2), dim3(64), 0, c->stream, c->d_w1, c->lpOff,
+                       c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
+    hipLaunchKernelGGL(k_lanczos, dim3(nb, 2), dim3(kLzThreads), 0, c->stream, c->d_w1, c->lpOff,
                        c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
     SDPA_HIP_CHECK(hipGetLastError());
     std::vector<double> h(2 * nb);

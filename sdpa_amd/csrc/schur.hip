@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_f3_heavy(
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (k >= ncon) return;
-  if (ecnt[k] > kLight && k > h) return;  // heavy-heavy pairs once, from the larger index
+  if (ecnt[k] > kLight) return;  // heavy-heavy pairs: k_f3_heavy_pair
   const int ki = k > h ? k : h, kj = k > h ? h : k;
   const int nzi = nzeff[ki], nzj = nzeff[kj];
   const int o = (nzi > nzj) ? ki : kj;
@@ -100,6 +100,48 @@ __global__ __launch_bounds__(256) void k_f3_heavy(
   double v = f3_pair(er, ec, ev, estart[o], ecnt[o], estart[p], ecnt[p], n, X, Zi, lane, 64);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if (lane == 0) B[(size_t)con[ki] + (size_t)m * con[kj]] += v;
+}
+
+// Pairs of two heavy lists: one work-group per pair, the owner x partner
+// term grid flattened over 256 threads, block reduction.  grid = (nheavy, nheavy).
+__global__ __launch_bounds__(256) void k_f3_heavy_pair(
+    const int* __restrict__ con, const int* __restrict__ estart, const int* __restrict__ ecnt,
+    const int* __restrict__ nzeff, const int* __restrict__ form, const int* __restrict__ heavy,
+    const int* __restrict__ er, const int* __restrict__ ec, const double* __restrict__ ev, int n,
+    const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ B, int m) {
+  const int h1 = heavy[blockIdx.x], h2 = heavy[blockIdx.y];
+  if (h1 < h2) return;
+  const int ki = h1, kj = h2;
+  const int o = (nzeff[ki] > nzeff[kj]) ? ki : kj;
+  const int p = (o == ki) ? kj : ki;
+  if (form[o] != 3) return;
+  const int so = estart[o], co = ecnt[o], sp = estart[p], cp = ecnt[p];
+  const long total = (long)co * cp;
+  double acc = 0.0;
+  for (long t = threadIdx.x; t < total; t += 256) {
+    const int e1 = (int)(t / cp), e2 = (int)(t % cp);
+    const int a = er[so + e1], b = ec[so + e1];
+    const int g = er[sp + e2], d = ec[sp + e2];
+    const double* Xa = X + (size_t)n * a;
+    const double* Xb = X + (size_t)n * b;
+    const double* Za = Zi + (size_t)n * a;
+    const double* Zb = Zi + (size_t)n * b;
+    double plu;
+    if (a != b) {
+      if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d] + Za[d] * Xb[g] + Za[g] * Xb[d];
+      else plu = Zb[d] * Xa[g] + Za[d] * Xb[g];
+    } else {
+      if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d];
+      else plu = Zb[d] * Xa[g];
+    }
+    acc += ev[so + e1] * (ev[sp + e2] * plu);
+  }
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    B[(size_t)con[ki] + (size_t)m * con[kj]] += (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // ---- F1/F2 ----------------------------------------------------------------
@@ -300,6 +342,9 @@ int make_bmat(Ctx* c) {
         hipLaunchKernelGGL(k_f3_heavy, gh, dim3(256), 0, s, T.d_con, T.d_estart, T.d_ecnt,
                            T.d_nzeff, T.d_form, T.d_heavy, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n,
                            X, Zi, c->d_B, m);
+        hipLaunchKernelGGL(k_f3_heavy_pair, dim3(T.nheavy, T.nheavy), dim3(256), 0, s, T.d_con,
+                           T.d_estart, T.d_ecnt, T.d_nzeff, T.d_form, T.d_heavy, c->d_er, c->d_ec,
+                           c->d_ev, T.n, X, Zi, c->d_B, m);
       }
     }
   }

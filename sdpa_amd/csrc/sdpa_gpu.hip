@@ -60,7 +60,7 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_Dx, (void**)&c->d_Dz, (void**)&c->d_dy, (void**)&c->d_rzinv,
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
-                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_G, (void**)&c->d_F1C,
+                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
   for (void** p : ptrs) {
@@ -400,6 +400,8 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   for (double** p : mvec)
     if (int e = dalloc(p, m)) return e;
   if (int e = dalloc(&c->d_B, (size_t)m * m)) return e;
+  if (int e = dalloc(&c->d_Binv, (size_t)((m + 63) / 64) * 4096)) return e;
+  if (int e = dalloc(&c->d_Winv, (size_t)((m + 255) / 256) * 65536)) return e;
   if (int e = dalloc(&c->d_red, 3 * 4096 + 64)) return e;
   if (int e = dalloc(&c->d_eig, 2 * (size_t)(nS + 1))) return e;
   SDPA_HIP_CHECK(hipMalloc(&c->d_info, sizeof(int) * (2 * nS + 2)));
@@ -564,10 +566,14 @@ int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* h, int precision) {
   if (precision != 0) return fail(c, -30, "only FP64 (precision 0) is implemented");
   tic(c);
   MatDesc d{c->d_B, c->m, c->m};
-  int r = potrf_batched(c->stream, &c->dw, &d, 1, 1, c->d_info);
+  int r = potrf_batched(c->stream, &c->dw, &d, 1, 1, c->d_info, c->d_Binv);
   if (r) return r;
   SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (c->h_info[0] == 0) {
+    r = diag_block_inverses(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv);
+    if (r) return r;
+  }
   c->t_chol += toc(c);
   c->n_chol++;
   return c->h_info[0];
@@ -580,7 +586,7 @@ int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* h, int refine) {
   tic(c);
   SDPA_HIP_CHECK(hipMemcpyAsync(c->d_dy, c->d_g, sizeof(double) * c->m, hipMemcpyDeviceToDevice,
                                 c->stream));
-  int r = chol_solve(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_dy);
+  int r = chol_solve(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv, c->d_dy);
   c->t_solve += toc(c);
   c->n_solve++;
   return r;

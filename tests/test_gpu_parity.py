@@ -44,8 +44,28 @@ def ctx_for(d):
     return sa.GpuContext(0).load(d)
 
 
+def sparse_mixed(n=30, m=60, nnz_per=60, seed=5):
+    """Sparse constraints with 1..nnz_per entries: F1, F2 and F3 owners, many
+    heavy lists (more than 8 entries) and heavy-heavy pairs."""
+    rng = np.random.default_rng(seed)
+    recs = []
+    for k in range(1, m + 1):
+        seen = set()
+        for _ in range(int(rng.integers(1, nnz_per + 1))):
+            a, b = sorted(rng.integers(0, n, 2))
+            if (a, b) not in seen:
+                seen.add((a, b))
+                recs.append((k, 1, a + 1, b + 1, rng.standard_normal()))
+    for a in range(n):
+        recs.append((0, 1, a + 1, a + 1, 1.0))
+    K, L, I, J, V = PR._records(recs)
+    return PR._make(m, [n], np.ones(m), K, L, I, J, V, f"sparse_mixed{n}_{m}")
+
+
 CASES = {
     "maxcut100": lambda: PR.maxcut(100, 0.1, 1),
+    "maxcut60_trace": lambda: PR.maxcut(60, 0.2, 4, trace_row=True),
+    "sparse_mixed": sparse_mixed,
     "lovasz40": lambda: PR.lovasz(40, 0.3, 5),
     "dense_n20_m30": lambda: PR.dense_random(n=20, m=30, seed=2),
     "dense_n70_m40": lambda: PR.dense_random(n=70, m=40, seed=6),
@@ -70,7 +90,7 @@ def test_bmat_matches_oracle(name):
     print(name, "formulas", g.formula_counts())
 
 
-@pytest.mark.parametrize("m", [2, 5, 64, 65, 130, 300])
+@pytest.mark.parametrize("m", [2, 5, 64, 65, 130, 300, 700])
 def test_factor_and_solve_match_oracle(m):
     rng = np.random.default_rng(m)
     d = PR.maxcut(m, 0.5, seed=3)  # any problem with this m; only bMat is used
