@@ -57,12 +57,15 @@ int gemm(hipStream_t s, DenseWork* w, const double* A, int lda, const double* B,
 int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitems, int mode,
                   int* d_info, double* dinv_out = nullptr);
 // W = L^-1 for a batch of lower factors (upper parts of W zeroed).
+// dinv_pre (optional): the 64x64 diagonal inverses potrf_batched produced for
+// the same batch (slot (item, k0/64)); otherwise they are computed here.
 int trtri_batched(hipStream_t s, DenseWork* w, const MatDesc* h_L, const MatDesc* h_W,
-                  int nitems);
+                  int nitems, const double* dinv_pre = nullptr);
 // Inverses of the 256x256 diagonal blocks of the m x m lower factor L into
-// Winv (ceil(m/256) blocks of 256*256 doubles, ld 256).
+// Winv (ceil(m/256) blocks of 256*256 doubles, ld 256).  dinv64 (optional):
+// potrf_batched's 64x64 diagonal inverses of L.
 int diag_block_inverses(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
-                        double* Winv);
+                        double* Winv, const double* dinv64 = nullptr);
 // x := L^-T L^-1 x (two triangular solves with the lower factor L, m x m),
 // using the block inverses from diag_block_inverses.
 int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, const double* Winv,
@@ -77,6 +80,11 @@ struct DenseWork {
   size_t cap_dinv = 0;
   double* d_vec = nullptr;   // m-vector scratch of the Schur solves
   size_t cap_vec = 0;
+  // look-ahead Cholesky: side stream (default priority) and per-panel events
+  hipStream_t side = nullptr;
+  std::vector<hipEvent_t> evp, evn;
+  hipEvent_t evjoin = nullptr;
+  int ensure_side(int npanels);
   int ensure_items(int n);
   int ensure_vec(size_t n);
   int ensure_mats(int n);

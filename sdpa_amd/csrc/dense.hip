@@ -177,6 +177,15 @@ int DenseWork::ensure_vec(size_t n) {
   return 0;
 }
 void DenseWork::release() {
+  if (side) (void)hipStreamSynchronize(side);
+  for (auto e : evp) (void)hipEventDestroy(e);
+  for (auto e : evn) (void)hipEventDestroy(e);
+  evp.clear();
+  evn.clear();
+  if (evjoin) (void)hipEventDestroy(evjoin);
+  evjoin = nullptr;
+  if (side) (void)hipStreamDestroy(side);
+  side = nullptr;
   if (d_vec) (void)hipFree(d_vec);
   d_vec = nullptr; cap_vec = 0;
   if (d_items) (void)hipFree(d_items);
@@ -198,20 +207,44 @@ int gemm(hipStream_t s, DenseWork* w, const double* A, int lda, const double* B,
 
 // ---------------------------------------------------------------------------
 // Cholesky of the kb x kb diagonal block at (k0, k0) of every matrix in the
-// batch, and the inverse of that factor into dinv (64x64, zero padded) at
-// slot (item, k0/64).  One 256-thread work-group per item, the block in LDS.
-// The 64 columns go in four 16-wide mini-panels: wave 0 factors a mini-panel
-// with the rows in registers and cross-lane broadcasts by readlane (no LDS
-// round trip per column); then all four waves apply the mini-panel to the
-// trailing block.  The inverse is built the same way: the four 16x16
-// diagonal inverses by forward substitution, then block rows
-// W_ij = -W_ii sum_k L_ik W_kj.  Pivot rules: mode 0 = dpotrf (d > 0 else
-// fail), mode 1 = rATL_dpotrfL (d < -1e-6 fails, d < 1e-14 becomes 1e100).
-// Rows/columns beyond kb are padded with the identity.
+// batch, and the inverse of that factor into dinv (64x64, column-major,
+// strictly upper part zero) at slot (item, k0/64).  One 256-thread
+// work-group per item, the block in LDS.
+//
+// Factor: four 16-wide mini-panels.  Wave 0 factors a mini-panel with its
+// rows in registers (lane = row) and cross-lane broadcasts by readlane; the
+// column is scaled by rsqrt of the pivot (one transcendental per column on
+// the serial chain instead of a square root and a divide).  All four waves
+// then apply the mini-panel to the trailing block with 16x16x4 MFMAs.
+// Inverse: the four 16x16 diagonal inverses by right-looking substitution
+// (one wave, lane = (block, column)), then two doubling levels with MFMAs,
+// W21 = -W22 L21 W11, on 16x16 and then on 32x32 blocks.
+// Pivot rules: mode 0 = dpotrf (d > 0 else fail), mode 1 = rATL_dpotrfL
+// (d < -1e-6 fails, d < 1e-14 becomes 1e100).  Rows/columns beyond kb are
+// padded with the identity.
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
   return __hiloint2double(hi, lo);
+}
+
+// One 16x16 tile C = A(16 x K) * B(K x 16) from LDS.  A is column-major
+// (A[i][k] = A[i + lda k]); B[k][j] = B[k + ldb j], or B[j + ldb k] when
+// TRB (B given as the rows of its transpose).  On return lane l, element v
+// holds C[l % 16][4 v + l / 16].  K is a multiple of 8.
+template <int TRB>
+__device__ __forceinline__ v4d mma16_lds(const double* A, int lda, const double* B, int ldb,
+                                         int K) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  v4d acc0 = v4d{0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+  for (int kk = 0; kk < K; kk += 8) {
+    const int ka = kk + lk, kb2 = kk + 4 + lk;
+    const double b0 = TRB ? B[li + ldb * ka] : B[ka + ldb * li];
+    const double b1 = TRB ? B[li + ldb * kb2] : B[kb2 + ldb * li];
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, A[li + lda * ka], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, A[li + lda * kb2], acc1, 0, 0, 0);
+  }
+  return acc0 + acc1;
 }
 
 template <int MODE>
@@ -223,15 +256,18 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ 
   if (M.n <= k0) return;
   if (info[b] != 0) return;
   const int kb = min(TB, M.n - k0);
+  constexpr int SLD = 33;
   __shared__ double T[TB * TLD];
   __shared__ double W[TB * TLD];
-  __shared__ double S[16 * 17 * 3];
+  __shared__ double S[32 * SLD];
+  __shared__ double rdiag[TB];
   __shared__ int failc_s;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
   double* A = M.ptr + k0 + (size_t)M.ld * k0;
   for (int e = t; e < TB * TB; e += 256) {
     const int r = e & 63, c = e >> 6;
     T[r + TLD * c] = (r < kb && c < kb) ? A[r + (size_t)M.ld * c] : (r == c ? 1.0 : 0.0);
+    W[r + TLD * c] = 0.0;
   }
   if (t == 0) failc_s = 0;
   __syncthreads();
@@ -251,9 +287,10 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ 
         if (MODE == 1) d = (d < 1.0e-14) ? 1.0e+100 : d;
         failc = (failc == 0 && c < kb && bad) ? c + 1 : failc;
         d = failc ? 1.0 : d;
-        const double l = sqrt(d);
-        const double lrc = (r == c) ? l : v[jj] / l;
+        const double rs = rsqrt(d);
+        const double lrc = (r == c) ? d * rs : v[jj] * rs;
         v[jj] = lrc;
+        if (r == c) rdiag[c] = rs;
 #pragma unroll
         for (int kk = jj + 1; kk < 16; ++kk) v[kk] -= lrc * readlane_d(lrc, c0 + kk);
       }
@@ -263,20 +300,16 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ 
       if (r == 0) failc_s = failc;
     }
     __syncthreads();
-    // trailing update of rows/columns >= c0 + 16 (lower part)
-    const int o = c0 + 16, nt = TB - o;
-    const int npairs = nt * (nt + 1) / 2;
-    for (int e = t; e < npairs; e += 256) {
-      // decode e -> (i, j) with i >= j in the trailing block (row-major lower)
-      int i = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
-      while ((i + 1) * (i + 2) / 2 <= e) ++i;
-      while (i * (i + 1) / 2 > e) --i;
-      const int j = e - i * (i + 1) / 2;
-      const int gi = o + i, gj = o + j;
-      double acc = 0.0;
+    // trailing update of the lower 16x16 tiles (bi, bj), p < bj <= bi < 4:
+    // T[bi][bj] -= L[bi][p] L[bj][p]^T   (K = 16)
+    const int nt = 3 - p, ntiles = nt * (nt + 1) / 2;
+    for (int q = wv; q < ntiles; q += 4) {
+      int bj = p + 1, rem = q;
+      while (rem >= 4 - bj) { rem -= 4 - bj; ++bj; }
+      const int bi = bj + rem;
+      const v4d acc = mma16_lds<1>(&T[16 * bi + TLD * c0], TLD, &T[16 * bj + TLD * c0], TLD, 16);
 #pragma unroll
-      for (int kk = 0; kk < 16; ++kk) acc += T[gi + TLD * (c0 + kk)] * T[gj + TLD * (c0 + kk)];
-      T[gi + TLD * gj] -= acc;
+      for (int v = 0; v < 4; ++v) T[(16 * bi + li) + TLD * (16 * bj + 4 * v + lk)] -= acc[v];
     }
     __syncthreads();
   }
@@ -289,49 +322,61 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ 
     const int r = e & 63, c = e >> 6;
     if (r >= c && r < kb && c < kb) A[r + (size_t)M.ld * c] = T[r + TLD * c];
   }
-  // inverse: four 16x16 diagonal inverses (thread = block bi, column jj)
-  for (int e = t; e < TB * TB; e += 256) W[(e & 63) + TLD * (e >> 6)] = 0.0;
-  __syncthreads();
-  if (t < 64) {
-    const int bi = t >> 4, jj = t & 15, o = 16 * bi;
-    double w[16];
+  // the four 16x16 diagonal inverses: lane = (block p, column j); column j
+  // of inv(L_pp) by right-looking substitution, 1 / L_ii = rdiag[i]
+  if (wv == 0) {
+    const int o = 16 * lk, j = li;
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = (i == j) ? 1.0 : 0.0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-      double acc = (i == jj) ? 1.0 : 0.0;
+      const double wi = acc[i] * rdiag[o + i];
+      acc[i] = wi;
 #pragma unroll
-      for (int k = 0; k < i; ++k) acc -= T[(o + i) + TLD * (o + k)] * w[k];
-      w[i] = (i >= jj) ? acc / T[(o + i) + TLD * (o + i)] : 0.0;
+      for (int m = i + 1; m < 16; ++m) acc[m] -= T[(o + m) + TLD * (o + i)] * wi;
     }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) W[(o + i) + TLD * (o + jj)] = w[i];
+    for (int i = 0; i < 16; ++i) W[(o + i) + TLD * (o + j)] = (i >= j) ? acc[i] : 0.0;
   }
   __syncthreads();
-  // block rows: W_ij = -W_ii * (sum_{k=j}^{i-1} L_ik W_kj), j < i
-  for (int bi = 1; bi < 4; ++bi) {
-    // S_j = sum_k L_ik W_kj for all j < bi  (bi blocks of 16x16, <= 768 entries)
-    for (int e = t; e < bi * 256; e += 256) {
-      const int bj = e >> 8, a = (e >> 4) & 15, cc = e & 15;
-      double acc = 0.0;
-      for (int k = 16 * bj; k < 16 * bi; ++k)
-        acc += T[(16 * bi + a) + TLD * k] * W[k + TLD * (16 * bj + cc)];
-      S[bj * 272 + a * 17 + cc] = acc;
-    }
-    __syncthreads();
-    for (int e = t; e < bi * 256; e += 256) {
-      const int bj = e >> 8, a = (e >> 4) & 15, cc = e & 15;
-      double acc = 0.0;
+  // level 1 (16x16 blocks): W[h+1][h] = -W[h+1][h+1] (L[h+1][h] W[h][h]), h = 0, 2
+  if (wv < 2) {
+    const int h = 2 * wv;
+    double* Sw = S + 16 * wv;  // 16x16 scratch per wave, ld SLD
+    const v4d u = mma16_lds<0>(&T[16 * (h + 1) + TLD * 16 * h], TLD, &W[16 * h + TLD * 16 * h],
+                               TLD, 16);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) acc += W[(16 * bi + a) + TLD * (16 * bi + k)] * S[bj * 272 + k * 17 + cc];
-      W[(16 * bi + a) + TLD * (16 * bj + cc)] = -acc;
-    }
-    __syncthreads();
+    for (int v = 0; v < 4; ++v) Sw[li + SLD * (4 * v + lk)] = u[v];
+    __builtin_amdgcn_wave_barrier();
+    const v4d x = mma16_lds<0>(&W[16 * (h + 1) + TLD * 16 * (h + 1)], TLD, Sw, SLD, 16);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) W[(16 * (h + 1) + li) + TLD * (16 * h + 4 * v + lk)] = -x[v];
   }
+  __syncthreads();
+  // level 2 (32x32 blocks): W[B][A] = -W[B][B] (L[B][A] W[A][A]), A = 0:32, B = 32:64;
+  // wave = 16x16 output tile (ti, tj)
+  {
+    const int ti = wv & 1, tj = wv >> 1;
+    const v4d u = mma16_lds<0>(&T[(32 + 16 * ti) + TLD * 0], TLD, &W[0 + TLD * (16 * tj)], TLD, 32);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[(16 * ti + li) + SLD * (16 * tj + 4 * v + lk)] = u[v];
+    __syncthreads();
+    const v4d x = mma16_lds<0>(&W[(32 + 16 * ti) + TLD * 32], TLD, &S[0 + SLD * (16 * tj)], SLD, 32);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) W[(32 + 16 * ti + li) + TLD * (16 * tj + 4 * v + lk)] = -x[v];
+  }
+  __syncthreads();
   double* Dv = dinv + ((size_t)b * nslot + k0 / TB) * TB * TB;
   for (int e = t; e < TB * TB; e += 256) Dv[e] = W[(e & 63) + TLD * (e >> 6)];
 }
 
-// Rows below the diagonal block: L21 := A21 * inv(L11)^T, in place, 64 rows
-// per work-group.  grid = (ceil(maxRows/64), nitems).
+// Rows below the diagonal block: L21 := A21 * inv(L11)^T, in place.  One
+// work-group per 16 rows; wave w computes output columns [16 w, 16 w + 16)
+// with K = 64, operands loaded straight into registers (A21 rows from the
+// matrix, inv(L11) from dinv).  All waves finish their MFMAs (so every load
+// of the 16 rows has been consumed) before any wave overwrites them.
+// grid = (ceil(maxRows/16), nitems).
 __global__ __launch_bounds__(256) void k_trsm_panel(const MatDesc* __restrict__ mats, int k0,
                                                     const int* __restrict__ info,
                                                     const double* __restrict__ dinv, int nslot) {
@@ -339,81 +384,179 @@ __global__ __launch_bounds__(256) void k_trsm_panel(const MatDesc* __restrict__ 
   const MatDesc M = mats[b];
   if (M.n <= k0 + TB) return;
   if (info[b] != 0) return;
-  const int kb = TB;  // a full 64-wide panel exists whenever rows remain below it
-  const int rows = M.n - k0 - kb;
-  const int rb = blockIdx.x * TB;
-  if (rb >= rows) return;
-  __shared__ double As[TB * TLD];
-  __shared__ double Bs[TB * TLD];
-  double* A21 = M.ptr + (k0 + kb + rb) + (size_t)M.ld * k0;
+  const int rows = M.n - k0 - TB;
+  const int r0 = blockIdx.x * 16;
+  if (r0 >= rows) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  double* A21 = M.ptr + (k0 + TB) + (size_t)M.ld * k0;
   const double* Dv = dinv + ((size_t)b * nslot + k0 / TB) * TB * TB;
-  for (int e = threadIdx.x; e < TB * TB; e += 256) {
-    int r = e & 63, c = e >> 6;
-    As[r + TLD * c] = (rb + r < rows) ? A21[r + (size_t)M.ld * c] : 0.0;
-    // B = inv(L11)^T : B[k][c] = Dv[c][k]
-    Bs[c + TLD * r] = Dv[r + TB * c];
+  const double* pa = A21 + min(r0 + li, rows - 1) + (size_t)M.ld * lk;
+  const double* pw = Dv + (16 * w + li) + TB * lk;
+  double av[16], wv[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    av[s] = pa[(size_t)M.ld * 4 * s];
+    wv[s] = pw[TB * 4 * s];
   }
+  v4d acc0 = v4d{0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+#pragma unroll
+  for (int s = 0; s < 16; s += 2) {
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(wv[s], av[s], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(wv[s + 1], av[s + 1], acc1, 0, 0, 0);
+  }
+  const v4d acc = acc0 + acc1;
   __syncthreads();
-  v4d acc[2][2];
-  acc_zero(acc);
-  mma_tile(acc, As, TLD, Bs, TLD, TB);
+  if (r0 + li < rows) {
 #pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
+    for (int v = 0; v < 4; ++v) A21[(r0 + li) + (size_t)M.ld * (16 * w + 4 * v + lk)] = acc[v];
+  }
+}
+
+// Update C[r, c] -= sum_{k < kw} L[r, k0 + k] L[c, k0 + k] for columns c in
+// [cb, ce) and rows c <= r < n: the update of the rest of a panel after an
+// inner 64-column step (kw = 64), or the trailing update of an outer panel
+// (kw = 256), whole or split into the next panel's columns and the rest.
+// One wave per (16 TM) x (16 TN) output tile, operands loaded straight from
+// the matrix into registers (16 consecutive rows of one column per 16-lane
+// group: four 128-byte segments per load), D k-steps of loads in flight; no
+// LDS, no barrier.  Rows beyond the matrix are clamped on load and masked on
+// store.  grid = (lower tiles, nitems): blockIdx.x enumerates the tiles
+// column by column, rows from the diagonal down.
+template <int TM, int TN, int D>
+__global__ __launch_bounds__(64) void k_syrk_update(const MatDesc* __restrict__ mats, int k0,
+                                                    int kw, int cb, int ce,
+                                                    const int* __restrict__ info) {
+  constexpr int BMt = 16 * TM, BNt = 16 * TN;
+  const int b = blockIdx.y;
+  const MatDesc M = mats[b];
+  if (info[b] != 0) return;
+  const int rows = M.n - cb;
+  const int cend = min(ce, M.n) - cb;
+  if (rows <= 0 || cend <= 0) return;
+  const int R = (rows + BMt - 1) / BMt, Cn = (cend + BNt - 1) / BNt;
+  int g = blockIdx.x, i = 0, j = 0;
+  for (; j < Cn; ++j) {
+    const int cnt = R - (j * BNt) / BMt;
+    if (g < cnt) { i = (j * BNt) / BMt + g; break; }
+    g -= cnt;
+  }
+  if (j >= Cn) return;
+  const int wm = i * BMt, wn = j * BNt;
+  if (wm + BMt - 1 < wn) return;
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const size_t ld = M.ld;
+  const double* P = M.ptr + cb + ld * k0;  // L[cb.., k0..]
+  const double* pr[TM];
+  const double* pc[TN];
 #pragma unroll
-    for (int tj = 0; tj < 2; ++tj)
+  for (int a = 0; a < TM; ++a) pr[a] = P + min(wm + 16 * a + li, rows - 1) + ld * lk;
+#pragma unroll
+  for (int a = 0; a < TN; ++a) pc[a] = P + min(wn + 16 * a + li, cend - 1) + ld * lk;
+  v4d acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int c = 0; c < TN; ++c) acc[a][c] = v4d{0.0, 0.0, 0.0, 0.0};
+  double ra[D][TM], ca[D][TN];
+  const int ns = kw >> 2;
+  const size_t st = ld * 4;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const size_t off = st * (d < ns ? d : 0);
+#pragma unroll
+    for (int a = 0; a < TM; ++a) ra[d][a] = pr[a][off];
+#pragma unroll
+    for (int a = 0; a < TN; ++a) ca[d][a] = pc[a][off];
+  }
+  for (int s0 = 0; s0 < ns; s0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int s = s0 + d;
+      if (s < ns) {
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int c = 0; c < TN; ++c)
+            acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[d][c], ra[d][a], acc[a][c], 0, 0, 0);
+        if (s + D < ns) {
+          const size_t off = st * (s + D);
+#pragma unroll
+          for (int a = 0; a < TM; ++a) ra[d][a] = pr[a][off];
+#pragma unroll
+          for (int a = 0; a < TN; ++a) ca[d][a] = pc[a][off];
+        }
+      }
+    }
+  }
+  double* C = M.ptr + cb + ld * cb;
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int c = 0; c < TN; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int r, c;
-        acc_pos(ti, tj, v, r, c);
-        if (rb + r < rows) A21[r + (size_t)M.ld * c] = acc[ti][tj][v];
+        const int gr = wm + 16 * a + li, gc = wn + 16 * c + 4 * v + lk;
+        if (gr < rows && gc < cend && gr >= gc) C[gr + ld * gc] -= acc[a][c][v];
       }
 }
 
-// Update C[r, c] -= sum_{k < kw} L[r, k0 + k] L[c, k0 + k] for rows r >= c
-// (both >= off = k0 + kw, r < n) and columns c < colEnd: the trailing update
-// of an outer panel (kw = 256, colEnd = n) or the update of the rest of the
-// panel after an inner 64-column step (kw = 64, colEnd = panel end).
-// 64 x 64 output tile per work-group, k in LDS chunks of 64.
-// grid = (row tiles, column tiles, nitems).
-__global__ __launch_bounds__(256) void k_syrk_update(const MatDesc* __restrict__ mats, int k0,
-                                                     int kw, int colEnd,
-                                                     const int* __restrict__ info) {
-  const int b = blockIdx.z;
-  const MatDesc M = mats[b];
-  if (info[b] != 0) return;
-  const int off = k0 + kw;
-  const int rows = M.n - off;
-  const int cend = min(colEnd, M.n) - off;
-  const int bm = blockIdx.x * TB, bn = blockIdx.y * TB;
-  if (rows <= 0 || bm >= rows || bn >= cend || bm + TB - 1 < bn) return;
-  __shared__ double As[TB * TLD];
-  __shared__ double Bs[TB * TLD];
-  const double* P = M.ptr + off + (size_t)M.ld * k0;  // L[off.., k0..]
-  v4d acc[2][2];
-  acc_zero(acc);
-  for (int kc = 0; kc < kw; kc += TB) {
-    for (int e = threadIdx.x; e < TB * TB; e += 256) {
-      const int r = e & 63, k = e >> 6;
-      const double* col = P + (size_t)M.ld * (kc + k);
-      As[r + TLD * k] = (bm + r < rows) ? col[bm + r] : 0.0;
-      Bs[k + TLD * r] = (bn + r < cend) ? col[bn + r] : 0.0;
-    }
-    __syncthreads();
-    mma_tile(acc, As, TLD, Bs, TLD, TB);
-    __syncthreads();
+template <int TM, int TN, int D>
+static void launch_syrk_t(hipStream_t s, const MatDesc* d_mats, const int* d_info, int nitems,
+                          int rows, int cols, int k0, int kw, int cb, int ce) {
+  constexpr int BMt = 16 * TM, BNt = 16 * TN;
+  const int R = (rows + BMt - 1) / BMt, Cn = (cols + BNt - 1) / BNt;
+  int ntiles = 0;
+  for (int j = 0; j < Cn; ++j) ntiles += R - (j * BNt) / BMt;
+  hipLaunchKernelGGL((k_syrk_update<TM, TN, D>), dim3(ntiles, nitems), dim3(64), 0, s, d_mats, k0,
+                     kw, cb, ce, d_info);
+}
+
+// Tile shapes measured on MI355X (FP64): 32x64 tiles with 3 k-steps in flight
+// for the wide trailing updates; 32x32 tiles (more waves, deeper prefetch)
+// when the column range is at most one outer panel.
+static void launch_syrk(hipStream_t s, const MatDesc* d_mats, const int* d_info, int nitems,
+                        int maxn, int k0, int kw, int cb, int ce) {
+  const int rows = maxn - cb, cols = std::min(ce, maxn) - cb;
+  if (rows <= 0 || cols <= 0) return;
+  if (cols > 256)
+    launch_syrk_t<2, 4, 3>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce);
+  else if (kw > 64)
+    launch_syrk_t<2, 2, 8>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce);
+  else
+    launch_syrk_t<2, 2, 4>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce);
+}
+
+// Factor the outer panel [K0, K0 + PW): 64-column steps (diagonal block,
+// rows below, rest of the panel).
+static void factor_panel(hipStream_t s, DenseWork* w, int nitems, int maxn, int K0, int PW,
+                         int mode, int* d_info, double* dinv, int nslot) {
+  const int pend = std::min(K0 + PW, maxn);
+  for (int k0 = K0; k0 < pend; k0 += TB) {
+    if (mode == 0)
+      hipLaunchKernelGGL(k_potrf_diag<0>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
+                         dinv, nslot);
+    else
+      hipLaunchKernelGGL(k_potrf_diag<1>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
+                         dinv, nslot);
+    const int rows = maxn - k0 - TB;
+    if (rows <= 0) break;
+    hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 15) / 16, nitems), dim3(256), 0, s, w->d_mats,
+                       k0, d_info, dinv, nslot);
+    launch_syrk(s, w->d_mats, d_info, nitems, maxn, k0, TB, k0 + TB, K0 + PW);
   }
-  double* C = M.ptr + off + (size_t)M.ld * off;
-#pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int r, c;
-        acc_pos(ti, tj, v, r, c);
-        const int gr = bm + r, gc = bn + c;
-        if (gr < rows && gc < cend && gr >= gc) C[gr + (size_t)M.ld * gc] -= acc[ti][tj][v];
-      }
+}
+
+int DenseWork::ensure_side(int npanels) {
+  if (!side) SDPA_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  if (!evjoin) SDPA_HIP_CHECK(hipEventCreateWithFlags(&evjoin, hipEventDisableTiming));
+  while ((int)evp.size() < npanels) {
+    hipEvent_t a, b;
+    SDPA_HIP_CHECK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+    SDPA_HIP_CHECK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+    evp.push_back(a);
+    evn.push_back(b);
+  }
+  return 0;
 }
 
 int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitems, int mode,
@@ -432,33 +575,36 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
                                 hipMemcpyHostToDevice, s));
   SDPA_HIP_CHECK(hipMemsetAsync(d_info, 0, sizeof(int) * nitems, s));
   // outer panels of PW = 256 columns; inside, 64-column steps whose updates
-  // stay within the panel; then one trailing update with K = 256.
+  // stay within the panel; then the trailing update with K = 256.
   constexpr int PW = 256;
-  for (int K0 = 0; K0 < maxn; K0 += PW) {
-    const int pend = std::min(K0 + PW, maxn);
-    for (int k0 = K0; k0 < pend; k0 += TB) {
-      if (mode == 0)
-        hipLaunchKernelGGL(k_potrf_diag<0>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
-                           dinv, nslot);
-      else
-        hipLaunchKernelGGL(k_potrf_diag<1>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
-                           dinv, nslot);
-      const int rows = maxn - k0 - TB;
-      if (rows <= 0) break;
-      const int rtiles = (rows + TB - 1) / TB;
-      hipLaunchKernelGGL(k_trsm_panel, dim3(rtiles, nitems), dim3(256), 0, s, w->d_mats, k0,
-                         d_info, dinv, nslot);
-      const int pcols = pend - (k0 + TB);
-      if (pcols > 0)
-        hipLaunchKernelGGL(k_syrk_update, dim3(rtiles, (pcols + TB - 1) / TB, nitems), dim3(256), 0,
-                           s, w->d_mats, k0, TB, K0 + PW, d_info);
+  const int np = (maxn + PW - 1) / PW;
+  if (nitems > 1 || np <= 2) {
+    for (int K0 = 0; K0 < maxn; K0 += PW) {
+      factor_panel(s, w, nitems, maxn, K0, PW, mode, d_info, dinv, nslot);
+      launch_syrk(s, w->d_mats, d_info, nitems, maxn, K0, PW, K0 + PW, maxn);
     }
-    const int trail = maxn - (K0 + PW);
-    if (trail > 0) {
-      const int tt = (trail + TB - 1) / TB;
-      hipLaunchKernelGGL(k_syrk_update, dim3(tt, tt, nitems), dim3(256), 0, s, w->d_mats, K0, PW,
-                         maxn, d_info);
+  } else {
+    // Look-ahead: panel factorizations on `s`, trailing updates on a side
+    // stream.  The update of panel p+1's columns goes first, so panel p+1 can
+    // be factored while the rest of panel p's trailing update runs; all
+    // trailing updates stay in order on the side stream (no overlapping
+    // read-modify-writes), and panel p+1's columns are disjoint from the rest.
+    if (int e = w->ensure_side(np)) return e;
+    SDPA_HIP_CHECK(hipEventRecord(w->evjoin, s));
+    SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evjoin, 0));
+    for (int p = 0; p < np; ++p) {
+      const int K0 = p * PW;
+      if (p > 0) SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));
+      factor_panel(s, w, 1, maxn, K0, PW, mode, d_info, dinv, nslot);
+      if (K0 + PW >= maxn) break;
+      SDPA_HIP_CHECK(hipEventRecord(w->evp[p], s));
+      SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
+      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + PW, K0 + 2 * PW);
+      SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
+      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, maxn);
     }
+    SDPA_HIP_CHECK(hipEventRecord(w->evjoin, w->side));
+    SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evjoin, 0));
   }
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
@@ -558,22 +704,54 @@ __global__ __launch_bounds__(256) void k_trtri_row(const MatDesc* __restrict__ L
       }
 }
 
+// Step 1 when the factorization already produced the 64x64 diagonal
+// inverses (potrf_batched's dinv, same slot layout): place them on W's
+// diagonal and zero the strictly upper blocks of the block row.
+// grid = (nblk, nitems).
+__global__ __launch_bounds__(256) void k_diag_place(const MatDesc* __restrict__ L,
+                                                    const MatDesc* __restrict__ W, int nblk_max,
+                                                    const double* __restrict__ dinv) {
+  const int b = blockIdx.y, I = blockIdx.x;
+  const MatDesc Lm = L[b], Wm = W[b];
+  const int k0 = I * TB;
+  if (k0 >= Lm.n) return;
+  const int kb = min(TB, Lm.n - k0);
+  const double* Dv = dinv + ((size_t)b * nblk_max + I) * TB * TB;
+  double* Wd = Wm.ptr + k0 + (size_t)Wm.ld * k0;
+  for (int e = threadIdx.x; e < TB * TB; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    if (r < kb && c < kb) Wd[r + (size_t)Wm.ld * c] = Dv[r + TB * c];
+  }
+  const int ncol = Wm.n - (k0 + kb);
+  for (int e = threadIdx.x; e < kb * ncol; e += 256) {
+    const int r = e % kb, c = e / kb;
+    Wm.ptr[k0 + r + (size_t)Wm.ld * (k0 + kb + c)] = 0.0;
+  }
+}
+
 int trtri_batched(hipStream_t s, DenseWork* w, const MatDesc* h_L, const MatDesc* h_W,
-                  int nitems) {
+                  int nitems, const double* dinv_pre) {
   if (nitems <= 0) return 0;
   int maxn = 0;
   for (int b = 0; b < nitems; ++b) maxn = h_L[b].n > maxn ? h_L[b].n : maxn;
   const int nblk = (maxn + TB - 1) / TB;
   if (int e = w->ensure_mats(2 * nitems)) return e;
-  if (int e = w->ensure_dinv((size_t)nitems * nblk * TB * TB)) return e;
   SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats, h_L, sizeof(MatDesc) * nitems, hipMemcpyHostToDevice, s));
   SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats + nitems, h_W, sizeof(MatDesc) * nitems,
                                 hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_diag_inv, dim3(nblk, nitems), dim3(64), 0, s, w->d_mats,
-                     w->d_mats + nitems, nblk, w->d_dinv);
+  const double* dv = dinv_pre;
+  if (dv) {
+    hipLaunchKernelGGL(k_diag_place, dim3(nblk, nitems), dim3(256), 0, s, w->d_mats,
+                       w->d_mats + nitems, nblk, dv);
+  } else {
+    if (int e = w->ensure_dinv((size_t)nitems * nblk * TB * TB)) return e;
+    hipLaunchKernelGGL(k_diag_inv, dim3(nblk, nitems), dim3(64), 0, s, w->d_mats,
+                       w->d_mats + nitems, nblk, w->d_dinv);
+    dv = w->d_dinv;
+  }
   for (int I = 1; I < nblk; ++I)
     hipLaunchKernelGGL(k_trtri_row, dim3(I, nitems), dim3(256), 0, s, w->d_mats,
-                       w->d_mats + nitems, I, nblk, w->d_dinv);
+                       w->d_mats + nitems, I, nblk, dv);
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -590,7 +768,7 @@ int trtri_batched(hipStream_t s, DenseWork* w, const MatDesc* h_L, const MatDesc
 constexpr int SB = 256;
 
 int diag_block_inverses(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
-                        double* Winv) {
+                        double* Winv, const double* dinv64) {
   const int nb = (m + SB - 1) / SB;
   std::vector<MatDesc> Ls(nb), Ws(nb);
   for (int b = 0; b < nb; ++b) {
@@ -598,7 +776,9 @@ int diag_block_inverses(hipStream_t s, DenseWork* w, const double* L, int ld, in
     Ls[b] = MatDesc{const_cast<double*>(L) + k0 + (size_t)ld * k0, std::min(SB, m - k0), ld};
     Ws[b] = MatDesc{Winv + (size_t)b * SB * SB, std::min(SB, m - k0), SB};
   }
-  return trtri_batched(s, w, Ls.data(), Ws.data(), nb);
+  // the 64x64 inverses of the factorization (slot k0/64) are slot
+  // (256-block b, 64-block I) = 4 b + I of the 256-block batch
+  return trtri_batched(s, w, Ls.data(), Ws.data(), nb, dinv64);
 }
 
 // Solve step, part 1: x_b = W_b y_b (trans 0) or W_b^T y_b (trans 1), by

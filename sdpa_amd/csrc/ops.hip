@@ -403,7 +403,9 @@ int inverse_chol(Ctx* c) {
     for (int b = 0; b < 2 * nb; ++b)
       if (c->h_info[b] != 0) fail = 1;
     if (fail) return 1;
-    if (int e = trtri_batched(c->stream, &c->dw, L.data(), W.data(), 2 * nb)) return e;
+    // reuse the 64x64 diagonal inverses the factorization left in dw.d_dinv
+    if (int e = trtri_batched(c->stream, &c->dw, L.data(), W.data(), 2 * nb, c->dw.d_dinv))
+      return e;
     // Zinv = invCholZ^T * invCholZ
     std::vector<GemmItem> it(nb);
     int maxn = 0;

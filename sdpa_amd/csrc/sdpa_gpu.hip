@@ -571,7 +571,7 @@ int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* h, int precision) {
   SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
   if (c->h_info[0] == 0) {
-    r = diag_block_inverses(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv);
+    r = diag_block_inverses(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv, c->d_Binv);
     if (r) return r;
   }
   c->t_chol += toc(c);
