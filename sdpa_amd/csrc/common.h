@@ -84,6 +84,7 @@ struct DenseWork {
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> evp, evn;
   hipEvent_t evjoin = nullptr;
+  int side_waves = 0;  // grid bound of the side stream's updates
   int ensure_side(int npanels);
   int ensure_items(int n);
   int ensure_vec(size_t n);

@@ -212,13 +212,15 @@ int gemm(hipStream_t s, DenseWork* w, const double* A, int lda, const double* B,
 // work-group per item, the block in LDS.
 //
 // Factor: four 16-wide mini-panels.  Wave 0 factors a mini-panel with its
-// rows in registers (lane = row) and cross-lane broadcasts by readlane; the
-// column is scaled by rsqrt of the pivot (one transcendental per column on
-// the serial chain instead of a square root and a divide).  All four waves
-// then apply the mini-panel to the trailing block with 16x16x4 MFMAs.
-// Inverse: the four 16x16 diagonal inverses by right-looking substitution
-// (one wave, lane = (block, column)), then two doubling levels with MFMAs,
-// W21 = -W22 L21 W11, on 16x16 and then on 32x32 blocks.
+// rows in registers (lane = row): the pivot comes by readlane, the column is
+// scaled by rsqrt of the pivot (one transcendental per column on the serial
+// chain instead of a square root and a divide), and the entries the rank-1
+// update needs are read back from LDS as broadcasts.  All four waves then
+// apply the mini-panel to the trailing block with 16x16x4 MFMAs.
+// Inverse: the 16x16 diagonal inverses by right-looking substitution and
+// the doubling W21 = -W22 L21 W11, first on 16x16 blocks, then on 32x32
+// blocks (MFMAs); the waves idle during a mini-panel factor compute the
+// inverses of the blocks already final.
 // Pivot rules: mode 0 = dpotrf (d > 0 else fail), mode 1 = rATL_dpotrfL
 // (d < -1e-6 fails, d < 1e-14 becomes 1e100).  Rows/columns beyond kb are
 // padded with the identity.
@@ -247,57 +249,148 @@ __device__ __forceinline__ v4d mma16_lds(const double* A, int lda, const double*
   return acc0 + acc1;
 }
 
+// Column j (lane % 16) of inv(L_oo), the 16x16 diagonal block of T at
+// (o, o), by right-looking substitution (1 / L_ii = rdiag[o + i]), into W.
+__device__ __forceinline__ void diag16_inverse(const double* T, const double* rdiag, double* W,
+                                               int o) {
+  const int lane = threadIdx.x & 63, j = lane & 15;
+  double acc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const double wi = acc[i] * rdiag[o + i];
+    acc[i] = wi;
+#pragma unroll
+    for (int m = i + 1; m < 16; ++m) acc[m] -= T[(o + m) + TLD * (o + i)] * wi;
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) W[(o + i) + TLD * (o + j)] = (i >= j) ? acc[i] : 0.0;
+  }
+}
+
+// W[h+1][h] = -W[h+1][h+1] (L[h+1][h] W[h][h]) on 16x16 blocks, one wave;
+// Sw = 16x16 scratch (ld 17) private to the wave.
+__device__ __forceinline__ void offdiag16_inverse(const double* T, double* W, double* Sw, int h) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const v4d u = mma16_lds<0>(&T[16 * (h + 1) + TLD * 16 * h], TLD, &W[16 * h + TLD * 16 * h],
+                             TLD, 16);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Sw[li + 17 * (4 * v + lk)] = u[v];
+  __builtin_amdgcn_wave_barrier();
+  const v4d x = mma16_lds<0>(&W[16 * (h + 1) + TLD * 16 * (h + 1)], TLD, Sw, 17, 16);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) W[(16 * (h + 1) + li) + TLD * (16 * h + 4 * v + lk)] = -x[v];
+}
+
+// The diagonal-block work of one item (256 threads).  kp >= 0: first apply
+// the update by the 64 columns [kp, kp + 64) of L to the block,
+// A[k0.., k0..] -= L[k0.., kp..] L[k0.., kp..]^T (the block's share of the
+// previous step's rest-of-panel update).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ mats, int k0,
-                                                    int* __restrict__ info,
-                                                    double* __restrict__ dinv, int nslot) {
-  const int b = blockIdx.x;
-  const MatDesc M = mats[b];
-  if (M.n <= k0) return;
-  if (info[b] != 0) return;
+__device__ __forceinline__ void diag_block(const MatDesc& M, int b, int k0, int kp,
+                                           int* __restrict__ info, double* __restrict__ dinv,
+                                           int nslot) {
   const int kb = min(TB, M.n - k0);
   constexpr int SLD = 33;
   __shared__ double T[TB * TLD];
   __shared__ double W[TB * TLD];
   __shared__ double S[32 * SLD];
+  __shared__ double S1[16 * 17];
+  __shared__ double S2[16 * 17];
   __shared__ double rdiag[TB];
   __shared__ int failc_s;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
   double* A = M.ptr + k0 + (size_t)M.ld * k0;
-  for (int e = t; e < TB * TB; e += 256) {
-    const int r = e & 63, c = e >> 6;
-    T[r + TLD * c] = (r < kb && c < kb) ? A[r + (size_t)M.ld * c] : (r == c ? 1.0 : 0.0);
-    W[r + TLD * c] = 0.0;
+  {
+    // a thread's 16 loads in flight at once, then the LDS stores
+    double a[16];
+    const int r = t & 63, cq = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cq + 4 * q;
+      a[q] = (r < kb && c < kb) ? A[r + (size_t)M.ld * c] : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[r + TLD * (cq + 4 * q)] = a[q];
+    const double* X = M.ptr + k0 + (size_t)M.ld * (kp < 0 ? 0 : kp);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = (kp >= 0 && r < kb) ? X[r + (size_t)M.ld * (cq + 4 * q)] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = a[q];
   }
   if (t == 0) failc_s = 0;
   __syncthreads();
+  if (kp >= 0) {
+    // T[bi][bj] -= X[bi] X[bj]^T on the ten lower 16x16 tiles (K = 64)
+    for (int q = wv; q < 10; q += 4) {
+      int bj = 0, rem = q;
+      while (rem >= 4 - bj) { rem -= 4 - bj; ++bj; }
+      const int bi = bj + rem;
+      const v4d acc = mma16_lds<1>(&W[16 * bi], TLD, &W[16 * bj], TLD, 64);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) T[(16 * bi + li) + TLD * (16 * bj + 4 * v + lk)] -= acc[v];
+    }
+    __syncthreads();
+    const int r = t & 63, cq = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = 0.0;
+    __syncthreads();
+  }
+#pragma unroll 1
   for (int p = 0; p < 4; ++p) {
     const int c0 = 16 * p;
     if (wv == 0) {
+      // lane = row.  Serial chain per column: pivot -> rsqrt -> the next
+      // pivot, kept in uniform registers: d_{c+1} = a - l^2 with a the entry
+      // (c+1, c+1) before column c's update and l = L[c+1][c] (the same fma
+      // the vector update does at lane c+1).  The entries of column c for the
+      // next two rows come by readlane; the later rows' multipliers are read
+      // back from LDS as broadcasts, off the chain.  A failed pivot lets
+      // NaN/Inf through; the block is then reported and discarded.
       const int r = lane;
       double v[16];
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) v[jj] = T[r + TLD * (c0 + jj)];
       int failc = failc_s;
+      double dcur = readlane_d(v[0], c0);
+      double anext = readlane_d(v[1], c0 + 1);
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         const int c = c0 + jj;
-        double d = readlane_d(v[jj], c);
-        const bool bad = (MODE == 0) ? !(d > 0.0) : (d < -1.0e-6);
-        if (MODE == 1) d = (d < 1.0e-14) ? 1.0e+100 : d;
+        const bool bad = (MODE == 0) ? !(dcur > 0.0) : (dcur < -1.0e-6);
         failc = (failc == 0 && c < kb && bad) ? c + 1 : failc;
-        d = failc ? 1.0 : d;
-        const double rs = rsqrt(d);
-        const double lrc = (r == c) ? d * rs : v[jj] * rs;
-        v[jj] = lrc;
-        if (r == c) rdiag[c] = rs;
+        const double dd = (MODE == 1 && dcur < 1.0e-14) ? 1.0e+100 : dcur;
+        double e1 = 0.0, e2 = 0.0;
+        if (jj + 1 < 16) e1 = readlane_d(v[jj], c + 1);
+        if (jj + 2 < 16) e2 = readlane_d(v[jj], c + 2);
+        const double rs = rsqrt(dd);
+        const double lrc = (r == c) ? dd * rs : v[jj] * rs;
+        T[r + TLD * c] = lrc;
+        if (r == 0) rdiag[c] = rs;
+        double dnext = 0.0, anext2 = 0.0;
+        if (jj + 1 < 16) {
+          const double l1 = e1 * rs;
+          v[jj + 1] = fma(-lrc, l1, v[jj + 1]);
+          dnext = fma(-l1, l1, anext);
+        }
+        if (jj + 2 < 16) {
+          const double l2 = e2 * rs;
+          v[jj + 2] = fma(-lrc, l2, v[jj + 2]);
+          anext2 = readlane_d(v[jj + 2], c + 2);
+        }
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int kk = jj + 1; kk < 16; ++kk) v[kk] -= lrc * readlane_d(lrc, c0 + kk);
+        for (int kk = jj + 3; kk < 16; ++kk) v[kk] = fma(-lrc, T[(c0 + kk) + TLD * c], v[kk]);
+        dcur = dnext;
+        anext = anext2;
       }
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj)
-        if (r >= c0 + jj) T[r + TLD * (c0 + jj)] = v[jj];
       if (r == 0) failc_s = failc;
+    } else if (wv == 1 && p >= 1) {
+      diag16_inverse(T, rdiag, W, 16 * (p - 1));  // block p-1 is final
+    } else if (wv == 2 && p == 3) {
+      offdiag16_inverse(T, W, S1, 0);  // W[1][0] from D_0, D_1, L[1][0]
     }
     __syncthreads();
     // trailing update of the lower 16x16 tiles (bi, bj), p < bj <= bi < 4:
@@ -311,57 +404,40 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ 
 #pragma unroll
       for (int v = 0; v < 4; ++v) T[(16 * bi + li) + TLD * (16 * bj + 4 * v + lk)] -= acc[v];
     }
-    __syncthreads();
+    if (p < 3) __syncthreads();
   }
   if (failc_s) {
     if (t == 0) info[b] = k0 + failc_s;
     return;
   }
   // write the factor back (lower part of the block)
-  for (int e = t; e < TB * TB; e += 256) {
-    const int r = e & 63, c = e >> 6;
-    if (r >= c && r < kb && c < kb) A[r + (size_t)M.ld * c] = T[r + TLD * c];
-  }
-  // the four 16x16 diagonal inverses: lane = (block p, column j); column j
-  // of inv(L_pp) by right-looking substitution, 1 / L_ii = rdiag[i]
-  if (wv == 0) {
-    const int o = 16 * lk, j = li;
-    double acc[16];
+  {
+    const int r = t & 63, cq = t >> 6;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const double wi = acc[i] * rdiag[o + i];
-      acc[i] = wi;
-#pragma unroll
-      for (int m = i + 1; m < 16; ++m) acc[m] -= T[(o + m) + TLD * (o + i)] * wi;
+    for (int q = 0; q < 16; ++q) {
+      const int c = cq + 4 * q;
+      if (r >= c && r < kb && c < kb) A[r + (size_t)M.ld * c] = T[r + TLD * c];
     }
+  }
+  // D_3 (wave 1) alongside U = L[B][A] W[A][A] (A = 0:32, B = 32:64; waves
+  // 0, 2, 3 share its four 16x16 tiles)
+  if (wv == 1) {
+    diag16_inverse(T, rdiag, W, 48);
+  } else {
+    for (int q = (wv == 0 ? 0 : wv - 1); q < 4; q += 3) {
+      const int ti = q & 1, tj = q >> 1;
+      const v4d u = mma16_lds<0>(&T[(32 + 16 * ti) + TLD * 0], TLD, &W[0 + TLD * (16 * tj)], TLD,
+                                 32);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) W[(o + i) + TLD * (o + j)] = (i >= j) ? acc[i] : 0.0;
+      for (int v = 0; v < 4; ++v) S[(16 * ti + li) + SLD * (16 * tj + 4 * v + lk)] = u[v];
+    }
   }
   __syncthreads();
-  // level 1 (16x16 blocks): W[h+1][h] = -W[h+1][h+1] (L[h+1][h] W[h][h]), h = 0, 2
-  if (wv < 2) {
-    const int h = 2 * wv;
-    double* Sw = S + 16 * wv;  // 16x16 scratch per wave, ld SLD
-    const v4d u = mma16_lds<0>(&T[16 * (h + 1) + TLD * 16 * h], TLD, &W[16 * h + TLD * 16 * h],
-                               TLD, 16);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Sw[li + SLD * (4 * v + lk)] = u[v];
-    __builtin_amdgcn_wave_barrier();
-    const v4d x = mma16_lds<0>(&W[16 * (h + 1) + TLD * 16 * (h + 1)], TLD, Sw, SLD, 16);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) W[(16 * (h + 1) + li) + TLD * (16 * h + 4 * v + lk)] = -x[v];
-  }
+  if (wv == 0) offdiag16_inverse(T, W, S2, 2);  // W[3][2] from D_2, D_3, L[3][2]
   __syncthreads();
-  // level 2 (32x32 blocks): W[B][A] = -W[B][B] (L[B][A] W[A][A]), A = 0:32, B = 32:64;
-  // wave = 16x16 output tile (ti, tj)
+  // W[B][A] = -W[B][B] U, wave = 16x16 output tile (ti, tj)
   {
     const int ti = wv & 1, tj = wv >> 1;
-    const v4d u = mma16_lds<0>(&T[(32 + 16 * ti) + TLD * 0], TLD, &W[0 + TLD * (16 * tj)], TLD, 32);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) S[(16 * ti + li) + SLD * (16 * tj + 4 * v + lk)] = u[v];
-    __syncthreads();
     const v4d x = mma16_lds<0>(&W[(32 + 16 * ti) + TLD * 32], TLD, &S[0 + SLD * (16 * tj)], SLD, 32);
 #pragma unroll
     for (int v = 0; v < 4; ++v) W[(32 + 16 * ti + li) + TLD * (16 * tj + 4 * v + lk)] = -x[v];
@@ -369,6 +445,18 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ 
   __syncthreads();
   double* Dv = dinv + ((size_t)b * nslot + k0 / TB) * TB * TB;
   for (int e = t; e < TB * TB; e += 256) Dv[e] = W[(e & 63) + TLD * (e >> 6)];
+}
+
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_potrf_diag(const MatDesc* __restrict__ mats, int k0,
+                                                    int* __restrict__ info,
+                                                    double* __restrict__ dinv, int nslot) {
+  const int b = blockIdx.x;
+  const MatDesc M = mats[b];
+  if (M.n <= k0) return;
+  if (info[b] != 0) return;
+  diag_block<MODE>(M, b, k0, -1, info, dinv, nslot);
 }
 
 // Rows below the diagonal block: L21 := A21 * inv(L11)^T, in place.  One
@@ -422,27 +510,15 @@ __global__ __launch_bounds__(256) void k_trsm_panel(const MatDesc* __restrict__ 
 // LDS, no barrier.  Rows beyond the matrix are clamped on load and masked on
 // store.  grid = (lower tiles, nitems): blockIdx.x enumerates the tiles
 // column by column, rows from the diagonal down.
+// One (16 TM) x (16 TN) output tile at (wm, wn) of the update
+// C[cb + r][cb + c] -= sum_{k < kw} L[cb + r][k0 + k] L[cb + c][k0 + k], by one
+// wave: operands loaded straight from the matrix into registers (16
+// consecutive rows of one column per 16-lane group: four 128-byte segments
+// per load), D k-steps of loads in flight, no LDS.  Rows beyond the matrix are
+// clamped on load and masked on store; only r >= c is written.
 template <int TM, int TN, int D>
-__global__ __launch_bounds__(64) void k_syrk_update(const MatDesc* __restrict__ mats, int k0,
-                                                    int kw, int cb, int ce,
-                                                    const int* __restrict__ info) {
-  constexpr int BMt = 16 * TM, BNt = 16 * TN;
-  const int b = blockIdx.y;
-  const MatDesc M = mats[b];
-  if (info[b] != 0) return;
-  const int rows = M.n - cb;
-  const int cend = min(ce, M.n) - cb;
-  if (rows <= 0 || cend <= 0) return;
-  const int R = (rows + BMt - 1) / BMt, Cn = (cend + BNt - 1) / BNt;
-  int g = blockIdx.x, i = 0, j = 0;
-  for (; j < Cn; ++j) {
-    const int cnt = R - (j * BNt) / BMt;
-    if (g < cnt) { i = (j * BNt) / BMt + g; break; }
-    g -= cnt;
-  }
-  if (j >= Cn) return;
-  const int wm = i * BMt, wn = j * BNt;
-  if (wm + BMt - 1 < wn) return;
+__device__ __forceinline__ void syrk_tile(const MatDesc& M, int k0, int kw, int cb, int rows,
+                                          int cend, int wm, int wn) {
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   const size_t ld = M.ld;
   const double* P = M.ptr + cb + ld * k0;  // L[cb.., k0..]
@@ -501,53 +577,161 @@ __global__ __launch_bounds__(64) void k_syrk_update(const MatDesc* __restrict__ 
 }
 
 template <int TM, int TN, int D>
+__global__ __launch_bounds__(64) void k_syrk_update(const MatDesc* __restrict__ mats, int k0,
+                                                    int kw, int cb, int ce,
+                                                    const int* __restrict__ info) {
+  constexpr int BMt = 16 * TM, BNt = 16 * TN;
+  const int b = blockIdx.y;
+  const MatDesc M = mats[b];
+  if (info[b] != 0) return;
+  const int rows = M.n - cb;
+  const int cend = min(ce, M.n) - cb;
+  if (rows <= 0 || cend <= 0) return;
+  const int R = (rows + BMt - 1) / BMt, Cn = (cend + BNt - 1) / BNt;
+  // a bounded grid walks the tiles (persistent waves); the tiles are
+  // enumerated column by column, rows from the diagonal down
+  for (int tile = blockIdx.x;; tile += gridDim.x) {
+    int g = tile, i = 0, j = 0;
+    for (; j < Cn; ++j) {
+      const int cnt = R - (j * BNt) / BMt;
+      if (g < cnt) { i = (j * BNt) / BMt + g; break; }
+      g -= cnt;
+    }
+    if (j >= Cn) return;
+    syrk_tile<TM, TN, D>(M, k0, kw, cb, rows, cend, i * BMt, j * BNt);
+  }
+}
+
+// One step of a panel factorization, fused into one launch: work-group 0
+// factors the diagonal block at k0 after applying to it the update by the
+// previous 64 columns [kp, k0), kp = k0 - 64; the other work-groups (4 waves,
+// one 32x32 tile each) apply that update to the rest of the panel, rows
+// [k0, n) x columns [k0, pend) less the diagonal block.  The factorization's
+// serial chain then waits on one launch per step for the update and the
+// diagonal block together.  grid = (1 + ceil(tiles / 4), nitems).
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void k_panel_step(const MatDesc* __restrict__ mats, int k0,
+                                                    int pend, int* __restrict__ info,
+                                                    double* __restrict__ dinv, int nslot) {
+  const int b = blockIdx.y;
+  const MatDesc M = mats[b];
+  if (M.n <= k0) return;
+  if (info[b] != 0) return;
+  const int kp = k0 - TB;
+  if (blockIdx.x == 0) {
+    diag_block<MODE>(M, b, k0, kp, info, dinv, nslot);
+    return;
+  }
+  // 32 x 64 tiles: row tile i, column tile j, from the diagonal down
+  // (i >= 2 j); rows [0, 64) x columns [0, 64) is the diagonal block
+  const int rows = M.n - k0, cend = min(pend, M.n) - k0;
+  const int R = (rows + 31) / 32, Cn = (cend + 63) / 64;
+  int g = (blockIdx.x - 1) * 4 + (threadIdx.x >> 6);
+  for (int j = 0; j < Cn; ++j) {
+    const int i0 = j == 0 ? 2 : 2 * j;
+    const int cnt = R - i0;
+    if (cnt <= 0) continue;
+    if (g < cnt) {
+      syrk_tile<2, 4, 3>(M, kp, TB, k0, rows, cend, (i0 + g) * 32, j * 64);
+      return;
+    }
+    g -= cnt;
+  }
+}
+
+static int panel_step_tiles(int maxn, int k0, int pend) {
+  const int rows = maxn - k0, cend = std::min(pend, maxn) - k0;
+  const int R = (rows + 31) / 32, Cn = (cend + 63) / 64;
+  int n = 0;
+  for (int j = 0; j < Cn; ++j) n += std::max(0, R - (j == 0 ? 2 : 2 * j));
+  return n;
+}
+
+template <int TM, int TN, int D>
 static void launch_syrk_t(hipStream_t s, const MatDesc* d_mats, const int* d_info, int nitems,
-                          int rows, int cols, int k0, int kw, int cb, int ce) {
+                          int rows, int cols, int k0, int kw, int cb, int ce, int maxwaves) {
   constexpr int BMt = 16 * TM, BNt = 16 * TN;
   const int R = (rows + BMt - 1) / BMt, Cn = (cols + BNt - 1) / BNt;
   int ntiles = 0;
   for (int j = 0; j < Cn; ++j) ntiles += R - (j * BNt) / BMt;
+  if (maxwaves > 0 && nitems == 1) ntiles = std::min(ntiles, maxwaves);
   hipLaunchKernelGGL((k_syrk_update<TM, TN, D>), dim3(ntiles, nitems), dim3(64), 0, s, d_mats, k0,
                      kw, cb, ce, d_info);
 }
 
 // Tile shapes measured on MI355X (FP64): 32x64 tiles with 3 k-steps in flight
 // for the wide trailing updates; 32x32 tiles (more waves, deeper prefetch)
-// when the column range is at most one outer panel.
+// when the column range is at most one outer panel.  maxwaves > 0 bounds the
+// grid (persistent waves), so an update running beside the panel
+// factorization leaves no backlog of work-groups in the dispatcher.
 static void launch_syrk(hipStream_t s, const MatDesc* d_mats, const int* d_info, int nitems,
-                        int maxn, int k0, int kw, int cb, int ce) {
+                        int maxn, int k0, int kw, int cb, int ce, int maxwaves = 0) {
   const int rows = maxn - cb, cols = std::min(ce, maxn) - cb;
   if (rows <= 0 || cols <= 0) return;
   if (cols > 256)
-    launch_syrk_t<2, 4, 3>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce);
+    launch_syrk_t<2, 4, 3>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce, maxwaves);
   else if (kw > 64)
-    launch_syrk_t<2, 2, 8>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce);
+    launch_syrk_t<2, 2, 8>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce, maxwaves);
   else
-    launch_syrk_t<2, 2, 4>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce);
+    launch_syrk_t<2, 2, 4>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce, maxwaves);
 }
 
-// Factor the outer panel [K0, K0 + PW): 64-column steps (diagonal block,
-// rows below, rest of the panel).
+// Factor the outer panel [K0, K0 + PW): 64-column steps.  Step 0 is the
+// diagonal block alone; every later step is one fused launch (the previous
+// step's rest-of-panel update with the next diagonal block); each step ends
+// with the rows below its diagonal block.
 static void factor_panel(hipStream_t s, DenseWork* w, int nitems, int maxn, int K0, int PW,
                          int mode, int* d_info, double* dinv, int nslot) {
   const int pend = std::min(K0 + PW, maxn);
   for (int k0 = K0; k0 < pend; k0 += TB) {
-    if (mode == 0)
-      hipLaunchKernelGGL(k_potrf_diag<0>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
-                         dinv, nslot);
-    else
-      hipLaunchKernelGGL(k_potrf_diag<1>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
-                         dinv, nslot);
+    if (k0 == K0) {
+      if (mode == 0)
+        hipLaunchKernelGGL(k_potrf_diag<0>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
+                           dinv, nslot);
+      else
+        hipLaunchKernelGGL(k_potrf_diag<1>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
+                           dinv, nslot);
+    } else {
+      const int nwg = 1 + (panel_step_tiles(maxn, k0, K0 + PW) + 3) / 4;
+      if (mode == 0)
+        hipLaunchKernelGGL(k_panel_step<0>, dim3(nwg, nitems), dim3(256), 0, s, w->d_mats, k0,
+                           K0 + PW, d_info, dinv, nslot);
+      else
+        hipLaunchKernelGGL(k_panel_step<1>, dim3(nwg, nitems), dim3(256), 0, s, w->d_mats, k0,
+                           K0 + PW, d_info, dinv, nslot);
+    }
     const int rows = maxn - k0 - TB;
     if (rows <= 0) break;
     hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 15) / 16, nitems), dim3(256), 0, s, w->d_mats,
                        k0, d_info, dinv, nslot);
-    launch_syrk(s, w->d_mats, d_info, nitems, maxn, k0, TB, k0 + TB, K0 + PW);
   }
 }
 
+// The side stream runs the wide trailing updates.  Its CU mask (the first
+// three quarters of the mask bits) leaves the last quarter of the CUs to the
+// panel factorization on the main stream: a trailing update with thousands
+// of waves would otherwise hold every CU, and the one-work-group diagonal
+// kernel that the whole factorization waits on could not start until it
+// drained.  Measured on MI355X: with a contiguous mask and a full grid (a
+// backlog of waves on the masked CUs) kernels of another stream run at full
+// speed on the remaining CUs; masks interleaved at finer grain, or a grid
+// that leaves masked CUs partly free, let them queue behind or share CUs with
+// the update.
 int DenseWork::ensure_side(int npanels) {
-  if (!side) SDPA_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  if (!side) {
+    int dev = 0, ncu = 0;
+    SDPA_HIP_CHECK(hipGetDevice(&dev));
+    SDPA_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu >= 128) {
+      std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+      for (int i = 0; i < ncu * 3 / 4; ++i) mask[i >> 5] |= 1u << (i & 31);
+      if (hipExtStreamCreateWithCUMask(&side, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+        (void)hipGetLastError();
+        side = nullptr;
+      }
+    }
+    if (!side) SDPA_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  }
   if (!evjoin) SDPA_HIP_CHECK(hipEventCreateWithFlags(&evjoin, hipEventDisableTiming));
   while ((int)evp.size() < npanels) {
     hipEvent_t a, b;
@@ -599,9 +783,10 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
       if (K0 + PW >= maxn) break;
       SDPA_HIP_CHECK(hipEventRecord(w->evp[p], s));
       SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
-      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + PW, K0 + 2 * PW);
+      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + PW, K0 + 2 * PW,
+                  w->side_waves);
       SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
-      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, maxn);
+      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, maxn, w->side_waves);
     }
     SDPA_HIP_CHECK(hipEventRecord(w->evjoin, w->side));
     SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evjoin, 0));
