@@ -1,19 +1,35 @@
 #!/usr/bin/env python3
-"""Benchmark: time to a 1e-7 gap on the Lovasz-theta SDP (G(200, 0.3), seed 2,
-m = 5971) on one MI355X, FP64 path.
+"""Benchmark: time to a 1e-7 gap on the Lovasz-theta SDP of a seeded
+G(200, 0.3) graph (seed 2: m = 6011 constraints, one 200x200 block, dense
+6011 x 6011 Schur matrix), FP64 GPU path, one MI355X.
 
-A "step" is one complete interior-point solve (SDPA::solve, default
-parameters: epsilonStar = epsilonDash = 1e-7) of the device-resident problem;
-the problem is uploaded and analysed before the timed region.  The JSON line
-also carries the per-phase device times of one extra solve run with HIP-event
-timing, the roofline of the Schur Cholesky, the F3 bMat bandwidth, and a CPU
-baseline from the FP64 oracle on a bounded sample of the same workload.
+A "step" is one complete interior-point solve (initializeSolve + solve with
+the default parameters, epsilonStar = epsilonDash = 1e-7) of the problem,
+which is uploaded and analysed before the timed region (inputs resident in
+HBM).  The JSON line also carries:
+  - per-phase device times of one extra solve run with HIP-event timing;
+  - the F3 bMat assembly bandwidth;
+  - "roofline": the dominant kernel, the Schur Cholesky's wide trailing
+    update (k_syrk_update, MFMA-bound): algorithmic flops per launch over its
+    average launch duration, both from HIP events recorded on the side stream
+    it runs on; "traffic" = HBM bytes per launch from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a device-copy
+    calibration), or null when that summary is absent;
+  - "cpu_baseline": the FP64 CPU oracle (OpenMP) on a bounded sample of the
+    same workload (its first IPM iterations), scaled to the GPU's iteration
+    count.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config lovasz|maxcut|dense|multiblock]
+Usage:
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config lovasz|maxcut|dense|multiblock]
+  python bench.py --copy-calibration        # one 256 MiB device copy (for PMC scaling)
+  python bench.py --pmc-summary FETCH_DIR WRITE_DIR   # print the traffic summary
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import re
 import sys
 import time
 
@@ -24,6 +40,10 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+METRIC = "time-to-1e-7 gap; bMat assembly GB/s & MFMA util; mixed vs FP64 GPU, m=6000"
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_syrk_update_trailing_lovasz_m6011_traffic.txt")
+DOMINANT = "k_syrk_update<2, 4, 3>"
+COPY_BYTES = 256 << 20
 
 CONFIGS = {
     "lovasz": ("lovasz_theta_G200_p0.3_seed2", lambda P: P.lovasz(200, 0.3, 2)),
@@ -38,19 +58,102 @@ def log(msg):
 
 
 def cpu_baseline(d, gpu_iterations, sample_iters=2):
-    """FP64 oracle (OpenMP) on the same instance: `sample_iters` IPM iterations
-    timed, scaled to the GPU's iteration count."""
+    """FP64 oracle (OpenMP, all host threads it is given) on the same
+    instance: `sample_iters` IPM iterations timed, scaled to the GPU's
+    iteration count."""
     from oracle import oracle as O
-    O.build()
     P = O.Problem(d)
     t0 = time.perf_counter()
     out = P.solve(max_iter=sample_iters)
     t = time.perf_counter() - t0
-    per_it = t / max(1, out.iterations)
+    done = max(1, out.iterations)
+    per_it = t / done
     return {"value": per_it * gpu_iterations, "unit": "s", "cores": int(O.lib().orc_num_threads()),
             "kind": "port",
-            "sample": f"{out.iterations} of {gpu_iterations} IPM iterations of the same instance "
-                      f"({t:.2f} s wall, FP64 C oracle, OpenMP), scaled to {gpu_iterations}"}
+            "sample": f"the first {done} of {gpu_iterations} IPM iterations of the same instance "
+                      f"({t:.2f} s wall incl. setup, FP64 C oracle with OpenMP), time per iteration "
+                      f"x {gpu_iterations}"}
+
+
+def read_traffic():
+    """Traffic line of the committed PMC summary, if present."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                mm = re.match(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", line.strip())
+                if mm:
+                    return float(mm.group(1))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """One device-to-device copy of COPY_BYTES through the HIP runtime (a
+    blit kernel), so that a PMC run can scale FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(COPY_BYTES), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"bench: copied {COPY_BYTES} bytes device to device")
+
+
+def _pmc_rows(d):
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    rows = []
+    for fn in files:
+        with open(fn) as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def _per_dispatch(rows, counter):
+    """{dispatch id: (kernel name, value)} for one counter."""
+    out = {}
+    for r in rows:
+        if r.get("Counter_Name") != counter:
+            continue
+        did = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        name = r.get("Kernel_Name", "")
+        v = float(r.get("Counter_Value", 0.0))
+        if did in out:
+            out[did] = (name, out[did][1] + v)
+        else:
+            out[did] = (name, v)
+    return out
+
+
+def pmc_summary(fetch_dir, write_dir):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the dominant kernel,
+    scaled by what the same counters read for the calibration copy."""
+    lines = []
+    res = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        disp = _per_dispatch(_pmc_rows(d), counter)
+        dom = [v for (n, v) in disp.values() if DOMINANT in n]
+        cpy = [v for (n, v) in disp.values() if "copyBuffer" in n]
+        if not dom or not cpy:
+            raise SystemExit(f"{counter}: dominant dispatches {len(dom)}, copy dispatches {len(cpy)}")
+        copy_kib = max(cpy)  # the 256 MiB calibration copy is the largest blit
+        expect_kib = COPY_BYTES / 1024.0
+        factor = expect_kib / copy_kib
+        mean_kib = float(np.mean(dom))
+        res[counter] = mean_kib * 1024.0 * factor
+        lines.append(f"{counter}: {len(dom)} dispatches of {DOMINANT}, mean {mean_kib:.1f} KiB/dispatch; "
+                     f"calibration copy of {COPY_BYTES} B reads {copy_kib:.1f} KiB "
+                     f"(expected {expect_kib:.1f}) -> scale {factor:.4f}; "
+                     f"scaled {res[counter] / 1e6:.2f} MB/dispatch")
+    lines.append(f"traffic_bytes_per_launch: {res['FETCH_SIZE'] + res['WRITE_SIZE']:.6e}")
+    print("\n".join(lines))
 
 
 def main():
@@ -60,10 +163,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="lovasz", choices=list(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, one 256 MiB device copy (PMC scaling)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the Schur factorization would shard).", file=sys.stderr)
+              "(the design notes describe how the Schur factorization would shard).",
+              file=sys.stderr)
         sys.exit(2)
 
     import sdpa_amd
@@ -71,6 +181,7 @@ def main():
 
     name, gen = CONFIGS[args.config]
     d = gen(PR)
+    n_sdp = int(d.block_struct[0])
     log(f"bench: config {name}: m={d.m} blocks={list(d.block_struct)[:4]} nnz={d.nnz}")
     s = sdpa_amd.Solver(0).load(d).prepare()
     for w in range(args.warmup):
@@ -88,19 +199,27 @@ def main():
     s.enable_timing(True)
     rt = s.run()
     pt = s.phase_times()
+    tr_ms, tr_n, tr_flops = s.trailing_update_timing()
     s.enable_timing(False)
+    log(f"bench: timed solve: phase={rt.phase} it={rt.iterations}; trailing update {tr_n} launches")
     its = rt.iterations
     m = d.m
-    chol_ms = pt["chol"] / max(1, its)
-    chol_flops = m ** 3 / 3.0
-    chol_tflops = chol_flops / (chol_ms * 1e-3) / 1e12
     # F3 bytes per assembly (lower triangle written): 8*T + 16*sum(nnz) + 16*n^2
-    n_sdp = int(d.block_struct[0])
     bmat_bytes = 8 * m * (m + 1) / 2 + 16 * d.nnz + 16 * n_sdp * n_sdp
     bmat_ms = pt["bmat"] / max(1, its)
-    bmat_gbs = bmat_bytes / (bmat_ms * 1e-3) / 1e9
+    chol_ms = pt["chol"] / max(1, its)
+    roof = None
+    if tr_n > 0 and tr_ms > 0:
+        avg_ms = tr_ms / tr_n
+        fl = tr_flops / tr_n
+        ach = fl / (avg_ms * 1e-3) / 1e12
+        roof = {"kernel": "k_syrk_update<2,4,3> (Schur Cholesky trailing update, FP64 MFMA)",
+                "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
+                "flops_per_launch": fl, "avg_launch_us": avg_ms * 1e3, "launches": tr_n,
+                "timing": "HIP events on the side stream around each launch, one timed solve"}
     line = {
-        "metric": "time-to-1e-7 gap, Lovasz theta m=5971, FP64 GPU path",
+        "metric": METRIC,
         "value": value,
         "unit": "s",
         "n_gpus": 1,
@@ -117,19 +236,25 @@ def main():
                    "primal_error": rt.primal_error, "dual_error": rt.dual_error,
                    "primal_obj": rt.primal_obj, "dual_obj": rt.dual_obj},
         "phase_ms_per_iteration": {k: v / max(1, its) for k, v in pt.items()},
-        "bmat_f3": {"bytes": bmat_bytes, "ms": bmat_ms, "GBps": bmat_gbs,
-                    "frac_hbm": bmat_gbs / HBM_PEAK_GBS},
-        "roofline": {"kernel": "Schur Cholesky (blocked potrf, FP64 MFMA)", "bound": "mfma",
-                     "achieved": chol_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": chol_tflops / FP64_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_factorization": chol_flops, "ms_per_factorization": chol_ms},
+        "bmat_f3": {"bytes": bmat_bytes, "ms": bmat_ms, "GBps": bmat_gbs(bmat_bytes, bmat_ms),
+                    "frac_hbm": bmat_gbs(bmat_bytes, bmat_ms) / HBM_PEAK_GBS},
+        "cholesky": {"flops": m ** 3 / 3.0, "ms": chol_ms,
+                     "TFLOPs": m ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None},
+        "mixed_vs_fp64": None,
+        "roofline": roof,
     }
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(d, its)
         except Exception as e:  # the baseline is reported, never the target
             line["cpu_baseline"] = {"value": None, "error": str(e)}
+    if args.copy_calibration:
+        copy_calibration()
     print(json.dumps(line), flush=True)
+
+
+def bmat_gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9 if ms > 0 else None
 
 
 if __name__ == "__main__":

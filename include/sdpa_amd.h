@@ -46,6 +46,21 @@ class SDPA {
   void setParameterBetaBar(double v) { param_.betaBar = v; }
   void setParameterGammaStar(double v) { param_.gammaStar = v; }
   void setParameterEpsilonDash(double v) { param_.epsilonDash = v; }
+  // the reference spells this setter "Uppwer" in its examples; both work
+  void setParameterUppwerBound(double v) { param_.upperBound = v; }
+  // printf formats of printResultXVec / XMat / YMat and of the summary
+  // (Parameter::xPrint, XPrint, YPrint, infPrint; "NOPRINT" suppresses)
+  void setParameterPrintXVec(const char* f) { xPrint_ = f; }
+  void setParameterPrintXMat(const char* f) { XPrint_ = f; }
+  void setParameterPrintYMat(const char* f) { YPrint_ = f; }
+  void setParameterPrintInformation(const char* f) { infPrint_ = f; }
+  const char* getParameterPrintXVec() const { return xPrint_.c_str(); }
+  const char* getParameterPrintXMat() const { return XPrint_.c_str(); }
+  const char* getParameterPrintYMat() const { return YPrint_.c_str(); }
+  const char* getParameterPrintInformation() const { return infPrint_.c_str(); }
+  ParameterType getParameterType() const { return paramType_; }
+  // threads of the reference's OpenMP bMat; the GPU path has no use for it
+  void setNumThreads(int = 0) {}
   int getParameterMaxIteration() const { return param_.maxIteration; }
   double getParameterEpsilonStar() const { return param_.epsilonStar; }
   double getParameterLambdaStar() const { return param_.lambdaStar; }
@@ -96,7 +111,20 @@ class SDPA {
   ConeType getBlockType(int l) const;
 
   void readInput(const char* filename, FILE* fpout = nullptr, SparseType type = AUTO);
+  void readInit(const char* filename, FILE* fpout = nullptr, SparseType type = AUTO);
   void readParameter(const char* filename, FILE* fpout = nullptr);
+  void writeInputSparse(const char* filename, const char* printFormat);
+  void writeInitSparse(const char* filename, const char* printFormat);
+
+  // DIMACS errors of the final point, DimacsError[1..6] (index 0 unused)
+  void getDimacsError(double* DimacsError);
+  void printDimacsError(double* DimacsError, const char* printFormat, FILE* fpout = stdout);
+  void printResultXVec(FILE* fp = stdout);
+  void printResultXMat(FILE* fp = stdout);
+  void printResultYMat(FILE* fp = stdout);
+  void printComputationTime(FILE* fp = stdout);
+  void printParameters(FILE* fp = stdout);
+  static void printSDPAVersion(FILE* fp = stdout);
   void terminate();
   void copyCurrentToInit();
 
@@ -124,6 +152,12 @@ class SDPA {
   FILE* fpout_ = nullptr;
   bool isInitPoint_ = false;
   bool timing_ = false;
+  ParameterType paramType_ = PARAMETER_DEFAULT;
+  std::string xPrint_ = "%+8.3e", XPrint_ = "%+8.3e", YPrint_ = "%+8.3e",
+              infPrint_ = "%+10.16e";
+  void printParametersWith(FILE* fp, const char* format);
+  void printBlocks(FILE* fp, const std::vector<double>& M, const char* format);
+  void printLastInfo(FILE* fp);
   double kappa_ = 12.0;
   int m_ = 0, nBlock_ = 0;
   std::vector<int> blockStruct_;
@@ -182,6 +216,15 @@ int sdpa_solver_iterate(sdpa_solver* s, double* X, double* Z, double* y);
 /* out[0..4] ms spent in bMat, Cholesky, solves, eigen, inverses (timing on). */
 int sdpa_solver_phase_times(sdpa_solver* s, double* out5);
 sdpa_gpu_ctx* sdpa_solver_context(sdpa_solver* s);
+/* SDPA::readInit / writeInputSparse / writeInitSparse (sdpa_call.cpp:946-990,
+ * 1099-1222); the init point read is used when the solve runs with setInitPoint. */
+int sdpa_solver_read_init(sdpa_solver* s, const char* path);
+int sdpa_solver_write_input(sdpa_solver* s, const char* path, const char* printFormat);
+int sdpa_solver_write_init(sdpa_solver* s, const char* path, const char* printFormat);
+/* SDPA::getDimacsError (sdpa_call.cpp:849-853): out[1..6] (out[0] = 0). */
+int sdpa_solver_dimacs(sdpa_solver* s, double* out7);
+/* printResultXVec / XMat / YMat (which = 0, 1, 2) into the file at path. */
+int sdpa_solver_print_result(sdpa_solver* s, int which, const char* path);
 
 #ifdef __cplusplus
 }

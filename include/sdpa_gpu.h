@@ -111,6 +111,12 @@ int sdpa_gpu_set_buffer(sdpa_gpu_ctx* ctx, int which, const double* src, long le
  * enabled each timed phase synchronizes.  out[0..9] = ms and call counts for
  * bMat, Cholesky, solve, eigen, inverse. */
 int sdpa_gpu_timing(sdpa_gpu_ctx* ctx, int enable, double* out10);
+/* Instrumentation with no counterpart in the reference: while timing is
+ * enabled, HIP events on the side stream bracket every launch of the wide
+ * trailing update of the Schur Cholesky (its dominant kernel).  out[0] = ms
+ * summed over those launches, out[1] = launches, out[2] = algorithmic flops
+ * (2 K per lower-triangle element updated), out[3] = 0. */
+int sdpa_gpu_kernel_timing(sdpa_gpu_ctx* ctx, double* out4);
 int sdpa_gpu_synchronize(sdpa_gpu_ctx* ctx);
 
 #ifdef __cplusplus

@@ -330,12 +330,36 @@ static void compute_formula(oprob *P, double kappa) {
 /* Chordal::ordering_bMat, sdpa_chordal.cpp:365-410 (dense decisions only).
  * Returns 1 when the reference decides DENSE before any MUMPS analysis,
  * 0 when it would go on to the aggregate-sparsity/MUMPS tests. */
+static void pattern_mark(unsigned char *pat, int m, const int *g, int ng) {
+  for (int a = 0; a < ng; a++)
+    for (int b = 0; b <= a; b++) {
+      int i = g[a], j = g[b];
+      if (i < j) { int t = i; i = j; j = t; }
+      pat[(size_t)i * m + j] = 1;
+    }
+}
+
+/* Chordal::ordering_bMat's dense tests (sdpa_chordal.cpp:365-410): m <= 100;
+ * a block or LP coordinate used by more than m sqrt(0.7) constraints; the
+ * aggregate pattern (Chordal::makeGraph, 193-300: pairs sharing a block or an
+ * LP coordinate, plus the diagonal) holding more than 0.7 m^2 entries.
+ * 0 means the reference would go on to the MUMPS analysis. */
 static int schur_dense_decision(const oprob *P) {
   if (P->m <= 100) return 1;
   double thr = P->m * sqrt(0.70);
   for (int q = 0; q < P->nSDP; q++) if (P->sdpNCon[q] > thr) return 1;
   for (int q = 0; q < P->nLP; q++) if (P->lpNCon[q] > thr) return 1;
-  return 0;
+  const int m = P->m;
+  unsigned char *pat = (unsigned char *)calloc((size_t)m * m, 1);
+  for (int q = 0; q < P->nSDP; q++) pattern_mark(pat, m, P->sdpCon[q], P->sdpNCon[q]);
+  for (int q = 0; q < P->nLP; q++) pattern_mark(pat, m, P->lpCon[q], P->lpNCon[q]);
+  long lower = 0;
+  for (int i = 0; i < m; i++) {
+    pat[(size_t)i * m + i] = 1;
+    for (int j = 0; j <= i; j++) lower += pat[(size_t)i * m + j];
+  }
+  free(pat);
+  return (double)(2 * lower - m) > 0.70 * (double)m * (double)m;
 }
 
 /* SDPA::initializeSolve, sdpa_solve.cpp:10-29 (structural part). */

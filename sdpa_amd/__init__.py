@@ -81,6 +81,7 @@ def lib():
         L.sdpa_gpu_get_buffer.argtypes = [V, C.c_int, _dp, C.c_long]
         L.sdpa_gpu_set_buffer.argtypes = [V, C.c_int, _dp, C.c_long]
         L.sdpa_gpu_timing.argtypes = [V, C.c_int, V]
+        L.sdpa_gpu_kernel_timing.argtypes = [V, V]
         L.sdpa_gpu_synchronize.argtypes = [V]
         L.sdpa_solver_new.restype = V
         L.sdpa_solver_new.argtypes = [C.c_int]
@@ -103,6 +104,11 @@ def lib():
         L.sdpa_solver_phase_times.argtypes = [V, _dp]
         L.sdpa_solver_context.restype = V
         L.sdpa_solver_context.argtypes = [V]
+        L.sdpa_solver_read_init.argtypes = [V, C.c_char_p]
+        L.sdpa_solver_write_input.argtypes = [V, C.c_char_p, C.c_char_p]
+        L.sdpa_solver_write_init.argtypes = [V, C.c_char_p, C.c_char_p]
+        L.sdpa_solver_dimacs.argtypes = [V, V]
+        L.sdpa_solver_print_result.argtypes = [V, C.c_int, C.c_char_p]
         _lib = L
     return _lib
 
@@ -285,6 +291,30 @@ class Solver:
         self._chk(lib().sdpa_solver_read_input(self.h, path.encode()), "read_input")
         return self
 
+    def read_init(self, path):
+        """SDPA::readInit (.ini or .ini-s by extension) and setInitPoint(true)."""
+        self._chk(lib().sdpa_solver_read_init(self.h, str(path).encode()), "read_init")
+        return self
+
+    def write_input(self, path, fmt="%+8.3e"):
+        self._chk(lib().sdpa_solver_write_input(self.h, str(path).encode(), fmt.encode()),
+                  "write_input")
+
+    def write_init(self, path, fmt="%+8.3e"):
+        self._chk(lib().sdpa_solver_write_init(self.h, str(path).encode(), fmt.encode()),
+                  "write_init")
+
+    def dimacs(self):
+        """DIMACS errors err1..err6 of the final point (index 0 unused)."""
+        out = np.zeros(7)
+        self._chk(lib().sdpa_solver_dimacs(self.h, out.ctypes.data_as(C.c_void_p)), "dimacs")
+        return out
+
+    def print_result(self, which, path):
+        """printResultXVec / XMat / YMat (which 0, 1, 2) into a file."""
+        self._chk(lib().sdpa_solver_print_result(self.h, int(which), str(path).encode()),
+                  "print_result")
+
     def read_parameter(self, path):
         self._chk(lib().sdpa_solver_read_parameter(self.h, path.encode()), "read_parameter")
         return self
@@ -329,3 +359,12 @@ class Solver:
         out = np.zeros(5)
         lib().sdpa_solver_phase_times(self.h, out)
         return dict(zip(("bmat", "chol", "solve", "eig", "inv"), out))
+
+    def trailing_update_timing(self):
+        """HIP-event timing of the Schur Cholesky's wide trailing update
+        kernel over the last timed solve: (ms total, launches, flops)."""
+        ctx = lib().sdpa_solver_context(self.h)
+        out = np.zeros(4)
+        if ctx:
+            lib().sdpa_gpu_kernel_timing(ctx, out.ctypes.data_as(C.c_void_p))
+        return float(out[0]), int(out[1]), float(out[2])

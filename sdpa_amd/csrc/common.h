@@ -86,6 +86,16 @@ struct DenseWork {
   hipEvent_t evjoin = nullptr;
   int side_waves = 0;  // grid bound of the side stream's updates
   int ensure_side(int npanels);
+  // Optional timing of the wide trailing updates (the factorization's
+  // dominant kernel): HIP events around each launch on the side stream.
+  bool time_trailing = false;
+  std::vector<hipEvent_t> evt0, evt1;
+  std::vector<double> evt_flops;
+  int trail_pending = 0;
+  double trail_ms = 0.0, trail_flops = 0.0;
+  long trail_launches = 0;
+  int ensure_trailing_events(int n);
+  int collect_trailing();  // call once the factorization has completed
   int ensure_items(int n);
   int ensure_vec(size_t n);
   int ensure_mats(int n);

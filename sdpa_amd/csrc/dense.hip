@@ -178,6 +178,12 @@ int DenseWork::ensure_vec(size_t n) {
 }
 void DenseWork::release() {
   if (side) (void)hipStreamSynchronize(side);
+  for (auto e : evt0) (void)hipEventDestroy(e);
+  for (auto e : evt1) (void)hipEventDestroy(e);
+  evt0.clear();
+  evt1.clear();
+  evt_flops.clear();
+  trail_pending = 0;
   for (auto e : evp) (void)hipEventDestroy(e);
   for (auto e : evn) (void)hipEventDestroy(e);
   evp.clear();
@@ -743,6 +749,30 @@ int DenseWork::ensure_side(int npanels) {
   return 0;
 }
 
+int DenseWork::ensure_trailing_events(int n) {
+  while ((int)evt0.size() < n) {
+    hipEvent_t a, b;
+    SDPA_HIP_CHECK(hipEventCreate(&a));
+    SDPA_HIP_CHECK(hipEventCreate(&b));
+    evt0.push_back(a);
+    evt1.push_back(b);
+    evt_flops.push_back(0.0);
+  }
+  return 0;
+}
+
+int DenseWork::collect_trailing() {
+  for (int q = 0; q < trail_pending; ++q) {
+    float ms = 0.f;
+    SDPA_HIP_CHECK(hipEventElapsedTime(&ms, evt0[q], evt1[q]));
+    trail_ms += ms;
+    trail_flops += evt_flops[q];
+    ++trail_launches;
+  }
+  trail_pending = 0;
+  return 0;
+}
+
 int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitems, int mode,
                   int* d_info, double* dinv_out) {
   if (nitems <= 0) return 0;
@@ -786,7 +816,20 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
       launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + PW, K0 + 2 * PW,
                   w->side_waves);
       SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
+      const int nrest = maxn - (K0 + 2 * PW);
+      // timed: the launches that take the wide-tile kernel (more than 256 columns)
+      const bool timed = w->time_trailing && nrest > 256;
+      if (timed) {
+        if (int e = w->ensure_trailing_events(w->trail_pending + 1)) return e;
+        SDPA_HIP_CHECK(hipEventRecord(w->evt0[w->trail_pending], w->side));
+      }
       launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, maxn, w->side_waves);
+      if (timed) {
+        SDPA_HIP_CHECK(hipEventRecord(w->evt1[w->trail_pending], w->side));
+        // algorithmic flops: 2 K per element of the lower triangle updated
+        w->evt_flops[w->trail_pending] = (double)PW * nrest * (nrest + 1.0);
+        ++w->trail_pending;
+      }
     }
     SDPA_HIP_CHECK(hipEventRecord(w->evjoin, w->side));
     SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evjoin, 0));

@@ -380,6 +380,34 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       if (c->blk[q].ncon > thr) c->schurDense = 1;
     for (int t = 0; t < c->nLP; ++t)
       if ((double)byc[t].size() > thr) c->schurDense = 1;
+    if (!c->schurDense) {
+      // aggregate pattern of B (sdpa_chordal.cpp:404-410): pairs of
+      // constraints sharing an SDP block or an LP coordinate, plus the
+      // diagonal; dense when the full count exceeds 0.7 m^2
+      const size_t words = ((size_t)m + 63) / 64;
+      std::vector<uint64_t> pat((size_t)m * words, 0ull);
+      auto mark = [&](const std::vector<int>& g) {
+        for (size_t a = 0; a < g.size(); ++a)
+          for (size_t b2 = 0; b2 <= a; ++b2) {
+            int i = g[a], j = g[b2];
+            if (i < j) std::swap(i, j);
+            pat[(size_t)i * words + (j >> 6)] |= 1ull << (j & 63);
+          }
+      };
+      for (int q = 0; q < nS; ++q) mark(c->blk[q].h_con);
+      std::vector<int> g;
+      for (int t = 0; t < c->nLP; ++t) {
+        g.clear();
+        for (auto& pr : byc[t]) g.push_back(pr.first);
+        mark(g);
+      }
+      long lower = 0;
+      for (int i = 0; i < m; ++i) {
+        pat[(size_t)i * words + (i >> 6)] |= 1ull << (i & 63);
+        for (size_t w = 0; w < words; ++w) lower += __builtin_popcountll(pat[(size_t)i * words + w]);
+      }
+      if ((double)(2 * lower - m) > 0.70 * (double)m * (double)m) c->schurDense = 1;
+    }
   }
   c->bmatStoreFirst = (nS == 1 && c->nLP == 0 && c->blk[0].ncon == m && c->blk[0].nF1 == 0);
 
@@ -570,6 +598,7 @@ int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* h, int precision) {
   if (r) return r;
   SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (int e = c->dw.collect_trailing()) return e;
   if (c->h_info[0] == 0) {
     r = diag_block_inverses(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv, c->d_Binv);
     if (r) return r;
@@ -641,7 +670,20 @@ int sdpa_gpu_timing(sdpa_gpu_ctx* h, int enable, double* out) {
     c->timing = enable != 0;
     c->t_bmat = c->t_chol = c->t_solve = c->t_eig = c->t_inv = 0;
     c->n_bmat = c->n_chol = c->n_solve = 0;
+    c->dw.time_trailing = c->timing;
+    c->dw.trail_ms = c->dw.trail_flops = 0.0;
+    c->dw.trail_launches = 0;
   }
+  return 0;
+}
+
+int sdpa_gpu_kernel_timing(sdpa_gpu_ctx* h, double* out4) {
+  if (!h || !out4) return -1;
+  Ctx* c = &h->c;
+  out4[0] = c->dw.trail_ms;
+  out4[1] = (double)c->dw.trail_launches;
+  out4[2] = c->dw.trail_flops;
+  out4[3] = 0.0;
   return 0;
 }
 

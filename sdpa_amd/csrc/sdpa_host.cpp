@@ -67,6 +67,10 @@ void SDPA::terminate() {
 
 // Parameter::setDefaultParameter (sdpa_parts.cpp:182-224)
 void SDPA::setParameterType(ParameterType type) {
+  // Parameter::setDefaultParameter (sdpa_parts.cpp:192-229)
+  paramType_ = type;
+  xPrint_ = XPrint_ = YPrint_ = "%+8.3e";
+  infPrint_ = "%+10.16e";
   if (type == PARAMETER_STABLE_BUT_SLOW)
     param_ = Param{1000, 1.0e-7, 1.0e+4, 2.0, -1.0e+5, 1.0e+5, 0.1, 0.3, 0.8, 1.0e-7};
   else if (type == PARAMETER_UNSTABLE_BUT_FAST)
@@ -153,7 +157,7 @@ void SDPA::loadRecords(int m, int nBlock, const int* bs, const double* c, long n
 }
 
 // IO::read for .dat-s and .dat (sdpa_io.cpp:33-66, 473-606)
-void SDPA::readInput(const char* filename, FILE*, SparseType type) {
+void SDPA::readInput(const char* filename, FILE* fpout, SparseType type) {
   FILE* f = std::fopen(filename, "rb");
   if (!f) throw std::runtime_error(std::string("cannot open ") + filename);
   std::string buf;
@@ -163,6 +167,7 @@ void SDPA::readInput(const char* filename, FILE*, SparseType type) {
   std::fclose(f);
   bool sparse = type == SPARSE || (type == AUTO && std::string(filename).size() > 2 &&
                                    std::string(filename).substr(std::string(filename).size() - 2) == "-s");
+  if (fpout) std::fprintf(fpout, "data   is %s  : %s\n", filename, sparse ? "sparse" : "dense");
   const char* p = buf.data();
   const char* end = p + buf.size();
   // comment lines start with '*' or '"'
@@ -226,14 +231,22 @@ void SDPA::readInput(const char* filename, FILE*, SparseType type) {
 }
 
 // Parameter::readFile (sdpa_parts.cpp:231-263): one value per line, positional
-void SDPA::readParameter(const char* filename, FILE*) {
+void SDPA::readParameter(const char* filename, FILE* fpout) {
   FILE* f = std::fopen(filename, "r");
   if (!f) throw std::runtime_error(std::string("cannot open ") + filename);
+  if (fpout) std::fprintf(fpout, "param  is %s \n", filename);
   double v[10];
   char line[512];
   for (int t = 0; t < 10; ++t) {
     if (!std::fgets(line, sizeof(line), f)) { std::fclose(f); throw std::runtime_error("short parameter file"); }
     v[t] = std::strtod(line, nullptr);
+  }
+  // lines 11-14: the print formats (first token of each line)
+  std::string* fmt[4] = {&xPrint_, &XPrint_, &YPrint_, &infPrint_};
+  for (int t = 0; t < 4; ++t) {
+    if (!std::fgets(line, sizeof(line), f)) break;
+    char tok[256];
+    if (std::sscanf(line, "%255s", tok) == 1) *fmt[t] = tok;
   }
   std::fclose(f);
   param_ = Param{(int)v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9]};
@@ -344,6 +357,393 @@ void SDPA::copyCurrentToInit() {
   isInitPoint_ = true;
 }
 
+
+// ---------------------------------------------------------------------------
+// Output in the reference's formats (they are the interface): sdpa_io.cpp
+// printHeader 608-620, printOneIteration 622-659, printLastInfo 661-773,
+// printSolution 844-870; sdpa_struct.cpp Vector::display 104-124,
+// DenseMatrix::display 795-835, DenseLinearSpace::displaySolution
+// 1624-1660; Parameter::display sdpa_parts.cpp:266-310.
+
+namespace {
+
+const char* kNoPrint = "NOPRINT";
+
+void print_header(FILE* fp) {
+  if (!fp) return;
+  std::fprintf(fp, "   mu      thetaP  thetaD  objP      objD      alphaP  alphaD  beta \n");
+  std::fflush(fp);
+}
+
+// one iteration line; the trace holds internal values, the line shows the
+// user's primal/dual (REVERSE_PRIMAL_DUAL, sdpa_io.cpp:640-645)
+void print_iteration(FILE* fp, int it, const IterationTrace& t) {
+  if (!fp) return;
+  std::fprintf(fp, "%2d %4.1e %4.1e %4.1e %+7.2e %+7.2e %4.1e %4.1e %4.2e\n", it, t.mu, t.thetaD,
+               t.thetaP, -t.objD, -t.objP, t.alphaD, t.alphaP, t.beta);
+}
+
+void print_vector(FILE* fp, const double* v, int n, const char* fmt) {
+  if (!fp) return;
+  if (std::strcmp(fmt, kNoPrint) == 0) { std::fprintf(fp, "%s\n", kNoPrint); return; }
+  std::fprintf(fp, "{");
+  for (int j = 0; j < n - 1; ++j) { std::fprintf(fp, fmt, v[j]); std::fprintf(fp, ","); }
+  if (n > 0) { std::fprintf(fp, fmt, v[n - 1]); std::fprintf(fp, "}\n"); }
+  else std::fprintf(fp, "  }\n");
+}
+
+// a dense n x n block, column-major, printed row by row
+void print_dense(FILE* fp, const double* a, int n, const char* fmt) {
+  std::fprintf(fp, "{");
+  for (int i = 0; i < n - 1; ++i) {
+    std::fprintf(fp, i == 0 ? " " : "  ");
+    std::fprintf(fp, "{");
+    for (int j = 0; j < n - 1; ++j) { std::fprintf(fp, fmt, a[i + (long)n * j]); std::fprintf(fp, ","); }
+    std::fprintf(fp, fmt, a[i + (long)n * (n - 1)]);
+    std::fprintf(fp, " },\n");
+  }
+  if (n > 1) std::fprintf(fp, "  {");
+  for (int j = 0; j < n - 1; ++j) { std::fprintf(fp, fmt, a[(n - 1) + (long)n * j]); std::fprintf(fp, ","); }
+  std::fprintf(fp, fmt, a[(n - 1) + (long)n * (n - 1)]);
+  std::fprintf(fp, " }");
+  std::fprintf(fp, n > 1 ? "   }\n" : "\n");
+}
+
+const char* phase_name(int p) {
+  static const char* names[] = {"noINFO    ", "pFEAS     ", "dFEAS     ", "pdFEAS    ",
+                                "pdINF     ", "pFEAS_dINF", "pINF_dFEAS", "pdOPT     ",
+                                "pUNBD     ", "dUNBD     "};
+  return (p >= 0 && p < 10) ? names[p] : "phase error";
+}
+
+// smallest eigenvalue of a symmetric n x n matrix (column-major, overwritten):
+// Householder reduction to tridiagonal form, then bisection on the Sturm
+// count (the textbook method; the reference calls LAPACK dsyev here)
+double sym_min_eig(std::vector<double>& a, int n) {
+  if (n == 1) return a[0];
+  std::vector<double> d(n), e(n, 0.0), v(n), w(n);
+  for (int k = 0; k < n - 2; ++k) {
+    double alpha = 0.0;
+    for (int i = k + 1; i < n; ++i) alpha += a[i + (long)n * k] * a[i + (long)n * k];
+    alpha = std::sqrt(alpha);
+    const double x0 = a[(k + 1) + (long)n * k];
+    if (alpha == 0.0) { e[k] = 0.0; continue; }
+    const double sgn = x0 >= 0.0 ? 1.0 : -1.0;
+    e[k] = -sgn * alpha;
+    // v = x - (-sgn alpha) e1, H = I - 2 v v^T / v^T v
+    for (int i = 0; i < n; ++i) v[i] = 0.0;
+    for (int i = k + 1; i < n; ++i) v[i] = a[i + (long)n * k];
+    v[k + 1] += sgn * alpha;
+    double vv = 0.0;
+    for (int i = k + 1; i < n; ++i) vv += v[i] * v[i];
+    if (vv == 0.0) continue;
+    // A <- H A H on the trailing block: p = A v * 2/vv, q = p - (v^T p / vv) v
+    for (int i = k + 1; i < n; ++i) {
+      double t = 0.0;
+      for (int j = k + 1; j < n; ++j) t += a[i + (long)n * j] * v[j];
+      w[i] = 2.0 * t / vv;
+    }
+    double vp = 0.0;
+    for (int i = k + 1; i < n; ++i) vp += v[i] * w[i];
+    const double c = vp / vv;
+    for (int i = k + 1; i < n; ++i) w[i] -= c * v[i];
+    for (int j = k + 1; j < n; ++j)
+      for (int i = k + 1; i < n; ++i) a[i + (long)n * j] -= v[i] * w[j] + w[i] * v[j];
+  }
+  for (int i = 0; i < n; ++i) d[i] = a[i + (long)n * i];
+  e[n - 2] = a[(n - 1) + (long)n * (n - 2)];
+  // Gershgorin bounds, then bisection for the smallest eigenvalue
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < n; ++i) {
+    double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  auto count_below = [&](double x) {
+    int cnt = 0;
+    double q = d[0] - x;
+    if (q < 0.0) ++cnt;
+    for (int i = 1; i < n; ++i) {
+      const double qq = (q == 0.0) ? 1e-300 : q;
+      q = d[i] - x - e[i - 1] * e[i - 1] / qq;
+      if (q < 0.0) ++cnt;
+    }
+    return cnt;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo) + std::fabs(hi)); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= 1) hi = mid; else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+}  // namespace
+
+void SDPA::printSDPAVersion(FILE* fp) {
+  if (fp) std::fprintf(fp, "SDPA 7.4.4 interface, sdpa_amd MI355X (gfx950) Newton-direction backend\n");
+}
+
+void SDPA::printParametersWith(FILE* fp, const char* f) {
+  if (!fp) return;
+  if (std::strcmp(f, kNoPrint) == 0) { std::fprintf(fp, "%s\n", kNoPrint); return; }
+  const Param& P = param_;
+  std::fprintf(fp, "** Parameters **\n");
+  std::fprintf(fp, "maxIteration = %d\n", P.maxIteration);
+  const char* names[9] = {"epsilonStar  = ", "lambdaStar   = ", "omegaStar    = ",
+                          "lowerBound   = ", "upperBound   = ", "betaStar     = ",
+                          "betaBar      = ", "gammaStar    = ", "epsilonDash  = "};
+  const double vals[9] = {P.epsilonStar, P.lambdaStar, P.omegaStar, P.lowerBound, P.upperBound,
+                          P.betaStar, P.betaBar, P.gammaStar, P.epsilonDash};
+  for (int t = 0; t < 9; ++t) {
+    std::fprintf(fp, "%s", names[t]);
+    std::fprintf(fp, f, vals[t]);
+    std::fprintf(fp, "\n");
+  }
+  std::fprintf(fp, "xPrint       = %s \n", xPrint_.c_str());
+  std::fprintf(fp, "XPrint       = %s \n", XPrint_.c_str());
+  std::fprintf(fp, "YPrint       = %s \n", YPrint_.c_str());
+  std::fprintf(fp, "infPrint     = %s \n", infPrint_.c_str());
+}
+
+void SDPA::printParameters(FILE* fp) { printParametersWith(fp, "%+10.16e"); }
+
+// internal blocks M (packed) in user block order, as displaySolution does
+void SDPA::printBlocks(FILE* fp, const std::vector<double>& M, const char* fmt) {
+  if (!fp) return;
+  if (std::strcmp(fmt, kNoPrint) == 0) { std::fprintf(fp, "%s\n", kNoPrint); return; }
+  std::fprintf(fp, "{\n");
+  for (int b = 0; b < nBlock_; ++b) {
+    const int s = blockStruct_[b];
+    if (M.size() != (size_t)total_) continue;
+    if (s >= 2) print_dense(fp, M.data() + blockOff_[b], s, fmt);
+    else print_vector(fp, M.data() + blockOff_[b], std::abs(s), fmt);
+  }
+  std::fprintf(fp, "}\n");
+}
+
+void SDPA::printResultXVec(FILE* fp) { print_vector(fp, xVec_.data(), (int)xVec_.size(), xPrint_.c_str()); }
+void SDPA::printResultXMat(FILE* fp) { printBlocks(fp, Z_, XPrint_.c_str()); }
+void SDPA::printResultYMat(FILE* fp) { printBlocks(fp, X_, YPrint_.c_str()); }
+
+void SDPA::printComputationTime(FILE* fp) {
+  if (!fp) return;
+  // the reference's ComputeTime table (sdpa_parts.cpp:78-190) with the buckets
+  // this build measures: device phase times when timing is enabled
+  const double main = solveTime_ > 0 ? solveTime_ : 1.0;
+  std::fprintf(fp, "\n");
+  std::fprintf(fp, "                         Time(sec)  Ratio(%% : MainLoop) \n");
+  const char* names[5] = {" Make bMat time  =       ", " Cholesky bMat   =       ",
+                          " solve           =       ", " EigxMatTime     =       ",
+                          " invzMatTime     =       "};
+  for (int t = 0; t < 5; ++t)
+    std::fprintf(fp, "%s%f,  %f\n", names[t], phaseTimes[t] * 1e-3, phaseTimes[t] * 1e-3 / main * 100.0);
+  std::fprintf(fp, " Main Loop       =       %f,  %f\n", solveTime_, 100.0);
+  std::fprintf(fp, " Total           =       %f,  %f\n", solveTime_, 100.0);
+}
+
+// IO::computeDimacs (sdpa_io.cpp:775-813).  The reference's "getOneNorm" is
+// the largest absolute entry (sdpa_linear.cpp:184-235); getTwoNorm is the
+// squared Euclidean / Frobenius norm (237-268).
+void SDPA::getDimacsError(double* err) {
+  for (int t = 0; t <= 6; ++t) err[t] = 0.0;
+  if (!ctx_ || X_.size() != (size_t)total_) return;
+  double b1 = 0.0, c1 = 0.0;
+  for (double v : c_) b1 = std::max(b1, std::fabs(v));
+  for (size_t e = 0; e < rk_.size(); ++e)
+    if (rk_[e] == 0) c1 = std::max(c1, std::fabs(rv_[e]));
+  double np, nd;
+  check(sdpa_gpu_residuals(ctx_, &np, &nd), ctx_, "residuals");
+  std::vector<double> pr(m_), dr(total_);
+  check(sdpa_gpu_get_buffer(ctx_, 11, pr.data(), m_), ctx_, "get_buffer");
+  check(sdpa_gpu_get_buffer(ctx_, 12, dr.data(), total_), ctx_, "get_buffer");
+  double p2 = 0.0, d2 = 0.0, xz = 0.0;
+  for (double v : pr) p2 += v * v;
+  for (double v : dr) d2 += v * v;
+  for (long t = 0; t < total_; ++t) xz += X_[t] * Z_[t];
+  auto min_eig = [&](const std::vector<double>& M) {
+    double mn = 1.0e50;
+    for (int b = 0; b < nBlock_; ++b) {
+      const int s = blockStruct_[b];
+      if (s >= 2) {
+        std::vector<double> a(M.begin() + blockOff_[b], M.begin() + blockOff_[b] + (long)s * s);
+        mn = std::min(mn, sym_min_eig(a, s));
+      } else {
+        for (int t = 0; t < std::abs(s); ++t) mn = std::min(mn, M[blockOff_[b] + t]);
+      }
+    }
+    return mn;
+  };
+  const double xmin = min_eig(X_), zmin = min_eig(Z_);
+  const double ctx = objValPrimal_, bty = objValDual_;
+  err[1] = std::sqrt(p2) / (1.0 + b1);
+  err[2] = std::max(0.0, -xmin / (1.0 + b1));
+  err[3] = std::sqrt(d2) / (1.0 + c1);
+  err[4] = std::max(0.0, -zmin / (1.0 + c1));
+  err[5] = (ctx - bty) / (1.0 + std::fabs(ctx) + std::fabs(bty));
+  err[6] = xz / (1.0 + std::fabs(ctx) + std::fabs(bty));
+}
+
+// IO::printDimacs (sdpa_io.cpp:815-842)
+void SDPA::printDimacsError(double* e, const char* f, FILE* fp) {
+  if (!fp) return;
+  const char* tail[6] = {"  [||Ax-b|| / (1+||b||_1)]\n", "  [max(0, -lambda(x)/(1+||b||_1))]\n",
+                         "  [||A^Ty + z - c || / (1+||c||_1)]\n", "  [max(0, -lambda(z)/(1+||c||_1))]\n",
+                         "  [(<c,x> - <b,y>) / (1 + |<c,x>| + |<b,y>|)]\n",
+                         "  [<x,z> / (1 + |<c,x>| + |<b,y>|)]\n"};
+  std::fprintf(fp, "\n* DIMACS_ERRORS * \n");
+  for (int t = 1; t <= 6; ++t) {
+    std::fprintf(fp, "err%d = ", t);
+    std::fprintf(fp, f, e[t]);
+    std::fprintf(fp, "%s", tail[t - 1]);
+  }
+}
+
+// SDPA::readInit + IO::read(xMat, yVec, zMat) (sdpa_call.cpp:946-990,
+// sdpa_io.cpp:75-228): first the user's x (internal y = -x), then the
+// user's X (target 1, internal zMat) and Y (target 2, internal xMat)
+void SDPA::readInit(const char* filename, FILE* fpout, SparseType type) {
+  FILE* f = std::fopen(filename, "rb");
+  if (!f) throw std::runtime_error(std::string("cannot open init file ") + filename);
+  std::string buf;
+  char tmp[1 << 16];
+  size_t got;
+  while ((got = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.append(tmp, got);
+  std::fclose(f);
+  const std::string fn(filename);
+  const bool sparse = type == SPARSE || (type == AUTO && fn.size() > 2 && fn.substr(fn.size() - 2) == "-s");
+  if (fpout) std::fprintf(fpout, "initial is %s  : %s\n", filename, sparse ? "sparse" : "dense");
+  buildLayout();
+  initX_.assign(total_, 0.0);
+  initZ_.assign(total_, 0.0);
+  initY_.assign(m_, 0.0);
+  const char* p = buf.data();
+  const char* end = p + buf.size();
+  double x;
+  for (int k = 1; k <= m_; ++k) {
+    if (!next_number(p, end, x)) throw std::runtime_error("init file: x vector ended early");
+    inputInitXVec(k, x);
+  }
+  if (sparse) {
+    double r[5];
+    for (;;) {
+      int n5 = 0;
+      for (; n5 < 5; ++n5)
+        if (!next_number(p, end, r[n5])) break;
+      if (n5 < 5) break;
+      if (r[4] == 0.0) continue;
+      const int target = (int)r[0], l = (int)r[1], i = (int)r[2], j = (int)r[3];
+      if (l < 1 || l > nBlock_) throw std::runtime_error("init file: block out of range");
+      if (blockStruct_[l - 1] < 2 && i != j)
+        throw std::runtime_error("init file: LP entries need row == column");
+      if (target == 1) inputInitXMat(l, i, j, r[4]);
+      else inputInitYMat(l, i, j, r[4]);
+    }
+  } else {
+    for (int target = 1; target <= 2; ++target)
+      for (int b = 0; b < nBlock_; ++b) {
+        const int s = blockStruct_[b];
+        const int size = s >= 2 ? s * s : std::abs(s);
+        for (int e = 0; e < size; ++e) {
+          if (!next_number(p, end, x)) throw std::runtime_error("init file: ended early");
+          const int i = s >= 2 ? e / s : e, j = s >= 2 ? e % s : e;
+          if (x == 0.0 || (s >= 2 && i > j)) continue;
+          if (target == 1) inputInitXMat(b + 1, i + 1, j + 1, x);
+          else inputInitYMat(b + 1, i + 1, j + 1, x);
+        }
+      }
+  }
+}
+
+// SDPA::writeInputSparse (sdpa_call.cpp:1099-1147): the problem as .dat-s
+void SDPA::writeInputSparse(const char* filename, const char* fmt) {
+  FILE* f = std::fopen(filename, "w");
+  if (!f) throw std::runtime_error(std::string("cannot open ") + filename + " to write");
+  std::fprintf(f, "%d\n%d\n", m_, nBlock_);
+  for (int b = 0; b < nBlock_; ++b) std::fprintf(f, b + 1 < nBlock_ ? "%d," : "%d\n", blockStruct_[b]);
+  if (std::strcmp(fmt, kNoPrint) == 0) {
+    std::fprintf(f, "%s\n", kNoPrint);
+  } else {
+    for (int k = 0; k < m_; ++k) { std::fprintf(f, fmt, c_[k]); std::fprintf(f, " "); }
+    std::fprintf(f, "\n");
+    // F0 first, then F1..Fm; within one matrix the records keep their order
+    std::vector<size_t> order(rk_.size());
+    for (size_t e = 0; e < order.size(); ++e) order[e] = e;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return rk_[a] < rk_[b]; });
+    for (size_t e : order) {
+      if (rv_[e] == 0.0) continue;
+      std::fprintf(f, "%d %d %d %d ", rk_[e], rl_[e], ri_[e], rj_[e]);
+      std::fprintf(f, fmt, rv_[e]);
+      std::fprintf(f, "\n");
+    }
+  }
+  std::fclose(f);
+}
+
+// SDPA::writeInitSparse (sdpa_call.cpp:1199-1222): the final point as an
+// .ini-s (the reference writes minus its final yVec, i.e. the internal y)
+void SDPA::writeInitSparse(const char* filename, const char* fmt) {
+  FILE* f = std::fopen(filename, "w");
+  if (!f) throw std::runtime_error(std::string("cannot open ") + filename + " to write");
+  if (std::strcmp(fmt, kNoPrint) == 0) { std::fprintf(f, "%s\n", kNoPrint); std::fclose(f); return; }
+  for (int k = 0; k < m_; ++k) { std::fprintf(f, fmt, k < (int)y_.size() ? y_[k] : 0.0); std::fprintf(f, " "); }
+  std::fprintf(f, "\n");
+  const std::vector<double>* mats[2] = {&Z_, &X_};
+  for (int target = 1; target <= 2; ++target) {
+    const std::vector<double>& M = *mats[target - 1];
+    if (M.size() != (size_t)total_) continue;
+    for (int b = 0; b < nBlock_; ++b) {
+      const int s = blockStruct_[b];
+      if (s < 2) continue;
+      for (int i = 0; i < s; ++i)
+        for (int j = i; j < s; ++j) {
+          const double v = M[blockOff_[b] + i + (long)s * j];
+          if (v == 0.0) continue;
+          std::fprintf(f, "%d %d %d %d ", target, b + 1, i + 1, j + 1);
+          std::fprintf(f, fmt, v);
+          std::fprintf(f, "\n");
+        }
+    }
+    for (int b = 0; b < nBlock_; ++b) {
+      const int s = blockStruct_[b];
+      if (s >= 2) continue;
+      for (int i = 0; i < std::abs(s); ++i) {
+        std::fprintf(f, "%d %d %d %d ", target, b + 1, i + 1, i + 1);
+        std::fprintf(f, fmt, M[blockOff_[b] + i]);
+        std::fprintf(f, "\n");
+      }
+    }
+  }
+  std::fclose(f);
+}
+
+// IO::printLastInfo's summary block for one stream
+void SDPA::printLastInfo(FILE* fp) {
+  if (!fp) return;
+  const double mean = (std::fabs(objValPrimal_) + std::fabs(objValDual_)) / 2.0;
+  const double pdgap = std::fabs(objValPrimal_ - objValDual_);
+  const double relgap = mean < 1.0 ? pdgap : pdgap / mean;
+  const double gap = objValPrimal_ - objValDual_;
+  const double digits = -std::log10(std::fabs(pdgap / mean));
+  const char* f = infPrint_.c_str();
+  auto line = [&](const char* name, double v) {
+    std::fprintf(fp, "%s", name);
+    std::fprintf(fp, f, v);
+    std::fprintf(fp, "\n");
+  };
+  std::fprintf(fp, "\n");
+  std::fprintf(fp, "phase.value  = %s\n", phase_name(phaseUser_));
+  std::fprintf(fp, "   Iteration = %d\n", pIteration_);
+  line("          mu = ", mu_);
+  line("relative gap = ", relgap);
+  line("        gap  = ", gap);
+  line("     digits  = ", digits);
+  line("objValPrimal = ", -objValDual_);
+  line("objValDual   = ", -objValPrimal_);
+  line("p.feas.error = ", normDualMat_);
+  line("d.feas.error = ", normPrimalVec_);
+  std::fprintf(fp, "total time   = %.6f\n", solveTime_);
+}
+
 // ---------------------------------------------------------------------------
 // SDPA::solve, sdpa_solve.cpp:59-233
 void SDPA::solve() {
@@ -418,6 +818,8 @@ void SDPA::solve() {
   };
   update_phase();
 
+  print_header(fpout_);
+  print_header(display_);
   int it = 0;
   while (update_phase() && it < P.maxIteration) {
     const bool affine = !(phase == noINFO || phase == pFEAS || phase == dFEAS);
@@ -514,6 +916,8 @@ void SDPA::solve() {
       if (maxRatio > 0.0 && maxRatio < 1.0) { alphaP *= maxRatio; alphaD *= maxRatio; }
     }
     trace_.push_back(IterationTrace{muCur, thetaP, thetaD, objP, objD, alphaP, alphaD, betaVal});
+    print_iteration(fpout_, it, trace_.back());
+    print_iteration(display_, it, trace_.back());
 
     // Solutions::update (sdpa_dataset.cpp:145-171)
     check(sdpa_gpu_update(g, alphaP, alphaD), g, "update");
@@ -579,6 +983,29 @@ void SDPA::solve() {
   phaseTimes[0] = tm[0]; phaseTimes[1] = tm[2]; phaseTimes[2] = tm[4]; phaseTimes[3] = tm[6];
   phaseTimes[4] = tm[8];
   solveTime_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // IO::printLastInfo + printSolution (sdpa_solve.cpp:207-232)
+  print_iteration(fpout_, it, trace_.back());
+  print_iteration(display_, it, trace_.back());
+  printLastInfo(display_);
+  printLastInfo(fpout_);
+  if (fpout_) {
+    printParametersWith(fpout_, infPrint_.c_str());
+    printComputationTime(fpout_);
+    std::fprintf(fpout_, "xVec = \n");
+    printResultXVec(fpout_);
+    std::fprintf(fpout_, "xMat = \n");
+    printResultXMat(fpout_);
+    std::fprintf(fpout_, "yMat = \n");
+    printResultYMat(fpout_);
+  }
+  if (display_) {
+    std::fprintf(display_, "  main loop time = %.6f\n", solveTime_);
+    std::fprintf(display_, "      total time = %.6f\n", solveTime_);
+  }
+  if (fpout_) {
+    std::fprintf(fpout_, "    main loop time = %.6f\n", solveTime_);
+    std::fprintf(fpout_, "        total time = %.6f\n", solveTime_);
+  }
 }
 
 }  // namespace sdpa_amd
@@ -709,5 +1136,26 @@ int sdpa_solver_phase_times(sdpa_solver* h, double* out) {
 }
 
 sdpa_gpu_ctx* sdpa_solver_context(sdpa_solver* h) { return h ? h->s->context() : nullptr; }
+
+int sdpa_solver_read_init(sdpa_solver* h, const char* path) {
+  SOLVER_TRY(h->s->readInit(path); h->s->setInitPoint(true))
+}
+int sdpa_solver_write_input(sdpa_solver* h, const char* path, const char* fmt) {
+  SOLVER_TRY(h->s->writeInputSparse(path, fmt))
+}
+int sdpa_solver_write_init(sdpa_solver* h, const char* path, const char* fmt) {
+  SOLVER_TRY(h->s->writeInitSparse(path, fmt))
+}
+int sdpa_solver_dimacs(sdpa_solver* h, double* out) { SOLVER_TRY(h->s->getDimacsError(out)) }
+int sdpa_solver_print_result(sdpa_solver* h, int which, const char* path) {
+  SOLVER_TRY({
+    FILE* f = std::fopen(path, "w");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    if (which == 0) h->s->printResultXVec(f);
+    else if (which == 1) h->s->printResultXMat(f);
+    else h->s->printResultYMat(f);
+    std::fclose(f);
+  })
+}
 
 }  // extern "C"

@@ -210,3 +210,29 @@ def test_formula_choice_on_benchmark_families():
     P = O.Problem(PR.dense_random(n=20, m=30, seed=3))
     c = P.formula_counts()
     assert c[2] == 0
+
+
+def _two_block_problem(ranges, m=120, n=4):
+    """Two SDP blocks; constraint k (1-based) touches block b when k-1 is in
+    ranges[b], with one diagonal entry there."""
+    from sdpa_amd.problems import SdpData
+    K, L, I, J, V = [], [], [], [], []
+    for b, (lo, hi) in enumerate(ranges):
+        for k in range(lo, hi):
+            K.append(k + 1); L.append(b + 1); I.append(1 + k % n); J.append(1 + k % n); V.append(1.0)
+    for b in range(2):  # F0 = identity on both blocks
+        for t in range(n):
+            K.append(0); L.append(b + 1); I.append(t + 1); J.append(t + 1); V.append(1.0)
+    arr = lambda x, t: np.asarray(x, t)
+    return SdpData(m, arr([n, n], np.int32), np.ones(m), arr(K, np.int32), arr(L, np.int32),
+                   arr(I, np.int32), arr(J, np.int32), arr(V, np.float64))
+
+
+def test_schur_dense_aggregate_pattern_rule():
+    """sdpa_chordal.cpp:404-410: with m > 100 and no block above m*sqrt(0.7)
+    constraints, the Schur path is dense iff the aggregate pattern (pairs of
+    constraints sharing a block, plus the diagonal) exceeds 0.7 m^2.
+    Overlapping 90-constraint sets: 2*90^2 - 60^2 = 12600 > 10080 -> dense;
+    disjoint 60-constraint sets: 2*60^2 = 7200 -> not dense."""
+    assert O.Problem(_two_block_problem([(0, 90), (30, 120)])).schur_dense
+    assert not O.Problem(_two_block_problem([(0, 60), (60, 120)])).schur_dense
