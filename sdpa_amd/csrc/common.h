@@ -80,6 +80,15 @@ struct DenseWork {
   size_t cap_dinv = 0;
   double* d_vec = nullptr;   // m-vector scratch of the Schur solves
   size_t cap_vec = 0;
+  // cooperative solves: per-strip flags, the launch epoch, the failure flag
+  // (device and a pinned host copy written after each solve)
+  int* d_flags = nullptr;
+  int cap_flags = 0;
+  int epoch = 0;
+  int* d_err = nullptr;
+  int* h_err = nullptr;
+  int coop_state = 0;  // 0 unknown, 1 cooperative launches available, 2 not
+  int ensure_flags(int n);
   // look-ahead Cholesky: side stream (default priority) and per-panel events
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> evp, evn;

@@ -169,6 +169,21 @@ int DenseWork::ensure_dinv(size_t n) {
   SDPA_HIP_CHECK(hipMalloc(&d_dinv, sizeof(double) * cap_dinv));
   return 0;
 }
+int DenseWork::ensure_flags(int n) {
+  if (!d_err) {
+    SDPA_HIP_CHECK(hipMalloc(&d_err, sizeof(int)));
+    SDPA_HIP_CHECK(hipMemset(d_err, 0, sizeof(int)));
+    SDPA_HIP_CHECK(hipHostMalloc(&h_err, sizeof(int)));
+    *h_err = 0;
+  }
+  if (n <= cap_flags) return 0;
+  if (d_flags) SDPA_HIP_CHECK(hipFree(d_flags));
+  cap_flags = n;
+  SDPA_HIP_CHECK(hipMalloc(&d_flags, sizeof(int) * cap_flags));
+  SDPA_HIP_CHECK(hipMemset(d_flags, 0, sizeof(int) * cap_flags));
+  epoch = 0;
+  return 0;
+}
 int DenseWork::ensure_vec(size_t n) {
   if (n <= cap_vec) return 0;
   if (d_vec) SDPA_HIP_CHECK(hipFree(d_vec));
@@ -194,6 +209,10 @@ void DenseWork::release() {
   side = nullptr;
   if (d_vec) (void)hipFree(d_vec);
   d_vec = nullptr; cap_vec = 0;
+  if (d_flags) (void)hipFree(d_flags);
+  if (d_err) (void)hipFree(d_err);
+  if (h_err) (void)hipHostFree(h_err);
+  d_flags = nullptr; d_err = nullptr; h_err = nullptr; cap_flags = 0; epoch = 0;
   if (d_items) (void)hipFree(d_items);
   if (d_mats) (void)hipFree(d_mats);
   if (d_dinv) (void)hipFree(d_dinv);
@@ -1109,9 +1128,257 @@ __global__ __launch_bounds__(256) void k_bwd_update(const double* __restrict__ L
   if (lane == 0) y[r] -= acc;
 }
 
+// ---------------------------------------------------------------------------
+// The same two solves, each as ONE cooperative launch (all work-groups
+// co-resident): work-group s owns the 64-row strip s; the 256-row block b is
+// strips 4b..4b+3.  Forward (L t = x): a strip subtracts L[strip, block b']
+// t_b' for every earlier block as soon as t_b' is published, writes its rows
+// of y_b = x_b - sum, and once the block's four strips have published y_b it
+// forms its rows of t_b = W_b y_b and publishes them.  Backward (L^T x = t)
+// runs the blocks from the last to the first with the columns of L.  Blocks
+// are published by release stores of a per-strip flag (value = this launch's
+// epoch, so flags never need clearing); readers poll with acquire loads,
+// then read the published vector with device-scope atomic loads.  Every poll
+// is bounded: after kSpinLimit polls the work-group records the failure in
+// *err and returns, so a launch always ends.
+constexpr unsigned kSpinLimit = 1u << 22;  // ~0.5 s at the poll interval
+
+struct TrsvParams {
+  const double* L;
+  int ld, m;
+  const double* W;    // 256 x 256 diagonal-block inverses, ld 256
+  const double* rhs;  // forward: x; backward: t
+  double* y;          // m-vector scratch (per-block right-hand sides)
+  double* out;        // forward: t; backward: x
+  int* flags;         // [0, ns): y published, [ns, 2 ns): out published
+  int epoch;
+  int* err;
+};
+
+// thread 0 polls flags[lo, hi) until each equals epoch; false if it gave up
+// (or another work-group already had)
+__device__ bool trsv_wait(const int* flags, int lo, int hi, int epoch, int* err) {
+  __shared__ int ok;
+  if (threadIdx.x == 0) {
+    int good = 1;
+    for (int f = lo; f < hi && good; ++f) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&flags[f], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            ++spins > kSpinLimit) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          good = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+__device__ void trsv_publish(int* flag, int epoch) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// v[0..n) <- src[0..n) through device-scope loads (n <= 256, one per thread)
+__device__ void trsv_fetch(double* v, const double* src, int n) {
+  const int t = threadIdx.x;
+  v[t] = (t < n) ? __hip_atomic_load(const_cast<double*>(src) + t, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT)
+                 : 0.0;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_trsv_fwd(TrsvParams P) {
+  const int ns = gridDim.x, s = blockIdx.x, b = s >> 2, q = s & 3;
+  const int k0 = b * SB, kb = min(SB, P.m - k0);
+  const int t = threadIdx.x, i = t & 63, w = t >> 6;
+  const int r = s * 64 + i;
+  __shared__ double part[4][64];
+  __shared__ double vb[SB];
+  double acc = 0.0;
+  // rows of the strip times the earlier blocks' solutions
+  for (int bp = 0; bp < b; ++bp) {
+    // the matrix entries do not depend on the flag: load them first, so their
+    // latency overlaps the wait (rows past m read a clamped, valid row and are
+    // never stored)
+    const double* Lr = P.L + min(r, P.m - 1) + (size_t)P.ld * (bp * SB + 64 * w);
+    double lv[64];
+#pragma unroll
+    for (int c = 0; c < 64; ++c) lv[c] = Lr[(size_t)P.ld * c];
+    if (!trsv_wait(P.flags + ns, 4 * bp, 4 * bp + 4, P.epoch, P.err)) return;
+    trsv_fetch(vb, P.out + bp * SB, SB);
+    const double* v = vb + 64 * w;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 64; ++c) a[c & 3] += lv[c] * v[c];
+    acc += (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();  // vb is refilled next round
+  }
+  part[w][i] = acc;
+  __syncthreads();
+  if (w == 0 && r < P.m)
+    P.y[r] = P.rhs[r] - ((part[0][i] + part[1][i]) + (part[2][i] + part[3][i]));
+  trsv_publish(&P.flags[s], P.epoch);
+  // rows 64 q .. 64 q + 63 of t_b = W_b y_b (W_b lower triangular; only the
+  // column quarters w <= q are nonzero).  Columns past kb: a clamped (valid)
+  // entry of W times vb = 0; rows past kb read a clamped row, never stored.
+  const int row = 64 * q + i;
+  double lw[64];
+  if (w <= q) {
+    const double* Wr = P.W + (size_t)b * SB * SB + min(row, kb - 1);
+#pragma unroll
+    for (int c = 0; c < 64; ++c) lw[c] = Wr[(size_t)SB * min(64 * w + c, kb - 1)];
+  }
+  if (!trsv_wait(P.flags, 4 * b, min(4 * b + 4, ns), P.epoch, P.err)) return;
+  trsv_fetch(vb, P.y + k0, kb);
+  acc = 0.0;
+  if (w <= q) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 64; ++c) a[c & 3] += lw[c] * vb[64 * w + c];
+    acc = (a[0] + a[1]) + (a[2] + a[3]);
+  }
+  part[w][i] = acc;
+  __syncthreads();
+  if (w == 0 && r < P.m) P.out[r] = (part[0][i] + part[1][i]) + (part[2][i] + part[3][i]);
+  trsv_publish(&P.flags[ns + s], P.epoch);
+}
+
+// backward: columns of L (contiguous); wave w owns output columns
+// 16 w .. 16 w + 15 of the strip, lanes run down the rows of a block and the
+// 16 per-lane partial sums are reduced through LDS at the end
+__global__ __launch_bounds__(256) void k_trsv_bwd(TrsvParams P) {
+  const int ns = gridDim.x, s = blockIdx.x, b = s >> 2, q = s & 3;
+  const int nb = (P.m + SB - 1) / SB;
+  const int k0 = b * SB, kb = min(SB, P.m - k0);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int c0 = s * 64 + 16 * w;  // first output column of this wave
+  __shared__ double red[4][16][65];
+  __shared__ double vb[SB];
+  __shared__ double col[64];
+  double acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+  for (int bp = nb - 1; bp > b; --bp) {
+    const int kbp = min(SB, P.m - bp * SB);
+    // the block's 64 matrix entries per thread are loaded before the wait;
+    // rows past the block and columns past m read clamped (valid) entries:
+    // the former are multiplied by vb = 0, the latter are never stored
+    double lv[4][16];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const double* Lc = P.L + (bp * SB + min(64 * rr + lane, kbp - 1));
+#pragma unroll
+      for (int j = 0; j < 16; ++j) lv[rr][j] = Lc[(size_t)P.ld * min(c0 + j, P.m - 1)];
+    }
+    if (!trsv_wait(P.flags + ns, 4 * bp, min(4 * bp + 4, ns), P.epoch, P.err)) return;
+    trsv_fetch(vb, P.out + bp * SB, kbp);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const double xv = vb[64 * rr + lane];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] += lv[rr][j] * xv;
+    }
+    __syncthreads();
+  }
+  // reduce the 16 columns over the lanes
+#pragma unroll
+  for (int j = 0; j < 16; ++j) red[w][j][lane] = acc[j];
+  __syncthreads();
+  {
+    const int j = t >> 2, part4 = t & 3;  // column j of the strip, quarter of the lanes
+    const int ww = j >> 4, jj = j & 15;
+    double sacc = 0.0;
+    for (int l = 16 * part4; l < 16 * part4 + 16; ++l) sacc += red[ww][jj][l];
+    sacc += __shfl_xor(sacc, 1, 64);
+    sacc += __shfl_xor(sacc, 2, 64);
+    const int r = s * 64 + j;
+    if (part4 == 0 && r < P.m) P.y[r] = P.rhs[r] - sacc;
+  }
+  trsv_publish(&P.flags[s], P.epoch);
+  // rows 64 q + j of x_b = W_b^T y_b: column 64 q + j of W_b, rows c >= 64 q + j
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+  {
+    double lv[4][16];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const double* Wc = P.W + (size_t)b * SB * SB + min(64 * rr + lane, kb - 1);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) lv[rr][j] = Wc[(size_t)SB * min(64 * q + 16 * w + j, kb - 1)];
+    }
+    if (!trsv_wait(P.flags, 4 * b, min(4 * b + 4, ns), P.epoch, P.err)) return;
+    trsv_fetch(vb, P.y + k0, kb);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const double yv = vb[64 * rr + lane];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] += lv[rr][j] * yv;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) red[w][j][lane] = acc[j];
+  __syncthreads();
+  {
+    const int j = t >> 2, part4 = t & 3;
+    const int ww = j >> 4, jj = j & 15;
+    double sacc = 0.0;
+    for (int l = 16 * part4; l < 16 * part4 + 16; ++l) sacc += red[ww][jj][l];
+    sacc += __shfl_xor(sacc, 1, 64);
+    sacc += __shfl_xor(sacc, 2, 64);
+    const int r = s * 64 + j;
+    if (part4 == 0 && r < P.m) col[j] = sacc;
+    __syncthreads();
+    if (t < 64 && s * 64 + t < P.m) P.out[s * 64 + t] = col[t];
+  }
+  trsv_publish(&P.flags[ns + s], P.epoch);
+}
+
+// x := L^-T L^-1 x by the two cooperative launches; returns 1 when the
+// device cannot launch them cooperatively (the caller falls back)
+static int chol_solve_coop(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
+                           const double* Winv, double* x) {
+  if (w->coop_state == 0) {
+    int dev = 0, coop = 0;
+    SDPA_HIP_CHECK(hipGetDevice(&dev));
+    SDPA_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    w->coop_state = coop ? 1 : 2;
+  }
+  if (w->coop_state != 1) return 1;
+  const int ns = (m + 63) / 64;
+  if (int e = w->ensure_vec(3 * (size_t)m)) return e;
+  if (int e = w->ensure_flags(2 * ns)) return e;
+  TrsvParams P;
+  P.L = L; P.ld = ld; P.m = m; P.W = Winv; P.flags = w->d_flags; P.err = w->d_err;
+  double* t = w->d_vec;
+  double* y = w->d_vec + m;
+  // forward: t = L^-1 x
+  P.rhs = x; P.y = y; P.out = t; P.epoch = ++w->epoch;
+  void* args[] = {&P};
+  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_fwd, dim3(ns), dim3(256), args, 0, s));
+  // backward: x = L^-T t
+  TrsvParams Q = P;
+  Q.rhs = t; Q.out = x; Q.epoch = ++w->epoch;
+  void* args2[] = {&Q};
+  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_bwd, dim3(ns), dim3(256), args2, 0, s));
+  SDPA_HIP_CHECK(hipMemcpyAsync(w->h_err, w->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
 int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, const double* Winv,
                double* x) {
   if (m <= 0) return 0;
+  if (m > 2 * SB) {
+    const int r = chol_solve_coop(s, w, L, ld, m, Winv, x);
+    if (r <= 0) return r;
+  }
   if (int e = w->ensure_vec((size_t)m)) return e;
   double* t = w->d_vec;
   const int nb = (m + SB - 1) / SB;

@@ -648,7 +648,12 @@ int sdpa_gpu_make_direction(sdpa_gpu_ctx* h, int infeasible) {
 int sdpa_gpu_dots(sdpa_gpu_ctx* h, int nq, const int* codes, double* out) {
   if (!h || !h->c.loaded) return -1;
   if (nq <= 0 || nq > 32) return -2;
-  return compute_dots(&h->c, nq, codes, out);
+  const int r = compute_dots(&h->c, nq, codes, out);
+  // compute_dots synchronizes the stream: the failure flag of the last
+  // cooperative Schur solve (copied to pinned memory behind it) is current
+  if (r == 0 && h->c.dw.h_err && *h->c.dw.h_err)
+    return fail(&h->c, -60, "a cooperative Schur solve gave up waiting (work-groups not co-resident)");
+  return r;
 }
 
 int sdpa_gpu_update(sdpa_gpu_ctx* h, double aP, double aD) {
