@@ -26,17 +26,30 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
+// 1/x to full FP64 accuracy without the IEEE divide sequence: the hardware
+// reciprocal estimate and two Newton steps (each doubles the correct bits).
+// x is a nonzero normal number here (Sturm pivots are kept away from 0).
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 // Smallest eigenvalue theta of the symmetric tridiagonal T (diagonal d[0..N),
 // off-diagonal e[0..N-1), in LDS) and the last component of its unit
 // eigenvector -- what the reference reads from dsteqr's output (the first
-// eigenvalue in ascending order and Q[k][0], sdpa_linear.cpp:98-127).  One
-// wave: multisection with Sturm counts (64 shifts per round, the interval of
-// the smallest eigenvalue shrinks 65-fold per round), then two steps of
-// inverse iteration from a shift just below theta, where T - sigma I is
-// positive definite and its LDL^T needs no pivoting.
-__device__ void tridiag_min_wave(int N, const double* d, const double* e, double& theta,
-                                 double& zlast) {
-  const int lane = threadIdx.x & 63;
+// eigenvalue in ascending order and Q[k][0], sdpa_linear.cpp:98-127).
+// All threads of the work-group (nthreads, a multiple of 64) take part:
+// multisection with Sturm counts, nthreads shifts per round (the interval of
+// the smallest eigenvalue shrinks nthreads+1-fold per round); then wave 0 runs
+// two steps of inverse iteration from a shift just below theta, where
+// T - sigma I is positive definite and its LDL^T needs no pivoting.
+// scratch: 2 * (nthreads / 64) doubles of LDS.  Call from all threads.
+__device__ void tridiag_min_block(int N, const double* d, const double* e, double& theta,
+                                  double& zlast, int nthreads, double* scratch) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = nthreads >> 6;
   const double eps = 2.220446049250313e-16;
   double lo = 1.0e300, hi = -1.0e300;
   for (int i = lane; i < N; i += 64) {
@@ -52,83 +65,90 @@ __device__ void tridiag_min_wave(int N, const double* d, const double* e, double
   const double pivmin = 1.0e-290 + eps * eps * tnorm * tnorm;
   lo -= 4.0 * eps * tnorm + pivmin;
   hi += 4.0 * eps * tnorm + pivmin;
+  int* firsts = reinterpret_cast<int*>(scratch);
   for (int round = 0; round < 20; ++round) {
-    const double x = lo + (hi - lo) * ((double)(lane + 1) / 65.0);
+    const double step = (hi - lo) / (double)(nthreads + 1);
+    const double x = lo + step * (double)(tid + 1);
     // Sturm count: eigenvalues below x
     int cnt = 0;
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
     for (int i = 1; i < N; ++i) {
-      q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+      q = (d[i] - x) - (e[i - 1] * e[i - 1]) * recip(q);
       if (fabs(q) < pivmin) q = -pivmin;
       cnt += q < 0.0;
     }
+    // the first shift (over the whole work-group) with at least one
+    // eigenvalue below it brackets the smallest eigenvalue
     const unsigned long long mask = __ballot(cnt >= 1);
-    const int first = mask ? (__ffsll((long long)mask) - 1) : 64;
-    const double nhi = (first < 64) ? __shfl(x, first, 64) : hi;
-    const double nlo = (first == 0) ? lo : __shfl(x, first > 0 ? first - 1 : 0, 64);
-    lo = (first == 64) ? __shfl(x, 63, 64) : nlo;
+    if (lane == 0) firsts[wv] = mask ? (__ffsll((long long)mask) - 1) + 64 * wv : nthreads;
+    __syncthreads();
+    int first = nthreads;
+    for (int w = 0; w < nw; ++w) first = min(first, firsts[w]);
+    __syncthreads();
+    const double nlo = lo + step * (double)first;
+    const double nhi = (first < nthreads) ? lo + step * (double)(first + 1) : hi;
+    lo = nlo;
     hi = nhi;
     if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
   }
   theta = 0.5 * (lo + hi);
-  // inverse iteration: LDL^T of T - sigma I (pivots piv_i), two solves from
-  // b = ones with a normalization between them.  The sequential sweeps run on
-  // every lane with the same scalars; lane i keeps entry i.
-  const double sigma = theta - (8.0 * eps * tnorm + pivmin);
-  double last = 1.0, nrm2 = 0.0;
-  {
-    double piv = 0.0, y = 0.0, zi = 0.0;
-    double pprev = 1.0, yprev = 0.0;
+  // inverse iteration on wave 0: LDL^T of T - sigma I (pivots piv_i), two
+  // solves from b = ones with a normalization between them.  The sequential
+  // sweeps run on every lane with the same scalars; lane i keeps entry i.
+  if (wv == 0) {
+    const double sigma = theta - (8.0 * eps * tnorm + pivmin);
+    double piv = 0.0, rpiv = 0.0, y = 0.0, zi = 0.0;
+    double rprev = 1.0, yprev = 0.0;
     for (int i = 0; i < N; ++i) {  // pivots and forward sweep L y = 1
       const double ei = (i > 0) ? e[i - 1] : 0.0;
-      double p = (d[i] - sigma) - ((i > 0) ? ei * ei / pprev : 0.0);
+      double p = (d[i] - sigma) - ((i > 0) ? ei * ei * rprev : 0.0);
       if (fabs(p) < pivmin) p = pivmin;
-      const double l = (i > 0) ? ei / pprev : 0.0;
+      const double rp = recip(p);
+      const double l = (i > 0) ? ei * rprev : 0.0;
       const double yi = 1.0 - l * yprev;
-      if (lane == i) { piv = p; y = yi; }
-      pprev = p;
+      if (lane == i) { piv = p; rpiv = rp; y = yi; }
+      rprev = rp;
       yprev = yi;
     }
-    // backward: z_i = y_i / piv_i - (e_i / piv_i) z_{i+1}
+    (void)piv;
+    // backward: z_i = (y_i - e_i z_{i+1}) / piv_i
     double znext = 0.0;
     for (int i = N - 1; i >= 0; --i) {
-      const double pi = __shfl(piv, i, 64), yi = __shfl(y, i, 64);
+      const double ri = __shfl(rpiv, i, 64), yi = __shfl(y, i, 64);
       const double ei = (i < N - 1) ? e[i] : 0.0;
-      const double zi2 = yi / pi - (ei / pi) * znext;
+      const double zi2 = (yi - ei * znext) * ri;
       if (lane == i) zi = zi2;
       znext = zi2;
     }
     // normalize and second solve with b = z
     double s2 = (lane < N) ? zi * zi : 0.0;
     for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
-    const double inv = 1.0 / sqrt(s2);
-    const double b = zi * inv;
-    pprev = 1.0;
+    const double b = zi / sqrt(s2);
+    rprev = 1.0;
     yprev = 0.0;
     for (int i = 0; i < N; ++i) {
       const double ei = (i > 0) ? e[i - 1] : 0.0;
-      const double p = __shfl(piv, i, 64);
-      const double l = (i > 0) ? ei / pprev : 0.0;
+      const double l = (i > 0) ? ei * rprev : 0.0;
       const double yi = __shfl(b, i, 64) - l * yprev;
       if (lane == i) y = yi;
-      pprev = p;
+      rprev = __shfl(rpiv, i, 64);
       yprev = yi;
     }
     znext = 0.0;
     for (int i = N - 1; i >= 0; --i) {
-      const double pi = __shfl(piv, i, 64), yi = __shfl(y, i, 64);
+      const double ri = __shfl(rpiv, i, 64), yi = __shfl(y, i, 64);
       const double ei = (i < N - 1) ? e[i] : 0.0;
-      const double zi2 = yi / pi - (ei / pi) * znext;
+      const double zi2 = (yi - ei * znext) * ri;
       if (lane == i) zi = zi2;
       znext = zi2;
     }
-    nrm2 = (lane < N) ? zi * zi : 0.0;
+    double nrm2 = (lane < N) ? zi * zi : 0.0;
     for (int o = 32; o > 0; o >>= 1) nrm2 += __shfl_xor(nrm2, o, 64);
-    last = __shfl(zi, N - 1, 64);
+    const double last = __shfl(zi, N - 1, 64);
+    zlast = last / sqrt(nrm2);
   }
-  zlast = last / sqrt(nrm2);
 }
 
 // Smallest eigenvalue of a symmetric n x n (n <= 64) matrix in LDS (ld 65),
@@ -212,7 +232,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // n > 20: the truncated Lanczos of Lal::getMinEigen on M, 256 threads per
-// (block, which); the matrix-vector product uses all four waves, the
+// (block, which); in the matrix-vector product each thread owns rows, the
 // tridiagonal eigenproblem runs on wave 0.
 __global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict__ Ms, long stride,
                                                         const long* __restrict__ off,
@@ -223,12 +243,11 @@ __global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict
   if (n <= 20) return;
   const double* M = Ms + which * stride + off[q];
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const bool w0 = tid < 64;
   __shared__ double dg[kMaxLanczos], od[kMaxLanczos];  // tridiagonal of the recurrence
   __shared__ double qv[kMaxN], qold[kMaxN], rv[kMaxN], wv[kMaxN];
   __shared__ double red[kLzThreads / 64];
   __shared__ double sh[4];  // theta, last eigenvector component
+  __shared__ double trisc[2 * (kLzThreads / 64)];
   double mn = 1.0e+51, mn_old = 1.0e+52, error = 1.0e+10;
   int k = 0, kk = 0;
   for (int i = tid; i < n; i += kLzThreads) { qv[i] = 0.0; rv[i] = 1.0; }
@@ -241,29 +260,23 @@ __global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict
     __syncthreads();
     for (int i = tid; i < n; i += kLzThreads) { qold[i] = qv[i]; qv[i] = rv[i] * inv; }
     __syncthreads();
-    // w = M q: one wave per row block of 4 rows; M is symmetric, so row i is
-    // read as column i (contiguous), lanes over j, four rows in flight.
-    {
-      const int wv_id = tid >> 6;
-      for (int i0 = 4 * wv_id; i0 < n; i0 += 4 * (kLzThreads / 64)) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        const double* c0 = M + (size_t)n * i0;
-        const bool h1 = i0 + 1 < n, h2 = i0 + 2 < n, h3 = i0 + 3 < n;
-        for (int j = lane; j < n; j += 64) {
-          const double qj = qv[j];
-          s0 += c0[j] * qj;
-          if (h1) s1 += c0[n + j] * qj;
-          if (h2) s2 += c0[2 * (size_t)n + j] * qj;
-          if (h3) s3 += c0[3 * (size_t)n + j] * qj;
-        }
-        s0 = wsum(s0); s1 = wsum(s1); s2 = wsum(s2); s3 = wsum(s3);
-        if (lane == 0) {
-          wv[i0] = s0;
-          if (h1) wv[i0 + 1] = s1;
-          if (h2) wv[i0 + 2] = s2;
-          if (h3) wv[i0 + 3] = s3;
-        }
+    // w = M q: one thread per row i, lanes over consecutive rows, so each
+    // step j reads 64 consecutive entries of column j (coalesced); q_j is an
+    // LDS broadcast.  32 columns of loads in flight per thread, no cross-lane
+    // reduction.
+    for (int i = tid; i < n; i += kLzThreads) {
+      const double* Mi = M + i;
+      double a[4] = {0.0, 0.0, 0.0, 0.0};
+      int j = 0;
+      for (; j + 32 <= n; j += 32) {
+        double mv[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) mv[u] = Mi[(size_t)n * (j + u)];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) a[u & 3] += mv[u] * qv[j + u];
       }
+      for (; j < n; ++j) a[j & 3] += Mi[(size_t)n * j] * qv[j];
+      wv[i] = (a[0] + a[1]) + (a[2] + a[3]);
     }
     __syncthreads();
     double a = 0.0;
@@ -274,10 +287,10 @@ __global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict
     if (kk >= sqrt((double)k) || k == n - 1 || k > sqrt((double)n + 9)) {
       kk = 0;
       __syncthreads();
-      if (w0) {
-        double th, zl;
-        tridiag_min_wave(k + 1, dg, od, th, zl);
-        if (lane == 0) { sh[0] = th; sh[1] = zl; }
+      {
+        double th = 0.0, zl = 0.0;
+        tridiag_min_block(k + 1, dg, od, th, zl, kLzThreads, trisc);
+        if (tid == 0) { sh[0] = th; sh[1] = zl; }
       }
       __syncthreads();
       mn_old = mn;
