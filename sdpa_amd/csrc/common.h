@@ -89,6 +89,13 @@ struct DenseWork {
   int* h_err = nullptr;
   int coop_state = 0;  // 0 unknown, 1 cooperative launches available, 2 not
   int ensure_flags(int n);
+  // epoch-tagged words of the pipelined solves (2 per value, zeroed when
+  // allocated) and the largest block count known to fit co-resident
+  unsigned long long* d_ll = nullptr;
+  size_t cap_ll = 0;
+  int ll_max_nb = -1;
+  unsigned ll_epoch = 0;  // last epoch used in d_ll (reset only with a fresh, zeroed d_ll)
+  int ensure_ll(size_t nwords);
   // look-ahead Cholesky: side stream (default priority) and per-panel events
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> evp, evn;

@@ -184,6 +184,16 @@ int DenseWork::ensure_flags(int n) {
   epoch = 0;
   return 0;
 }
+int DenseWork::ensure_ll(size_t nwords) {
+  if (nwords <= cap_ll) return 0;
+  if (d_ll) SDPA_HIP_CHECK(hipFree(d_ll));
+  cap_ll = nwords;
+  SDPA_HIP_CHECK(hipMalloc(&d_ll, sizeof(unsigned long long) * cap_ll));
+  SDPA_HIP_CHECK(hipMemset(d_ll, 0, sizeof(unsigned long long) * cap_ll));
+  SDPA_HIP_CHECK(hipDeviceSynchronize());
+  ll_epoch = 0;
+  return 0;
+}
 int DenseWork::ensure_vec(size_t n) {
   if (n <= cap_vec) return 0;
   if (d_vec) SDPA_HIP_CHECK(hipFree(d_vec));
@@ -213,6 +223,8 @@ void DenseWork::release() {
   if (d_err) (void)hipFree(d_err);
   if (h_err) (void)hipHostFree(h_err);
   d_flags = nullptr; d_err = nullptr; h_err = nullptr; cap_flags = 0; epoch = 0;
+  if (d_ll) (void)hipFree(d_ll);
+  d_ll = nullptr; cap_ll = 0; ll_max_nb = -1; ll_epoch = 0;
   if (d_items) (void)hipFree(d_items);
   if (d_mats) (void)hipFree(d_mats);
   if (d_dinv) (void)hipFree(d_dinv);
@@ -1137,11 +1149,11 @@ __global__ __launch_bounds__(256) void k_bwd_update(const double* __restrict__ L
 // forms its rows of t_b = W_b y_b and publishes them.  Backward (L^T x = t)
 // runs the blocks from the last to the first with the columns of L.  Blocks
 // are published by release stores of a per-strip flag (value = this launch's
-// epoch, so flags never need clearing); readers poll with acquire loads,
+// epoch, so flags never need clearing); readers poll with relaxed loads and fence once (acquire),
 // then read the published vector with device-scope atomic loads.  Every poll
 // is bounded: after kSpinLimit polls the work-group records the failure in
 // *err and returns, so a launch always ends.
-constexpr unsigned kSpinLimit = 1u << 22;  // ~0.5 s at the poll interval
+constexpr unsigned kSpinLimit = 1u << 21;  // about a second of polling
 
 struct TrsvParams {
   const double* L;
@@ -1156,23 +1168,26 @@ struct TrsvParams {
 };
 
 // thread 0 polls flags[lo, hi) until each equals epoch; false if it gave up
-// (or another work-group already had)
+// (or another work-group already had).  The polls are relaxed loads and one
+// acquire fence follows the last: an acquire load per poll invalidates the
+// L2 each time, which made a hop ~4.7 us instead of ~1.4 us with 94 pollers.
 __device__ bool trsv_wait(const int* flags, int lo, int hi, int epoch, int* err) {
   __shared__ int ok;
   if (threadIdx.x == 0) {
     int good = 1;
     for (int f = lo; f < hi && good; ++f) {
       unsigned spins = 0;
-      while (__hip_atomic_load(&flags[f], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-            ++spins > kSpinLimit) {
+      while (__hip_atomic_load(&flags[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > kSpinLimit ||
+            ((spins & 63) == 0 &&
+             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           good = 0;
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     ok = good;
   }
   __syncthreads();
@@ -1341,6 +1356,429 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(TrsvParams P) {
   trsv_publish(&P.flags[ns + s], P.epoch);
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined solves.  The same blocks (256 rows), strips (64 rows, one
+// work-group each) and diagonal-block inverses W_b as above, with three
+// changes that shorten each step of the chain:
+//  - values pass between work-groups as epoch-tagged 64-bit words: value v
+//    of a launch with epoch e is the pair (lo32(v) | e << 32, hi32(v) | e << 32),
+//    each word written by one atomic store, so a reader that sees e in both
+//    words has the value: no flag, fence or second read;
+//  - one wave of each work-group (the poller) does all the waiting and
+//    fetching, into an LDS copy of every finished block of the solution,
+//    while eight compute waves run the products;
+//  - the compute waves issue the loads of the next tile of L before they use
+//    the current one, so a strip's reads of L stream while the chain advances.
+// 576 threads: compute waves 0..7, poller wave 8.  Every wave passes the
+// same barriers: the loops run on a count the poller posts in LDS before
+// each barrier.
+constexpr int kLLThreads = 576;
+
+// Work-group barrier that orders LDS only.  __syncthreads() also waits for
+// every outstanding global load, which would stall the compute waves on the
+// tile they have just started to prefetch; nothing passes between the waves
+// of these kernels through global memory (the tagged words carry their own
+// validity).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Raw buffer loads: a 32-bit per-lane byte offset plus a wave-uniform one in a
+// scalar register, so 32 loads in flight cost 32 data registers and one
+// offset register rather than a 64-bit address each.  (Buffers of < 2 GiB;
+// reads past num_bytes return 0.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double* p, size_t num_bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0, (int)num_bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+struct LLParams {
+  const double* L;
+  int ld, m;
+  const double* W;            // 256 x 256 diagonal-block inverses, ld 256
+  const double* rhs;          // forward: x; backward: t (plain vectors)
+  double* out;                // forward: t; backward: x
+  unsigned long long* ybuf;   // tagged words of y (2 per value)
+  unsigned long long* sbuf;   // tagged words of the block solutions
+  unsigned epoch;
+  int* err;
+};
+
+__device__ __forceinline__ void ll_put(unsigned long long* p, double v, unsigned e) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long tag = (unsigned long long)e << 32;
+  __hip_atomic_store(p, (u & 0xffffffffull) | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, (u >> 32) | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// poller wave: values [i0, i0 + 256) of a tagged buffer into dst (LDS),
+// indices >= m as 0.  Waits until all carry epoch e; false if it gave up
+// (spin limit, or another work-group already had).  It polls one word per
+// 64-value strip (the strip's last value) and reads the block only once all
+// four show e, so that the many work-groups waiting on one block do not
+// flood its memory channel; dist (blocks between this one and the one the
+// caller needs next) lengthens the pause between polls of blocks that are
+// not yet urgent.  The block needed next (dist 0) is read whole each time.
+__device__ bool ll_get_block(const unsigned long long* buf, int i0, int m, unsigned e, double* dst,
+                             int lane, int* err, int dist) {
+  unsigned spins = 0;
+  const size_t probe = (size_t)min(i0 + 64 * (lane & 3) + 63, m - 1);
+  while (true) {
+    // the block needed next is read whole at once (one round trip); the
+    // others are probed first
+    const unsigned long long pw =
+        dist == 0 ? 0ull
+                  : __hip_atomic_load(buf + 2 * probe + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (dist == 0 || __all((unsigned)(pw >> 32) == e)) {
+      unsigned long long w0[4], w1[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const size_t idx = (size_t)min(i0 + lane + 64 * k, m - 1);
+        w0[k] = __hip_atomic_load(buf + 2 * idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w1[k] = __hip_atomic_load(buf + 2 * idx + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // written straight to dst: if some word is stale the block is read again
+      // (nobody reads dst before this returns true)
+      bool ok = true;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        ok = ok && (unsigned)(w0[k] >> 32) == e && (unsigned)(w1[k] >> 32) == e;
+        const long long bits = (long long)((w0[k] & 0xffffffffull) | (w1[k] << 32));
+        dst[lane + 64 * k] = (i0 + lane + 64 * k < m) ? __longlong_as_double(bits) : 0.0;
+      }
+      if (__all(ok)) break;
+    }
+    ++spins;
+    if (spins > kSpinLimit ||
+        ((spins & 63) == 0 &&
+         __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+      if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    for (int d = 0; d < dist; ++d) __builtin_amdgcn_s_sleep(8);
+  }
+  return true;
+}
+
+// forward, L t = x.  Strip s (rows 64 s + lane) of block b: y = x - sum over
+// earlier blocks of L[rows, block] t_block (compute wave w takes columns
+// 32 w .. 32 w + 31 of each block), then t_b = W_b y_b for its rows.
+__global__ __launch_bounds__(kLLThreads) void k_trsv_fwd_ll(LLParams P) {
+  extern __shared__ double sol[];  // finished blocks of t
+  __shared__ double ysh[SB];
+  __shared__ double part[8][64];
+  __shared__ int post[2];
+  __shared__ int yok;
+  const int s = blockIdx.x, b = s >> 2, q = s & 3;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  const bool poller = w == 8;
+  const int m = P.m, k0 = b * SB, kb = min(SB, m - k0);
+  const int r = s * 64 + lane;
+  // wave-uniform column bases (scalar registers) plus a 32-bit per-lane row
+  // offset: rows past m read a valid row and are never stored
+  const int wc = min(w, 7);
+  const unsigned rl = 8u * (unsigned)min(r, m - 1);
+  const unsigned ldb = 8u * (unsigned)P.ld;
+  const __amdgpu_buffer_rsrc_t Lrs = buf_rsrc(P.L, (size_t)P.ld * m * sizeof(double));
+  // this thread's W_b entries: row 64 q + lane, columns 32 w + c, clamped into
+  // the kb x kb block (clamped columns meet ysh = 0, clamped rows are not stored)
+  const __amdgpu_buffer_rsrc_t Wrs = buf_rsrc(P.W + (size_t)b * SB * SB, SB * SB * sizeof(double));
+  const unsigned wl = 8u * (unsigned)min(64 * q + lane, kb - 1);
+  double lv[32];
+  double acc = 0.0;
+#define LL_LOAD_L(dst, bp)                                                   \
+  _Pragma("unroll") for (int c = 0; c < 32; ++c)                             \
+    dst[c] = buf_ld(Lrs, rl, ldb * (unsigned)((bp) * SB + 32 * wc + c));
+#define LL_LOAD_W(dst)                                                       \
+  _Pragma("unroll") for (int c = 0; c < 32; ++c)                             \
+    dst[c] = buf_ld(Wrs, wl, 8u * SB * (unsigned)min(32 * wc + c, kb - 1));
+  // lw: this thread's W_b tile (waves whose columns meet the lower triangle),
+  // loaded once up front.  lv: the background tile of the next round.  A
+  // round multiplies lv and then issues the loads of the following tile into
+  // lv, so they land while the work-group waits for the next block (a second
+  // buffer would need register copies, whose reads wait for the loads just
+  // issued).  The last tile, the one the chain waits for, is multiplied after
+  // the loop with nothing issued behind it: 128 KB of loads take the CU about
+  // a microsecond to issue.
+  double lw[32];
+  const double xr = (w == 0 && r < m) ? P.rhs[r] : 0.0;  // read early: off the chain
+  if (!poller) {
+    if (w <= 2 * q + 1) LL_LOAD_W(lw)
+    if (b > 0) LL_LOAD_L(lv, 0)
+  }
+  // lv times 32 values of the LDS copy, read 8 at a time ahead of the products
+#define LL_FMA_TILE(tvp)                                                      \
+  {                                                                           \
+    const double* tv_ = (tvp);                                                \
+    double a[4] = {0.0, 0.0, 0.0, 0.0};                                       \
+    _Pragma("unroll") for (int h8 = 0; h8 < 32; h8 += 8) {                    \
+      double t8[8];                                                           \
+      _Pragma("unroll") for (int c = 0; c < 8; ++c) t8[c] = tv_[h8 + c];      \
+      _Pragma("unroll") for (int c = 0; c < 8; ++c)                           \
+        a[c & 3] = fma(lv[h8 + c], t8[c], a[c & 3]);                          \
+    }                                                                         \
+    acc += (a[0] + a[1]) + (a[2] + a[3]);                                     \
+  }
+  int have = 0, done = 0;  // tiles fetched (poller) / multiplied (compute waves)
+  for (int it = 0;; ++it) {
+    if (poller) {
+      int h = have;
+      if (h < b)
+        h = ll_get_block(P.sbuf, h * SB, m, P.epoch, sol + h * SB, lane, P.err, min(b - 1 - h, 8))
+                ? h + 1
+                : -1;
+      if (lane == 0) post[it & 1] = h;
+    }
+    lds_barrier();
+    const int h = __builtin_amdgcn_readfirstlane(post[it & 1]);  // uniform: scalar offsets
+    if (h < 0) return;
+    if (!poller) {
+      for (; done < h && done < b - 1; ++done) {
+        LL_FMA_TILE(sol + done * SB + 32 * wc)
+        // keep the reloads below the products (hoisted, they would need a
+        // second set of registers); the reload is of a tile <= b - 1
+        __asm__ volatile("" ::: "memory");
+        LL_LOAD_L(lv, done + 1)
+      }
+    }
+    have = h;
+    if (h >= b) break;
+  }
+  if (!poller && b > 0) LL_FMA_TILE(sol + (b - 1) * SB + 32 * wc)
+#undef LL_FMA_TILE
+#undef LL_LOAD_L
+#undef LL_LOAD_W
+  // y for the strip's rows
+  if (!poller) part[w][lane] = acc;
+  lds_barrier();
+  if (w == 0 && r < m) {
+    const double sum = ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) +
+                       ((part[4][lane] + part[5][lane]) + (part[6][lane] + part[7][lane]));
+    ll_put(P.ybuf + 2 * (size_t)r, xr - sum, P.epoch);
+  }
+  if (poller) {
+    const bool ok = ll_get_block(P.ybuf, k0, m, P.epoch, ysh, lane, P.err, 0);
+    if (lane == 0) yok = ok ? 1 : 0;
+  }
+  lds_barrier();
+  if (!yok) return;
+  // t_b rows: W_b[64 q + lane, :] y_b (columns past 64 q + 63 are zero)
+  double u = 0.0;
+  if (!poller && w <= 2 * q + 1) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 32; ++c) a[c & 3] = fma(lw[c], ysh[32 * w + c], a[c & 3]);
+    u = (a[0] + a[1]) + (a[2] + a[3]);
+  }
+  if (!poller) part[w][lane] = u;
+  lds_barrier();
+  if (w == 0 && r < m) {
+    const double tv = ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) +
+                      ((part[4][lane] + part[5][lane]) + (part[6][lane] + part[7][lane]));
+    ll_put(P.sbuf + 2 * (size_t)r, tv, P.epoch);
+    P.out[r] = tv;
+  }
+}
+
+// backward, L^T x = t.  Strip s (columns 64 s .. 64 s + 63 of L) of block b:
+// y = t - sum over later blocks of L[block, columns]^T x_block (compute wave
+// w takes columns 64 s + 8 w + j, lanes run down the block's rows), then
+// x_b = W_b^T y_b for its columns.
+__device__ __forceinline__ double ll_colsum(double (*red)[8][65], int tid) {
+  // thread tid < 512: column cc = tid >> 3 of the strip, lanes 8 p .. 8 p + 7
+  const int cc = tid >> 3, p = tid & 7;
+  const double* rr = red[cc >> 3][cc & 7] + 8 * p;
+  double v = ((rr[0] + rr[1]) + (rr[2] + rr[3])) + ((rr[4] + rr[5]) + (rr[6] + rr[7]));
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kLLThreads) void k_trsv_bwd_ll(LLParams P) {
+  extern __shared__ double sol[];  // finished blocks of x
+  __shared__ double ysh[SB];
+  __shared__ double red[8][8][65];
+  __shared__ int post[2];
+  __shared__ int yok;
+  const int s = blockIdx.x, b = s >> 2, q = s & 3;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  const bool poller = w == 8;
+  const int m = P.m, nb = (m + SB - 1) / SB, k0 = b * SB, kb = min(SB, m - k0);
+  const int wc = min(w, 7);
+  const int c0 = s * 64 + 8 * wc;  // this wave's first column
+  const int nbg = nb - 1 - b;      // background tiles: block nb - 1 - i, i < nbg
+  // columns c0 + j are < m whenever there is background (b < nb - 1); the
+  // column bases are wave-uniform, the rows a 32-bit per-lane offset
+  const unsigned ldb = 8u * (unsigned)P.ld;
+  const unsigned cb = ldb * (unsigned)min(c0, m - 8);
+  const __amdgpu_buffer_rsrc_t Lrs = buf_rsrc(P.L, (size_t)P.ld * m * sizeof(double));
+  const __amdgpu_buffer_rsrc_t Wrs = buf_rsrc(P.W + (size_t)b * SB * SB, SB * SB * sizeof(double));
+  double lv[32];
+  double acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+  // W_b^T tile: rows 64 k + lane of W_b (k >= q: the rows above are zero),
+  // columns 64 q + 8 w + j, clamped into the kb x kb block (clamped rows meet
+  // ysh = 0, clamped columns are not stored)
+#define LL_LOAD_W(dst)                                                                   \
+  _Pragma("unroll") for (int k = 0; k < 4; ++k)                                          \
+    _Pragma("unroll") for (int j = 0; j < 8; ++j)                                        \
+      dst[8 * k + j] = (k >= q) ? buf_ld(Wrs, 8u * (unsigned)min(64 * k + lane, kb - 1),  \
+                                         8u * SB * (unsigned)min(64 * q + 8 * wc + j, kb - 1)) \
+                                : 0.0;
+#define LL_LOAD_L(dst, bp)                                                               \
+  _Pragma("unroll") for (int k = 0; k < 4; ++k)                                          \
+    _Pragma("unroll") for (int j = 0; j < 8; ++j)                                        \
+      dst[8 * k + j] = buf_ld(Lrs, 8u * (unsigned)min((bp) * SB + 64 * k + lane, m - 1), cb + ldb * j);
+  // lv: the next background tile, reloaded right after each use (as in the
+  // forward kernel).  The W_b^T tile (lw) is loaded once y_b is published,
+  // so the loads overlap its exchange: held through the loop it would not
+  // fit in registers beside lv, and issued before the last tile's products
+  // it would delay the chain.
+  double lw[32];
+  const int col = s * 64 + (tid >> 3);  // the column ll_colsum gives this thread
+  const double xr = (!poller && (tid & 7) == 0 && col < m) ? P.rhs[col] : 0.0;  // read early
+  if (!poller && nbg > 0) LL_LOAD_L(lv, nb - 1)
+  int have = 0, done = 0;  // tiles fetched (poller) / multiplied (compute waves)
+  for (int it = 0;; ++it) {
+    if (poller) {
+      int h = have;
+      if (h < nbg) {
+        const int bp = nb - 1 - h;
+        h = ll_get_block(P.sbuf, bp * SB, m, P.epoch, sol + bp * SB, lane, P.err,
+                         min(bp - b - 1, 8))
+                ? h + 1
+                : -1;
+      }
+      if (lane == 0) post[it & 1] = h;
+    }
+    lds_barrier();
+    const int h = __builtin_amdgcn_readfirstlane(post[it & 1]);  // uniform: scalar offsets
+    if (h < 0) return;
+    if (!poller) {
+      // all but the last tile here; every reload is of a tile < nbg
+      for (; done < h && done < nbg - 1; ++done) {
+        const double* xv = sol + (nb - 1 - done) * SB + lane;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double xk = xv[64 * k];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) acc[j] = fma(lv[8 * k + j], xk, acc[j]);
+        }
+        __asm__ volatile("" ::: "memory");  // reloads stay below the products
+        LL_LOAD_L(lv, nb - 2 - done)
+      }
+    }
+    have = h;
+    if (h >= nbg) break;
+  }
+  if (!poller) {
+    if (nbg > 0) {  // the last tile: block b + 1
+      const double* xv = sol + (b + 1) * SB + lane;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double xk = xv[64 * k];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] = fma(lv[8 * k + j], xk, acc[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) red[w][j][lane] = acc[j];
+  }
+  lds_barrier();
+  if (!poller) {
+    const double v = ll_colsum(red, tid);
+    if ((tid & 7) == 0 && col < m) ll_put(P.ybuf + 2 * (size_t)col, xr - v, P.epoch);
+    LL_LOAD_W(lw)
+  } else {
+    const bool ok = ll_get_block(P.ybuf, k0, m, P.epoch, ysh, lane, P.err, 0);
+    if (lane == 0) yok = ok ? 1 : 0;
+  }
+  lds_barrier();
+  if (!yok) return;
+#undef LL_LOAD_W
+#undef LL_LOAD_L
+  if (!poller) {
+    double a2[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a2[j] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double yk = ysh[64 * k + lane];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a2[j] = fma(lw[8 * k + j], yk, a2[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) red[w][j][lane] = a2[j];
+  }
+  lds_barrier();
+  if (!poller) {
+    const double v = ll_colsum(red, tid);
+    if ((tid & 7) == 0 && col < m) {
+      ll_put(P.sbuf + 2 * (size_t)col, v, P.epoch);
+      P.out[col] = v;
+    }
+  }
+}
+
+// x := L^-T L^-1 x by the two pipelined cooperative launches; returns 1 when
+// they do not apply (no cooperative launch, or the grid cannot be co-resident
+// with its LDS) and the caller falls back
+static int chol_solve_ll(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
+                         const double* Winv, double* x) {
+  const int ns = (m + 63) / 64, nb = (m + SB - 1) / SB;
+  const size_t lds = (size_t)nb * SB * sizeof(double);
+  if (nb > 48 || (size_t)ld * m * sizeof(double) >= (size_t)1 << 31) return 1;
+  if (w->ll_max_nb < nb) {
+    int dev = 0, coop = 0, ncu = 0;
+    SDPA_HIP_CHECK(hipGetDevice(&dev));
+    SDPA_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    SDPA_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (!coop) return 1;
+    SDPA_HIP_CHECK(hipFuncSetAttribute((const void*)k_trsv_fwd_ll,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    SDPA_HIP_CHECK(hipFuncSetAttribute((const void*)k_trsv_bwd_ll,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int pf = 0, pb = 0;
+    SDPA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pf, k_trsv_fwd_ll, kLLThreads, lds));
+    SDPA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pb, k_trsv_bwd_ll, kLLThreads, lds));
+    if ((long)std::min(pf, pb) * ncu < ns) return 1;
+    w->ll_max_nb = nb;
+  }
+  if (int e = w->ensure_vec((size_t)m)) return e;
+  if (int e = w->ensure_flags(0)) return e;  // the failure flag
+  if (int e = w->ensure_ll(4 * (size_t)nb * SB)) return e;
+  LLParams P;
+  P.L = L; P.ld = ld; P.m = m; P.W = Winv; P.err = w->d_err;
+  P.ybuf = w->d_ll;
+  P.sbuf = w->d_ll + 2 * (size_t)nb * SB;
+  double* t = w->d_vec;
+  // forward: t = L^-1 x
+  if (++w->ll_epoch == 0) ++w->ll_epoch;
+  P.rhs = x; P.out = t; P.epoch = w->ll_epoch;
+  void* args[] = {&P};
+  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_fwd_ll, dim3(ns), dim3(kLLThreads),
+                                            args, lds, s));
+  // backward: x = L^-T t
+  LLParams Q = P;
+  if (++w->ll_epoch == 0) ++w->ll_epoch;
+  Q.rhs = t; Q.out = x; Q.epoch = w->ll_epoch;
+  void* args2[] = {&Q};
+  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_bwd_ll, dim3(ns), dim3(kLLThreads),
+                                            args2, lds, s));
+  SDPA_HIP_CHECK(hipMemcpyAsync(w->h_err, w->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
 // x := L^-T L^-1 x by the two cooperative launches; returns 1 when the
 // device cannot launch them cooperatively (the caller falls back)
 static int chol_solve_coop(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
@@ -1376,7 +1814,9 @@ int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, cons
                double* x) {
   if (m <= 0) return 0;
   if (m > 2 * SB) {
-    const int r = chol_solve_coop(s, w, L, ld, m, Winv, x);
+    int r = chol_solve_ll(s, w, L, ld, m, Winv, x);
+    if (r <= 0) return r;
+    r = chol_solve_coop(s, w, L, ld, m, Winv, x);
     if (r <= 0) return r;
   }
   if (int e = w->ensure_vec((size_t)m)) return e;
