@@ -138,6 +138,18 @@ class SDPA {
   void setInternalInitPoint(const double* X, const double* Z, const double* y);
   double phaseTimes[5] = {0, 0, 0, 0, 0};  // bmat, chol, solve, eig, inv (ms; when timing on)
   void enableTiming(bool on) { timing_ = on; }
+  // Extension with no counterpart in the reference: precision 1 factors the
+  // Schur matrix in mixed precision (FP32 factor of the equilibrated matrix
+  // on FP32 MFMA, solutions refined to FP64 accuracy against the FP64
+  // matrix) while mu > muSwitch and the refinement converges; from the first
+  // stall (or FP32 pivot failure) on, FP64 as the reference.  0 (default):
+  // FP64 only.
+  void setPrecision(int precision, double muSwitch = 0.0) {
+    precision_ = precision;
+    muSwitch_ = muSwitch;
+  }
+  int precision() const { return precision_; }
+  double muSwitch() const { return muSwitch_; }
   int failureCode() const { return failure_; }  // 1 chol, 2 step, 3 inverse
 
  private:
@@ -152,6 +164,8 @@ class SDPA {
   FILE* fpout_ = nullptr;
   bool isInitPoint_ = false;
   bool timing_ = false;
+  int precision_ = 0;
+  double muSwitch_ = 0.0;
   ParameterType paramType_ = PARAMETER_DEFAULT;
   std::string xPrint_ = "%+8.3e", XPrint_ = "%+8.3e", YPrint_ = "%+8.3e",
               infPrint_ = "%+10.16e";
@@ -191,7 +205,8 @@ sdpa_solver* sdpa_solver_new(int device);
 void sdpa_solver_free(sdpa_solver* s);
 const char* sdpa_solver_error(sdpa_solver* s);
 /* names: maxIteration epsilonStar lambdaStar omegaStar lowerBound upperBound
- * betaStar betaBar gammaStar epsilonDash kappa */
+ * betaStar betaBar gammaStar epsilonDash kappa, and the extensions
+ * precision (0 FP64, 1 mixed) and muSwitch (see SDPA::setPrecision) */
 int sdpa_solver_set_param(sdpa_solver* s, const char* name, double value);
 int sdpa_solver_load(sdpa_solver* s, int m, int nBlock, const int* blockStruct, const double* c,
                      long nnz, const int* k, const int* l, const int* i, const int* j,

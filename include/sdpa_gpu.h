@@ -68,16 +68,27 @@ int sdpa_gpu_keep_initial_point(sdpa_gpu_ctx* ctx);
  * a factorization fails. */
 int sdpa_gpu_inv_chol(sdpa_gpu_ctx* ctx);
 
-/* Newton::Make_bMat (sdpa_newton.cpp:2634-2670) into the device bMat.
- * precision: 0 = FP64. */
+/* Newton::Make_bMat (sdpa_newton.cpp:2634-2670) into the device bMat (FP64;
+ * precision is accepted for symmetry with the factorization and ignored). */
 int sdpa_gpu_make_bmat(sdpa_gpu_ctx* ctx, int precision);
 /* Lal::choleskyFactorWithAdjust -> rATL_dpotrfL (sdpa_linear.cpp:762-790,
- * sdpa_dpotrf.cpp:218-265): in-place lower Cholesky of bMat with SDPA's
- * pivot rule.  Returns 0 or the 1-based failing column. */
+ * sdpa_dpotrf.cpp:218-265): lower Cholesky of bMat with SDPA's pivot rule.
+ * precision 0: FP64, in place.  precision 1 (mixed; no counterpart in the
+ * reference): FP32 factor of the column-equilibrated bMat on FP32 MFMA, bMat
+ * kept in FP64 for refinement; where that factorization fails or m <= 512 it
+ * falls back to FP64 by itself.  Returns 0 or the 1-based failing column. */
 int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* ctx, int precision);
 /* Lal::solveSystems (sdpa_linear.cpp:843-862): dy = bMat^-1 gVec with the
- * factor left by sdpa_gpu_factor_bmat. */
+ * factor left by sdpa_gpu_factor_bmat.  After a mixed factorization the
+ * solution is refined in FP64 until the residual gVec - bMat dy is at FP64
+ * accuracy; returns 2 if the refinement stalls (dy is then not set: factor
+ * again with precision 0 and solve again). */
 int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* ctx, int refine);
+/* Mixed-precision counters (no counterpart in the reference): out[0]
+ * mixed factorizations, [1] fallbacks to FP64 at factorization, [2] refined
+ * solves, [3] refinement steps, [4] stalls, [5] 1 if the current factor is
+ * the mixed one. */
+int sdpa_gpu_mixed_stats(sdpa_gpu_ctx* ctx, long* out6);
 
 /* StepLength::computeStepLength's eigenvalue estimates (sdpa_parts.cpp:363-432,
  * Jal::getMinEigen sdpa_jordan.cpp:58-109): lamX = lambda_min(L_X^-1 Dx L_X^-T),

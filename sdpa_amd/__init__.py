@@ -83,6 +83,7 @@ def lib():
         L.sdpa_gpu_timing.argtypes = [V, C.c_int, V]
         L.sdpa_gpu_kernel_timing.argtypes = [V, V]
         L.sdpa_gpu_synchronize.argtypes = [V]
+        L.sdpa_gpu_mixed_stats.argtypes = [V, np.ctypeslib.ndpointer(np.int64)]
         L.sdpa_solver_new.restype = V
         L.sdpa_solver_new.argtypes = [C.c_int]
         L.sdpa_solver_free.argtypes = [V]
@@ -235,6 +236,13 @@ class GpuContext:
     def synchronize(self):
         self._chk(lib().sdpa_gpu_synchronize(self.h), "synchronize")
 
+    def mixed_stats(self):
+        """Mixed-precision counters: factorizations, fallbacks, refined solves,
+        refinement steps, stalls, current factor mixed (0/1)."""
+        out = np.zeros(6, np.int64)
+        self._chk(lib().sdpa_gpu_mixed_stats(self.h, out), "mixed_stats")
+        return out
+
 
 class Result:
     def __init__(self, vals, trace):
@@ -368,3 +376,14 @@ class Solver:
         if ctx:
             lib().sdpa_gpu_kernel_timing(ctx, out.ctypes.data_as(C.c_void_p))
         return float(out[0]), int(out[1]), float(out[2])
+
+    def mixed_stats(self):
+        """Mixed-precision counters of this solver's context (cumulative):
+        factorizations, fallbacks to FP64, refined solves, refinement steps,
+        stalls, current factor mixed."""
+        ctx = lib().sdpa_solver_context(self.h)
+        out = np.zeros(6, np.int64)
+        if ctx:
+            lib().sdpa_gpu_mixed_stats(ctx, out)
+        return dict(zip(("factors", "fallbacks", "solves", "refine_steps", "stalls", "current"),
+                        (int(v) for v in out)))

@@ -56,6 +56,16 @@ int gemm(hipStream_t s, DenseWork* w, const double* A, int lda, const double* B,
 // slot (item, k0/64), 64*64 doubles each, ceil(maxn/64) slots per item.
 int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitems, int mode,
                   int* d_info, double* dinv_out = nullptr);
+// Mixed-precision factorization: A32 (m x m floats, ld m) <- FP32 lower
+// Cholesky factor of S B S, S = diag(B_ii^-1/2) into scale; B (FP64, lower
+// triangle, ld) is left intact.  Panels are factored in FP64 in P64 (m x 256)
+// with SDPA's pivot rule; D64 receives the FP64 256x256 diagonal blocks of
+// the factor (ceil(m/256) blocks, ld 256) and dinv their 64x64 inverses.
+// *d_info: failing column or 0; *d_bad: a diagonal of B was not positive;
+// *d_bmax: bits of max |B_ij|.
+int potrf_mixed(hipStream_t s, DenseWork* w, const double* B, int ld, int m, float* A32,
+                double* scale, double* P64, double* D64, double* dinv, int* d_info, int* d_bad,
+                unsigned long long* d_bmax);
 // W = L^-1 for a batch of lower factors (upper parts of W zeroed).
 // dinv_pre (optional): the 64x64 diagonal inverses potrf_batched produced for
 // the same batch (slot (item, k0/64)); otherwise they are computed here.
@@ -70,6 +80,10 @@ int diag_block_inverses(hipStream_t s, DenseWork* w, const double* L, int ld, in
 // using the block inverses from diag_block_inverses.
 int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, const double* Winv,
                double* x);
+// The same with a float factor (mixed precision); returns 1 if the
+// pipelined kernels do not apply to this size (m <= 512 included).
+int chol_solve_f32(hipStream_t s, DenseWork* w, const float* L, int ld, int m, const double* Winv,
+                   double* x);
 
 struct DenseWork {
   GemmItem* d_items = nullptr;  // device array of descriptors
@@ -93,7 +107,7 @@ struct DenseWork {
   // allocated) and the largest block count known to fit co-resident
   unsigned long long* d_ll = nullptr;
   size_t cap_ll = 0;
-  int ll_max_nb = -1;
+  int ll_max_nb = -1, ll_max_nb32 = -1;
   unsigned ll_epoch = 0;  // last epoch used in d_ll (reset only with a fresh, zeroed d_ll)
   int ensure_ll(size_t nwords);
   // look-ahead Cholesky: side stream (default priority) and per-panel events

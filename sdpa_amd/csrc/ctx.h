@@ -91,6 +91,21 @@ struct Ctx {
   double* h_red = nullptr;  // pinned host mirror of reduction results
   int* h_info = nullptr;    // pinned
   double* d_eig = nullptr;  // per-item eigen results
+  // mixed precision (made on first use): the FP32 factor of S B S, the
+  // scaling, the FP64 panel and diagonal-block buffers, refinement vectors
+  float* d_B32 = nullptr;
+  double *d_scale = nullptr, *d_P64 = nullptr, *d_D64 = nullptr;
+  double *d_mr = nullptr, *d_mx = nullptr, *d_md = nullptr;
+  double* d_symv = nullptr;   // symmetric-product partials, nt * nt * 64
+  double* d_mnorm = nullptr;  // norms of the equilibrated residual, solution, rhs
+  double* h_mnorm = nullptr;  // pinned mirror
+  int* d_mflags = nullptr;    // [0] failing column, [1] non-positive diagonal
+  int* h_mflags = nullptr;    // pinned mirror
+  unsigned long long* d_bmax = nullptr;
+  int mixed_m = 0;            // the size the buffers were made for
+  int factor_mixed = 0;       // the current factor is the mixed one (B intact)
+  double mx_anorm = 0.0;      // ||S B S||_inf of that factorization
+  long mx_factors = 0, mx_fallbacks = 0, mx_solves = 0, mx_refine = 0, mx_stalls = 0;
   // timing (ms), accumulated since the last reset
   hipEvent_t ev[8];
   double t_bmat = 0, t_chol = 0, t_solve = 0, t_eig = 0, t_inv = 0;
@@ -118,5 +133,9 @@ int triple_product(Ctx* c, const double* A, const double* B, const double* C, do
 int lp_min_step(Ctx* c, const double* D, const double* L, double* out);
 // eig.hip
 int min_eig(Ctx* c, double* lamX, double* lamZ);
+// mixed.hip
+int factor_mixed(Ctx* c);  // 0 ok; otherwise the FP64 path must be used
+int solve_mixed(Ctx* c);   // 0 converged (dy set); 2 refinement stalled
+void free_mixed(Ctx* c);
 
 }  // namespace sdpa_amd

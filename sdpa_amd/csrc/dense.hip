@@ -224,7 +224,7 @@ void DenseWork::release() {
   if (h_err) (void)hipHostFree(h_err);
   d_flags = nullptr; d_err = nullptr; h_err = nullptr; cap_flags = 0; epoch = 0;
   if (d_ll) (void)hipFree(d_ll);
-  d_ll = nullptr; cap_ll = 0; ll_max_nb = -1; ll_epoch = 0;
+  d_ll = nullptr; cap_ll = 0; ll_max_nb = ll_max_nb32 = -1; ll_epoch = 0;
   if (d_items) (void)hipFree(d_items);
   if (d_mats) (void)hipFree(d_mats);
   if (d_dinv) (void)hipFree(d_dinv);
@@ -717,29 +717,29 @@ static void launch_syrk(hipStream_t s, const MatDesc* d_mats, const int* d_info,
 // diagonal block alone; every later step is one fused launch (the previous
 // step's rest-of-panel update with the next diagonal block); each step ends
 // with the rows below its diagonal block.
-static void factor_panel(hipStream_t s, DenseWork* w, int nitems, int maxn, int K0, int PW,
-                         int mode, int* d_info, double* dinv, int nslot) {
+static void factor_panel(hipStream_t s, const MatDesc* d_mats, int nitems, int maxn, int K0,
+                         int PW, int mode, int* d_info, double* dinv, int nslot) {
   const int pend = std::min(K0 + PW, maxn);
   for (int k0 = K0; k0 < pend; k0 += TB) {
     if (k0 == K0) {
       if (mode == 0)
-        hipLaunchKernelGGL(k_potrf_diag<0>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
+        hipLaunchKernelGGL(k_potrf_diag<0>, dim3(nitems), dim3(256), 0, s, d_mats, k0, d_info,
                            dinv, nslot);
       else
-        hipLaunchKernelGGL(k_potrf_diag<1>, dim3(nitems), dim3(256), 0, s, w->d_mats, k0, d_info,
+        hipLaunchKernelGGL(k_potrf_diag<1>, dim3(nitems), dim3(256), 0, s, d_mats, k0, d_info,
                            dinv, nslot);
     } else {
       const int nwg = 1 + (panel_step_tiles(maxn, k0, K0 + PW) + 3) / 4;
       if (mode == 0)
-        hipLaunchKernelGGL(k_panel_step<0>, dim3(nwg, nitems), dim3(256), 0, s, w->d_mats, k0,
+        hipLaunchKernelGGL(k_panel_step<0>, dim3(nwg, nitems), dim3(256), 0, s, d_mats, k0,
                            K0 + PW, d_info, dinv, nslot);
       else
-        hipLaunchKernelGGL(k_panel_step<1>, dim3(nwg, nitems), dim3(256), 0, s, w->d_mats, k0,
+        hipLaunchKernelGGL(k_panel_step<1>, dim3(nwg, nitems), dim3(256), 0, s, d_mats, k0,
                            K0 + PW, d_info, dinv, nslot);
     }
     const int rows = maxn - k0 - TB;
     if (rows <= 0) break;
-    hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 15) / 16, nitems), dim3(256), 0, s, w->d_mats,
+    hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 15) / 16, nitems), dim3(256), 0, s, d_mats,
                        k0, d_info, dinv, nslot);
   }
 }
@@ -825,7 +825,7 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
   const int np = (maxn + PW - 1) / PW;
   if (nitems > 1 || np <= 2) {
     for (int K0 = 0; K0 < maxn; K0 += PW) {
-      factor_panel(s, w, nitems, maxn, K0, PW, mode, d_info, dinv, nslot);
+      factor_panel(s, w->d_mats, nitems, maxn, K0, PW, mode, d_info, dinv, nslot);
       launch_syrk(s, w->d_mats, d_info, nitems, maxn, K0, PW, K0 + PW, maxn);
     }
   } else {
@@ -840,7 +840,7 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
     for (int p = 0; p < np; ++p) {
       const int K0 = p * PW;
       if (p > 0) SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));
-      factor_panel(s, w, 1, maxn, K0, PW, mode, d_info, dinv, nslot);
+      factor_panel(s, w->d_mats, 1, maxn, K0, PW, mode, d_info, dinv, nslot);
       if (K0 + PW >= maxn) break;
       SDPA_HIP_CHECK(hipEventRecord(w->evp[p], s));
       SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
@@ -865,6 +865,269 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
     SDPA_HIP_CHECK(hipEventRecord(w->evjoin, w->side));
     SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evjoin, 0));
   }
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision factorization of the Schur matrix.  B (FP64, lower
+// triangle) is left intact for the refinement residuals.  A32 = S B S in
+// FP32, S = diag(B_ii^-1/2) (column equilibration: unit diagonal), is
+// factored in place by 256-column panels: each panel is widened to FP64 and
+// factored by the FP64 panel kernels (diagonal blocks, pivot rule, 64x64
+// inverses), written back as FP32, and applied to the trailing matrix by an
+// FP32-MFMA update.  The 256x256 diagonal blocks of the FP64 panel factors
+// are kept (D64) for the block inverses of the solves.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// s_i = 1/sqrt(B_ii); a diagonal that is not positive sets *bad.  *bmax:
+// max B_ii (bits of a non-negative double, which order like the integers),
+// which bounds every |B_ij| of a positive definite B.
+__global__ __launch_bounds__(256) void k_eq_scale(const double* __restrict__ B, int ld, int m,
+                                                  double* __restrict__ scale, int* bad,
+                                                  unsigned long long* bmax) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double d = 0.0;
+  if (i < m) {
+    d = B[i + (size_t)ld * i];
+    if (d > 0.0) {
+      scale[i] = 1.0 / sqrt(d);
+    } else {
+      scale[i] = 1.0;
+      atomicOr(bad, 1);
+    }
+  }
+  double mx = d > 0.0 ? d : 0.0;
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0 && mx > 0.0)
+    atomicMax(bmax, (unsigned long long)__double_as_longlong(mx));
+}
+
+// A32 = float(S B S) on the lower triangle by 64x64 tiles (grid (nt, nt); the
+// tiles above the diagonal return at once), zeros above the diagonal inside
+// diagonal tiles
+__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ B, int ld, int m,
+                                                const double* __restrict__ scale,
+                                                float* __restrict__ A, int lda) {
+  const int I = blockIdx.x, J = blockIdx.y;
+  if (J > I) return;
+  const int t = threadIdx.x, r = I * 64 + (t & 63);
+  const int rr = min(r, m - 1);
+  const double sr = scale[rr];
+  double v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = B[rr + (size_t)ld * min(J * 64 + (t >> 6) + 4 * k, m - 1)];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = J * 64 + (t >> 6) + 4 * k;
+    if (r < m && c < m) A[r + (size_t)lda * c] = r >= c ? (float)(v[k] * sr * scale[c]) : 0.0f;
+  }
+}
+
+// P[r + ldp c] = A32[r, K0 + c] for rows K0 <= r < m, c < pw (zero above the
+// diagonal); grid (ceil((m - K0) / 64), ceil(pw / 64))
+__global__ __launch_bounds__(256) void k_panel_to64(const float* __restrict__ A, int lda, int m,
+                                                    int K0, int pw, double* __restrict__ P,
+                                                    int ldp) {
+  const int t = threadIdx.x, r = K0 + blockIdx.x * 64 + (t & 63);
+  if (r >= m) return;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = blockIdx.y * 64 + (t >> 6) + 4 * k;
+    if (c < pw) P[r + (size_t)ldp * c] = (r >= K0 + c) ? (double)A[r + (size_t)lda * (K0 + c)] : 0.0;
+  }
+}
+
+// the factored panel back into A32 (lower part), and its 256x256 diagonal
+// block (zero above the diagonal) into D (ld 256).  Rows [K0 + 256, K0 + 512)
+// of P are then zeroed, element by element after their read: the next
+// panel's diagonal block, whose lower part the next-panel update writes
+// there (widened) and whose upper part must read as zero.
+__global__ __launch_bounds__(256) void k_panel_to32(double* __restrict__ P, int ldp, int m, int K0,
+                                                    int pw, float* __restrict__ A, int lda,
+                                                    double* __restrict__ D) {
+  const int t = threadIdx.x, r = K0 + blockIdx.x * 64 + (t & 63);
+  if (r >= m) return;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = blockIdx.y * 64 + (t >> 6) + 4 * k;
+    if (c < pw) {
+      const bool low = r >= K0 + c;
+      const double v = low ? P[r + (size_t)ldp * c] : 0.0;
+      if (low) A[r + (size_t)lda * (K0 + c)] = (float)v;
+      if (r < K0 + 256) D[(r - K0) + 256 * (size_t)c] = v;
+      if (r >= K0 + 256 && r < K0 + 512) P[r + (size_t)ldp * c] = 0.0;
+    }
+  }
+}
+
+// FP32 counterpart of syrk_tile: C[cb + r][cb + c] -= sum_{k < kw}
+// A[cb + r][k0 + k] A[cb + c][k0 + k] for one (16 TM) x (16 TN) tile, one wave,
+// operands straight from the matrix, D k-steps of loads in flight.  The f32
+// MFMA leaves D[i][j] in lane j + 16 (i / 4), register i % 4 (the FP64 MFMA:
+// lane j + 16 (i % 4), register i / 4).
+template <int TM, int TN, int D>
+__device__ __forceinline__ void syrk32_tile(float* __restrict__ Ab, size_t ld, int k0, int kw,
+                                            int cb, int rows, int cend, int wm, int wn,
+                                            double* __restrict__ W64) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const float* Pp = Ab + cb + ld * k0;
+  const float* pr[TM];
+  const float* pc[TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a) pr[a] = Pp + min(wm + 16 * a + li, rows - 1) + ld * lk;
+#pragma unroll
+  for (int a = 0; a < TN; ++a) pc[a] = Pp + min(wn + 16 * a + li, cend - 1) + ld * lk;
+  v4f acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int c = 0; c < TN; ++c) acc[a][c] = v4f{0.f, 0.f, 0.f, 0.f};
+  float ra[D][TM], ca[D][TN];
+  const int ns = kw >> 2;
+  const size_t st = ld * 4;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const size_t off = st * (d < ns ? d : 0);
+#pragma unroll
+    for (int a = 0; a < TM; ++a) ra[d][a] = pr[a][off];
+#pragma unroll
+    for (int a = 0; a < TN; ++a) ca[d][a] = pc[a][off];
+  }
+  for (int s0 = 0; s0 < ns; s0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int s = s0 + d;
+      if (s < ns) {
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int c = 0; c < TN; ++c)
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[d][c], ra[d][a], acc[a][c], 0, 0, 0);
+        if (s + D < ns) {
+          const size_t off = st * (s + D);
+#pragma unroll
+          for (int a = 0; a < TM; ++a) ra[d][a] = pr[a][off];
+#pragma unroll
+          for (int a = 0; a < TN; ++a) ca[d][a] = pc[a][off];
+        }
+      }
+    }
+  }
+  float* C = Ab + cb + ld * cb;
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int c = 0; c < TN; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gr = wm + 16 * a + li, gc = wn + 16 * c + 4 * lk + v;
+        if (gr < rows && gc < cend && gr >= gc) {
+          const float nv = C[gr + ld * gc] - acc[a][c][v];
+          C[gr + ld * gc] = nv;
+          // the next panel, widened for its FP64 factorization (rows absolute,
+          // columns from the panel's first, ld = the matrix's)
+          if (W64) W64[(cb + gr) + ld * gc] = (double)nv;
+        }
+      }
+}
+
+template <int TM, int TN, int D>
+__global__ __launch_bounds__(64) void k_syrk32_update(float* __restrict__ A, int ld, int n, int k0,
+                                                      int kw, int cb, int ce,
+                                                      const int* __restrict__ info,
+                                                      double* __restrict__ W64) {
+  constexpr int BMt = 16 * TM, BNt = 16 * TN;
+  if (info[0] != 0) return;
+  const int rows = n - cb, cend = min(ce, n) - cb;
+  if (rows <= 0 || cend <= 0) return;
+  const int R = (rows + BMt - 1) / BMt, Cn = (cend + BNt - 1) / BNt;
+  for (int tile = blockIdx.x;; tile += gridDim.x) {
+    int g = tile, i = 0, j = 0;
+    for (; j < Cn; ++j) {
+      const int cnt = R - (j * BNt) / BMt;
+      if (g < cnt) { i = (j * BNt) / BMt + g; break; }
+      g -= cnt;
+    }
+    if (j >= Cn) return;
+    syrk32_tile<TM, TN, D>(A, ld, k0, kw, cb, rows, cend, i * BMt, j * BNt, W64);
+  }
+}
+
+template <int TM, int TN, int D>
+static void launch_syrk32_t(hipStream_t s, float* A, int ld, int n, int k0, int kw, int cb, int ce,
+                            const int* d_info, int maxwaves, double* W64) {
+  constexpr int BMt = 16 * TM, BNt = 16 * TN;
+  const int rows = n - cb, cols = std::min(ce, n) - cb;
+  if (rows <= 0 || cols <= 0) return;
+  const int R = (rows + BMt - 1) / BMt, Cn = (cols + BNt - 1) / BNt;
+  int ntiles = 0;
+  for (int j = 0; j < Cn; ++j) ntiles += R - (j * BNt) / BMt;
+  if (maxwaves > 0) ntiles = std::min(ntiles, maxwaves);
+  hipLaunchKernelGGL((k_syrk32_update<TM, TN, D>), dim3(ntiles), dim3(64), 0, s, A, ld, n, k0, kw,
+                     cb, ce, d_info, W64);
+}
+
+// W64 (optional): also write the updated columns, widened, to the FP64
+// panel buffer (the next panel: ce - cb <= 256)
+static void launch_syrk32(hipStream_t s, float* A, int ld, int n, int k0, int kw, int cb, int ce,
+                          const int* d_info, int maxwaves = 0, double* W64 = nullptr) {
+  if (std::min(ce, n) - cb > 256)
+    launch_syrk32_t<4, 4, 4>(s, A, ld, n, k0, kw, cb, ce, d_info, maxwaves, nullptr);
+  else
+    launch_syrk32_t<2, 2, 8>(s, A, ld, n, k0, kw, cb, ce, d_info, maxwaves, W64);
+}
+
+int potrf_mixed(hipStream_t s, DenseWork* w, const double* B, int ld, int m, float* A32,
+                double* scale, double* P64, double* D64, double* dinv, int* d_info, int* d_bad,
+                unsigned long long* d_bmax) {
+  constexpr int PW = 256;
+  const int nt = (m + 63) / 64, np = (m + PW - 1) / PW;
+  SDPA_HIP_CHECK(hipMemsetAsync(d_info, 0, sizeof(int), s));
+  SDPA_HIP_CHECK(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+  SDPA_HIP_CHECK(hipMemsetAsync(d_bmax, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_eq_scale, dim3((m + 255) / 256), dim3(256), 0, s, B, ld, m, scale, d_bad,
+                     d_bmax);
+  hipLaunchKernelGGL(k_to_f32, dim3(nt, nt), dim3(256), 0, s, B, ld, m, scale, A32, m);
+  // one descriptor per panel: the FP64 panel buffer P64 (m x PW, ld m) seen
+  // as columns [K0, K0 + PW) of an m x m matrix, so the panel kernels index it
+  // with the global row and column numbers
+  if (int e = w->ensure_mats(np)) return e;
+  std::vector<MatDesc> pd(np);
+  for (int p = 0; p < np; ++p)
+    pd[p] = MatDesc{P64 - (ptrdiff_t)m * (p * PW), m, m};
+  SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats, pd.data(), sizeof(MatDesc) * np, hipMemcpyHostToDevice, s));
+  const int nslot = (m + TB - 1) / TB;
+  // Look-ahead as in potrf_batched: the panel chain (widen, factor, narrow)
+  // on `s`, the FP32 trailing updates on the CU-masked side stream, panel
+  // p + 1's columns first.  Panel p + 1 is widened once that update (and,
+  // by the side stream's order, every earlier one) is done; the narrowed
+  // panel p is read by the side stream only after it is written (evp).
+  if (int e = w->ensure_side(np)) return e;
+  SDPA_HIP_CHECK(hipEventRecord(w->evjoin, s));
+  SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evjoin, 0));
+  for (int p = 0; p < np; ++p) {
+    const int K0 = p * PW, pw = std::min(PW, m - K0);
+    const dim3 grid((m - K0 + 63) / 64, (pw + 63) / 64);
+    if (p > 0)
+      SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));  // widened by that update
+    else
+      hipLaunchKernelGGL(k_panel_to64, grid, dim3(256), 0, s, A32, m, m, K0, pw, P64, m);
+    factor_panel(s, w->d_mats + p, 1, m, K0, PW, 1, d_info, dinv, nslot);
+    hipLaunchKernelGGL(k_panel_to32, grid, dim3(256), 0, s, P64, m, m, K0, pw, A32, m,
+                       D64 + (size_t)p * PW * PW);
+    if (K0 + PW >= m) break;
+    SDPA_HIP_CHECK(hipEventRecord(w->evp[p], s));
+    SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
+    // the next panel's columns, also written widened into P64 (free once the
+    // narrowing above has read it: evp)
+    launch_syrk32(w->side, A32, m, m, K0, PW, K0 + PW, K0 + 2 * PW, d_info, w->side_waves, P64);
+    SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
+    if (K0 + 2 * PW < m)
+      launch_syrk32(w->side, A32, m, m, K0, PW, K0 + 2 * PW, m, d_info, w->side_waves);
+  }
+  SDPA_HIP_CHECK(hipEventRecord(w->evjoin, w->side));
+  SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evjoin, 0));
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -1389,16 +1652,30 @@ __device__ __forceinline__ void lds_barrier() {
 // scalar register, so 32 loads in flight cost 32 data registers and one
 // offset register rather than a 64-bit address each.  (Buffers of < 2 GiB;
 // reads past num_bytes return 0.)
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double* p, size_t num_bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0, (int)num_bytes,
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, size_t num_bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)num_bytes,
                                            0x00020000);
 }
 __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+// an element of a factor stored as T (double, or float for the mixed-precision
+// factor), widened to double
+template <typename T>
+__device__ __forceinline__ double buf_ldT(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ double buf_ldT<double>(__amdgpu_buffer_rsrc_t r, unsigned voff,
+                                                  unsigned soff) {
+  return buf_ld(r, voff, soff);
+}
+template <>
+__device__ __forceinline__ double buf_ldT<float>(__amdgpu_buffer_rsrc_t r, unsigned voff,
+                                                 unsigned soff) {
+  return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
 
 struct LLParams {
-  const double* L;
+  const void* L;              // the factor, double or float (kernel template)
   int ld, m;
   const double* W;            // 256 x 256 diagonal-block inverses, ld 256
   const double* rhs;          // forward: x; backward: t (plain vectors)
@@ -1469,6 +1746,7 @@ __device__ bool ll_get_block(const unsigned long long* buf, int i0, int m, unsig
 // forward, L t = x.  Strip s (rows 64 s + lane) of block b: y = x - sum over
 // earlier blocks of L[rows, block] t_block (compute wave w takes columns
 // 32 w .. 32 w + 31 of each block), then t_b = W_b y_b for its rows.
+template <typename T>
 __global__ __launch_bounds__(kLLThreads) void k_trsv_fwd_ll(LLParams P) {
   extern __shared__ double sol[];  // finished blocks of t
   __shared__ double ysh[SB];
@@ -1484,9 +1762,9 @@ __global__ __launch_bounds__(kLLThreads) void k_trsv_fwd_ll(LLParams P) {
   // wave-uniform column bases (scalar registers) plus a 32-bit per-lane row
   // offset: rows past m read a valid row and are never stored
   const int wc = min(w, 7);
-  const unsigned rl = 8u * (unsigned)min(r, m - 1);
-  const unsigned ldb = 8u * (unsigned)P.ld;
-  const __amdgpu_buffer_rsrc_t Lrs = buf_rsrc(P.L, (size_t)P.ld * m * sizeof(double));
+  const unsigned rl = (unsigned)sizeof(T) * (unsigned)min(r, m - 1);
+  const unsigned ldb = (unsigned)sizeof(T) * (unsigned)P.ld;
+  const __amdgpu_buffer_rsrc_t Lrs = buf_rsrc(P.L, (size_t)P.ld * m * sizeof(T));
   // this thread's W_b entries: row 64 q + lane, columns 32 w + c, clamped into
   // the kb x kb block (clamped columns meet ysh = 0, clamped rows are not stored)
   const __amdgpu_buffer_rsrc_t Wrs = buf_rsrc(P.W + (size_t)b * SB * SB, SB * SB * sizeof(double));
@@ -1495,7 +1773,7 @@ __global__ __launch_bounds__(kLLThreads) void k_trsv_fwd_ll(LLParams P) {
   double acc = 0.0;
 #define LL_LOAD_L(dst, bp)                                                   \
   _Pragma("unroll") for (int c = 0; c < 32; ++c)                             \
-    dst[c] = buf_ld(Lrs, rl, ldb * (unsigned)((bp) * SB + 32 * wc + c));
+    dst[c] = buf_ldT<T>(Lrs, rl, ldb * (unsigned)((bp) * SB + 32 * wc + c));
 #define LL_LOAD_W(dst)                                                       \
   _Pragma("unroll") for (int c = 0; c < 32; ++c)                             \
     dst[c] = buf_ld(Wrs, wl, 8u * SB * (unsigned)min(32 * wc + c, kb - 1));
@@ -1602,6 +1880,7 @@ __device__ __forceinline__ double ll_colsum(double (*red)[8][65], int tid) {
   return v;
 }
 
+template <typename T>
 __global__ __launch_bounds__(kLLThreads) void k_trsv_bwd_ll(LLParams P) {
   extern __shared__ double sol[];  // finished blocks of x
   __shared__ double ysh[SB];
@@ -1618,9 +1897,9 @@ __global__ __launch_bounds__(kLLThreads) void k_trsv_bwd_ll(LLParams P) {
   const int nbg = nb - 1 - b;      // background tiles: block nb - 1 - i, i < nbg
   // columns c0 + j are < m whenever there is background (b < nb - 1); the
   // column bases are wave-uniform, the rows a 32-bit per-lane offset
-  const unsigned ldb = 8u * (unsigned)P.ld;
+  const unsigned ldb = (unsigned)sizeof(T) * (unsigned)P.ld;
   const unsigned cb = ldb * (unsigned)min(c0, m - 8);
-  const __amdgpu_buffer_rsrc_t Lrs = buf_rsrc(P.L, (size_t)P.ld * m * sizeof(double));
+  const __amdgpu_buffer_rsrc_t Lrs = buf_rsrc(P.L, (size_t)P.ld * m * sizeof(T));
   const __amdgpu_buffer_rsrc_t Wrs = buf_rsrc(P.W + (size_t)b * SB * SB, SB * SB * sizeof(double));
   double lv[32];
   double acc[8];
@@ -1638,7 +1917,8 @@ __global__ __launch_bounds__(kLLThreads) void k_trsv_bwd_ll(LLParams P) {
 #define LL_LOAD_L(dst, bp)                                                               \
   _Pragma("unroll") for (int k = 0; k < 4; ++k)                                          \
     _Pragma("unroll") for (int j = 0; j < 8; ++j)                                        \
-      dst[8 * k + j] = buf_ld(Lrs, 8u * (unsigned)min((bp) * SB + 64 * k + lane, m - 1), cb + ldb * j);
+      dst[8 * k + j] = buf_ldT<T>(Lrs, (unsigned)sizeof(T) * (unsigned)min((bp) * SB + 64 * k + lane, m - 1), \
+                                  cb + ldb * j);
   // lv: the next background tile, reloaded right after each use (as in the
   // forward kernel).  The W_b^T tile (lw) is loaded once y_b is published,
   // so the loads overlap its exchange: held through the loop it would not
@@ -1733,26 +2013,30 @@ __global__ __launch_bounds__(kLLThreads) void k_trsv_bwd_ll(LLParams P) {
 // x := L^-T L^-1 x by the two pipelined cooperative launches; returns 1 when
 // they do not apply (no cooperative launch, or the grid cannot be co-resident
 // with its LDS) and the caller falls back
-static int chol_solve_ll(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
+template <typename T>
+static int chol_solve_ll(hipStream_t s, DenseWork* w, const T* L, int ld, int m,
                          const double* Winv, double* x) {
   const int ns = (m + 63) / 64, nb = (m + SB - 1) / SB;
   const size_t lds = (size_t)nb * SB * sizeof(double);
-  if (nb > 48 || (size_t)ld * m * sizeof(double) >= (size_t)1 << 31) return 1;
-  if (w->ll_max_nb < nb) {
+  if (nb > 48 || (size_t)ld * m * sizeof(T) >= (size_t)1 << 31) return 1;
+  int& max_nb = sizeof(T) == 8 ? w->ll_max_nb : w->ll_max_nb32;
+  if (max_nb < nb) {
     int dev = 0, coop = 0, ncu = 0;
     SDPA_HIP_CHECK(hipGetDevice(&dev));
     SDPA_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
     SDPA_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (!coop) return 1;
-    SDPA_HIP_CHECK(hipFuncSetAttribute((const void*)k_trsv_fwd_ll,
+    SDPA_HIP_CHECK(hipFuncSetAttribute((const void*)k_trsv_fwd_ll<T>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    SDPA_HIP_CHECK(hipFuncSetAttribute((const void*)k_trsv_bwd_ll,
+    SDPA_HIP_CHECK(hipFuncSetAttribute((const void*)k_trsv_bwd_ll<T>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int pf = 0, pb = 0;
-    SDPA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pf, k_trsv_fwd_ll, kLLThreads, lds));
-    SDPA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pb, k_trsv_bwd_ll, kLLThreads, lds));
+    SDPA_HIP_CHECK(
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&pf, k_trsv_fwd_ll<T>, kLLThreads, lds));
+    SDPA_HIP_CHECK(
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&pb, k_trsv_bwd_ll<T>, kLLThreads, lds));
     if ((long)std::min(pf, pb) * ncu < ns) return 1;
-    w->ll_max_nb = nb;
+    max_nb = nb;
   }
   if (int e = w->ensure_vec((size_t)m)) return e;
   if (int e = w->ensure_flags(0)) return e;  // the failure flag
@@ -1766,14 +2050,14 @@ static int chol_solve_ll(hipStream_t s, DenseWork* w, const double* L, int ld, i
   if (++w->ll_epoch == 0) ++w->ll_epoch;
   P.rhs = x; P.out = t; P.epoch = w->ll_epoch;
   void* args[] = {&P};
-  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_fwd_ll, dim3(ns), dim3(kLLThreads),
+  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_fwd_ll<T>, dim3(ns), dim3(kLLThreads),
                                             args, lds, s));
   // backward: x = L^-T t
   LLParams Q = P;
   if (++w->ll_epoch == 0) ++w->ll_epoch;
   Q.rhs = t; Q.out = x; Q.epoch = w->ll_epoch;
   void* args2[] = {&Q};
-  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_bwd_ll, dim3(ns), dim3(kLLThreads),
+  SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_bwd_ll<T>, dim3(ns), dim3(kLLThreads),
                                             args2, lds, s));
   SDPA_HIP_CHECK(hipMemcpyAsync(w->h_err, w->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
   return 0;
@@ -1814,7 +2098,7 @@ int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, cons
                double* x) {
   if (m <= 0) return 0;
   if (m > 2 * SB) {
-    int r = chol_solve_ll(s, w, L, ld, m, Winv, x);
+    int r = chol_solve_ll<double>(s, w, L, ld, m, Winv, x);
     if (r <= 0) return r;
     r = chol_solve_coop(s, w, L, ld, m, Winv, x);
     if (r <= 0) return r;
@@ -1837,6 +2121,15 @@ int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, cons
   }
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+// x := L^-T L^-1 x with a float factor (the mixed-precision path); the
+// pipelined cooperative kernels only.  Returns 1 when they do not apply.
+int chol_solve_f32(hipStream_t s, DenseWork* w, const float* L, int ld, int m, const double* Winv,
+                   double* x) {
+  if (m <= 0) return 0;
+  if (m <= 2 * SB) return 1;
+  return chol_solve_ll<float>(s, w, L, ld, m, Winv, x);
 }
 
 }  // namespace sdpa_amd

@@ -68,6 +68,7 @@ void free_problem(Ctx* c) {
     *p = nullptr;
   }
   c->capG = c->capF1C = 0;
+  free_mixed(c);
   c->loaded = false;
 }
 
@@ -591,8 +592,19 @@ int sdpa_gpu_make_bmat(sdpa_gpu_ctx* h, int precision) {
 int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* h, int precision) {
   if (!h || !h->c.loaded) return -1;
   Ctx* c = &h->c;
-  if (precision != 0) return fail(c, -30, "only FP64 (precision 0) is implemented");
+  if (precision != 0 && precision != 1) return fail(c, -30, "precision must be 0 (FP64) or 1 (mixed)");
   tic(c);
+  c->factor_mixed = 0;
+  if (precision == 1) {
+    const int r = factor_mixed(c);
+    if (r < 0) return r;
+    if (r == 0) {
+      c->t_chol += toc(c);
+      c->n_chol++;
+      return 0;
+    }
+    ++c->mx_fallbacks;  // FP32 pivot failure or size out of range: FP64 below (B intact)
+  }
   MatDesc d{c->d_B, c->m, c->m};
   int r = potrf_batched(c->stream, &c->dw, &d, 1, 1, c->d_info, c->d_Binv);
   if (r) return r;
@@ -613,6 +625,12 @@ int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* h, int refine) {
   Ctx* c = &h->c;
   (void)refine;
   tic(c);
+  if (c->factor_mixed) {
+    const int r = solve_mixed(c);
+    c->t_solve += toc(c);
+    c->n_solve++;
+    return r;
+  }
   SDPA_HIP_CHECK(hipMemcpyAsync(c->d_dy, c->d_g, sizeof(double) * c->m, hipMemcpyDeviceToDevice,
                                 c->stream));
   int r = chol_solve(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv, c->d_dy);
@@ -654,6 +672,18 @@ int sdpa_gpu_dots(sdpa_gpu_ctx* h, int nq, const int* codes, double* out) {
   if (r == 0 && h->c.dw.h_err && *h->c.dw.h_err)
     return fail(&h->c, -60, "a cooperative Schur solve gave up waiting (work-groups not co-resident)");
   return r;
+}
+
+int sdpa_gpu_mixed_stats(sdpa_gpu_ctx* h, long* out6) {
+  if (!h || !out6) return -1;
+  const Ctx* c = &h->c;
+  out6[0] = c->mx_factors;
+  out6[1] = c->mx_fallbacks;
+  out6[2] = c->mx_solves;
+  out6[3] = c->mx_refine;
+  out6[4] = c->mx_stalls;
+  out6[5] = c->factor_mixed;
+  return 0;
 }
 
 int sdpa_gpu_update(sdpa_gpu_ctx* h, double aP, double aD) {

@@ -820,6 +820,22 @@ void SDPA::solve() {
 
   print_header(fpout_);
   print_header(display_);
+  // mixed precision: on until the first stall; a stalled refinement is
+  // replaced by an FP64 factorization of the same (intact) Schur matrix
+  bool mixedOn = precision_ == 1;
+  auto solve_schur = [&]() -> bool {
+    int sr = sdpa_gpu_solve_bmat(g, 0);
+    check(sr, g, "solve_bmat");
+    if (sr == 2) {
+      mixedOn = false;
+      const int inf2 = sdpa_gpu_factor_bmat(g, 0);
+      check(inf2, g, "factor_bmat");
+      if (inf2 > 0) { failure_ = 1; return false; }
+      sr = sdpa_gpu_solve_bmat(g, 0);
+      check(sr, g, "solve_bmat");
+    }
+    return true;
+  };
   int it = 0;
   while (update_phase() && it < P.maxIteration) {
     const bool affine = !(phase == noINFO || phase == pFEAS || phase == dFEAS);
@@ -830,10 +846,16 @@ void SDPA::solve() {
     // Newton::Mehrotra(PREDICTOR)
     check(sdpa_gpu_make_gvec(g, 0, betaVal * muCur, infeasible), g, "make_gvec");
     check(sdpa_gpu_make_bmat(g, 0), g, "make_bmat");
-    int info = sdpa_gpu_factor_bmat(g, 0);
+    const bool mixedNow = mixedOn && muCur > muSwitch_;
+    int info = sdpa_gpu_factor_bmat(g, mixedNow ? 1 : 0);
     check(info, g, "factor_bmat");
     if (info > 0) { failure_ = 1; break; }
-    check(sdpa_gpu_solve_bmat(g, 0), g, "solve_bmat");
+    if (mixedNow) {
+      long st[6];
+      check(sdpa_gpu_mixed_stats(g, st), g, "mixed_stats");
+      if (!st[5]) mixedOn = false;  // the mixed factorization fell back to FP64
+    }
+    if (!solve_schur()) break;
     check(sdpa_gpu_make_direction(g, infeasible), g, "make_direction");
 
     // StepLength::MehrotraPredictor + DirectionParameter::MehrotraCorrector
@@ -870,7 +892,7 @@ void SDPA::solve() {
 
     // Newton::Mehrotra(CORRECTOR)
     check(sdpa_gpu_make_gvec(g, 1, betaVal * muCur, infeasible), g, "make_gvec");
-    check(sdpa_gpu_solve_bmat(g, 0), g, "solve_bmat");
+    if (!solve_schur()) break;
     check(sdpa_gpu_make_direction(g, infeasible), g, "make_direction");
 
     // StepLength::MehrotraCorrector
@@ -1058,6 +1080,8 @@ int sdpa_solver_set_param(sdpa_solver* h, const char* name, double v) {
     else if (n == "gammaStar") s->setParameterGammaStar(v);
     else if (n == "epsilonDash") s->setParameterEpsilonDash(v);
     else if (n == "kappa") s->setKappa(v);
+    else if (n == "precision") s->setPrecision((int)v, s->muSwitch());
+    else if (n == "muSwitch") s->setPrecision(s->precision(), v);
     else throw std::runtime_error("unknown parameter " + n);
   })
 }

@@ -1,0 +1,114 @@
+"""Mixed-precision Schur factorization (precision 1 of sdpa_gpu_factor_bmat:
+FP32 factor of the column-equilibrated Schur matrix on FP32 MFMA, FP64
+iterative refinement of every solve) through the C ABI.
+
+Tolerances: a refined solution must be as accurate as the FP64 path's
+(1e-9 relative to the solution norm, against numpy's FP64 solve, on
+well-conditioned matrices; on an ill-conditioned one the protocol -- stall,
+FP64 refactorization, solve -- must end in a backward-stable solution,
+||A x - b|| <= 1e-12 ||A|| ||x||).  Whole solves: the oracle's phase,
+iteration count within one, objectives within 1e-6 relative, gap and
+feasibility at the 1e-7 stopping level.  The FP64 Schur matrix must be left
+untouched by the mixed factorization (the refinement residuals use it).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sdpa_amd import problems as PR
+
+pytestmark = pytest.mark.gpu
+
+sa = pytest.importorskip("sdpa_amd")
+
+
+def spd(n, rng, cond):
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (q * np.geomspace(1, cond, n)) @ q.T
+
+
+def stats(g):
+    return dict(zip(("factors", "fallbacks", "solves", "steps", "stalls", "current"),
+                    (int(v) for v in g.mixed_stats())))
+
+
+@pytest.mark.parametrize("m,cond", [(700, 1e3), (1100, 1e5)])
+def test_mixed_factor_and_refined_solve(m, cond):
+    rng = np.random.default_rng(m)
+    d = PR.maxcut(m, 0.5, seed=3)  # any problem with this m; only bMat is used
+    g = sa.GpuContext(0).load(d)
+    # a badly scaled SPD matrix: the equilibration must undo the scaling
+    s = np.geomspace(1e-3, 1e3, m)
+    A = spd(m, rng, cond) * np.outer(s, s)
+    g.set("B", A.reshape(-1, order="F"))
+    assert g.factor_bmat(1) == 0
+    st = stats(g)
+    assert st["current"] == 1 and st["factors"] == 1 and st["fallbacks"] == 0
+    np.testing.assert_array_equal(g.bmat(), A)  # the FP64 matrix is intact
+    # reference: numpy on the equilibrated matrix (well conditioned), scaled back;
+    # errors are measured in the equilibrated variables S^-1 x
+    d_ = 1.0 / np.sqrt(np.diag(A))
+    Ae = A * np.outer(d_, d_)
+    for rep in range(2):  # two right-hand sides with one factorization
+        rhs = rng.standard_normal(m) / s
+        g.set("g", rhs)
+        assert g.solve_bmat() == 0
+        x = g.get("dy")
+        want = d_ * np.linalg.solve(Ae, d_ * rhs)
+        assert np.linalg.norm((x - want) / d_) <= 1e-9 * np.linalg.norm(want / d_)
+    st2 = stats(g)
+    assert st2["solves"] == 2 and st2["stalls"] == 0
+    # an FP32 factor alone is not FP64-accurate: every solve refines at least once
+    assert st2["steps"] >= 4
+
+
+def test_mixed_ill_conditioned_falls_back_to_fp64():
+    rng = np.random.default_rng(11)
+    m = 800
+    d = PR.maxcut(m, 0.5, seed=3)
+    g = sa.GpuContext(0).load(d)
+    A = spd(m, rng, 1e12)
+    g.set("B", A.reshape(-1, order="F"))
+    rhs = rng.standard_normal(m)
+    assert g.factor_bmat(1) == 0
+    g.set("g", rhs)
+    rc = g.solve_bmat()
+    st = stats(g)
+    if st["current"] == 1:
+        # FP32 cannot resolve cond 1e12: the refinement must report a stall
+        assert rc == 2 and st["stalls"] == 1
+        assert g.factor_bmat(0) == 0
+        rc = g.solve_bmat()
+    else:
+        assert st["fallbacks"] == 1  # the FP32 factorization itself failed
+    assert rc == 0
+    x = g.get("dy")
+    res = np.linalg.norm(A @ x - rhs)
+    assert res <= 1e-12 * np.linalg.norm(A, 2) * np.linalg.norm(x)
+
+
+def test_mixed_small_m_uses_fp64():
+    d = PR.maxcut(100, 0.1, 1)
+    g = sa.GpuContext(0).load(d)
+    A = spd(100, np.random.default_rng(1), 10.0)
+    g.set("B", A.reshape(-1, order="F"))
+    assert g.factor_bmat(1) == 0
+    st = stats(g)
+    assert st["current"] == 0 and st["fallbacks"] == 1
+
+
+@pytest.mark.parametrize("n", [60, 70])
+def test_mixed_full_solve_matches_oracle(n):
+    d = PR.lovasz(n, 0.3, 2)  # m = 550, 753: inside the mixed path's range
+    s = sa.Solver(0).load(d).set_param(precision=1)
+    r = s.solve()
+    st = s.mixed_stats()
+    o = O.Problem(d).solve()
+    print(n, d.m, r.phase, r.iterations, o.iterations, r.primal_obj, o.primal_obj, st)
+    assert st["factors"] >= 1  # the mixed path was taken
+    assert r.phase == o.phase == "pdOPT"
+    assert abs(r.iterations - o.iterations) <= 1
+    assert r.primal_obj == pytest.approx(o.primal_obj, rel=1e-6, abs=1e-8)
+    assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)
+    assert r.relative_gap <= 1e-7
+    assert r.primal_error <= 1e-7 and r.dual_error <= 1e-7
