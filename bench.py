@@ -202,6 +202,38 @@ def main():
     tr_ms, tr_n, tr_flops = s.trailing_update_timing()
     s.enable_timing(False)
     log(f"bench: timed solve: phase={rt.phase} it={rt.iterations}; trailing update {tr_n} launches")
+    # the same workload with the mixed-precision Schur factorization (FP32
+    # factor + FP64 refinement, FP64 after the first stall), same step count
+    s.set_param(precision=1)
+    for w in range(args.warmup):
+        s.run()
+    st0 = s.mixed_stats()
+    mtimes = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        rm = s.run()
+        mtimes.append(time.perf_counter() - t0)
+        log(f"bench: mixed step {k}: phase={rm.phase} it={rm.iterations} t={mtimes[-1]:.4f}s "
+            f"gap={rm.relative_gap:.2e}")
+    st1 = s.mixed_stats()
+    s.enable_timing(True)
+    s.run()
+    mpt = s.phase_times()
+    s.enable_timing(False)
+    s.set_param(precision=0)
+    mval = float(np.mean(mtimes))
+    nrun = max(1, args.steps)
+    mixed = {
+        "value": mval, "unit": "s",
+        "fp64_over_mixed": value / mval,
+        "phase": rm.phase, "iterations": rm.iterations, "relative_gap": rm.relative_gap,
+        "primal_error": rm.primal_error, "dual_error": rm.dual_error,
+        "objective_rel_diff_vs_fp64": max(abs(rm.primal_obj - rt.primal_obj) / abs(rt.primal_obj),
+                                          abs(rm.dual_obj - rt.dual_obj) / abs(rt.dual_obj)),
+        "per_solve": {k: (st1[k] - st0[k]) / nrun for k in
+                      ("factors", "fallbacks", "solves", "refine_steps", "stalls")},
+        "phase_ms_per_iteration": {k: v / max(1, rm.iterations) for k, v in mpt.items()},
+    }
     its = rt.iterations
     m = d.m
     # F3 bytes per assembly (lower triangle written): 8*T + 16*sum(nnz) + 16*n^2
@@ -240,7 +272,7 @@ def main():
                     "frac_hbm": bmat_gbs(bmat_bytes, bmat_ms) / HBM_PEAK_GBS},
         "cholesky": {"flops": m ** 3 / 3.0, "ms": chol_ms,
                      "TFLOPs": m ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None},
-        "mixed_vs_fp64": None,
+        "mixed_vs_fp64": mixed,
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

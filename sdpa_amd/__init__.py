@@ -84,6 +84,17 @@ def lib():
         L.sdpa_gpu_kernel_timing.argtypes = [V, V]
         L.sdpa_gpu_synchronize.argtypes = [V]
         L.sdpa_gpu_mixed_stats.argtypes = [V, np.ctypeslib.ndpointer(np.int64)]
+        # lambda* model (include/sdpa_mlp.h)
+        L.sdpa_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(V)]
+        L.sdpa_mlp_destroy.argtypes = [V]
+        L.sdpa_mlp_nparams.restype = C.c_long
+        L.sdpa_mlp_nparams.argtypes = [V]
+        L.sdpa_mlp_set_params.argtypes = [V, _dp, C.c_long]
+        L.sdpa_mlp_get_params.argtypes = [V, _dp, C.c_long]
+        L.sdpa_mlp_set_data.argtypes = [V, _dp, _dp, C.c_int]
+        L.sdpa_mlp_loss_grad.argtypes = [V, C.POINTER(C.c_double), V]
+        L.sdpa_mlp_train.argtypes = [V, C.c_int, C.c_double, C.POINTER(C.c_double)]
+        L.sdpa_mlp_predict.argtypes = [V, _dp, C.c_int, _dp]
         L.sdpa_solver_new.restype = V
         L.sdpa_solver_new.argtypes = [C.c_int]
         L.sdpa_solver_free.argtypes = [V]
