@@ -26,7 +26,7 @@ struct BlockTable {
   int* d_f1own = nullptr;  // local indices of F1/F2 owners (nF1)
   int* d_dcon = nullptr;   // global constraint of each dense pool column (ndense)
   double* d_dense = nullptr;  // n*n x ndense dense pool (column per dense A_k)
-  std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart;
+  std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart, h_f1own;
 };
 
 struct Ctx {
@@ -86,6 +86,8 @@ struct Ctx {
   size_t capG = 0;
   double* d_F1C = nullptr;  // F1 dense-pair products
   size_t capF1C = 0;
+  double* d_F1P = nullptr;  // their partial sums over slices of the n^2 dimension
+  size_t capF1P = 0;
   int* d_info = nullptr;   // factorization info (one per batch item)
   double* d_red = nullptr;  // reduction partials
   double* h_red = nullptr;  // pinned host mirror of reduction results
@@ -106,6 +108,7 @@ struct Ctx {
   int factor_mixed = 0;       // the current factor is the mixed one (B intact)
   double mx_anorm = 0.0;      // ||S B S||_inf of that factorization
   long mx_factors = 0, mx_fallbacks = 0, mx_solves = 0, mx_refine = 0, mx_stalls = 0;
+  void* blas = nullptr;  // rocBLAS handle (made on first use)
   // timing (ms), accumulated since the last reset
   hipEvent_t ev[8];
   double t_bmat = 0, t_chol = 0, t_solve = 0, t_eig = 0, t_inv = 0;
@@ -133,6 +136,16 @@ int triple_product(Ctx* c, const double* A, const double* B, const double* C, do
 int lp_min_step(Ctx* c, const double* D, const double* L, double* out);
 // eig.hip
 int min_eig(Ctx* c, double* lamX, double* lamZ);
+// blas.cpp (rocBLAS, column-major; t = 1 transposes).  Single blocks of at
+// least kBlasMinN rows take the library GEMMs; many small blocks stay in one
+// batched launch of the own kernel.
+constexpr int kBlasMinN = 128;
+int blas_gemm_strided(Ctx* c, int ta, int tb, int M, int N, int K, double alpha, const double* A,
+                      int lda, long sA, const double* B, int ldb, long sB, double beta, double* C,
+                      int ldc, long sC, int batch);
+int blas_gemm(Ctx* c, int ta, int tb, int M, int N, int K, double alpha, const double* A, int lda,
+              const double* B, int ldb, double beta, double* C, int ldc);
+void blas_release(Ctx* c);
 // mixed.hip
 int factor_mixed(Ctx* c);  // 0 ok; otherwise the FP64 path must be used
 int solve_mixed(Ctx* c);   // 0 converged (dy set); 2 refinement stalled

@@ -328,13 +328,24 @@ int min_eig(Ctx* c, double* lamX, double* lamZ) {
                                 n, n, n, n, n, n};
       maxn = n > maxn ? n : maxn;
     }
-    if (int e = c->dw.ensure_items(4 * nb)) return e;
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * 4 * nb,
-                                  hipMemcpyHostToDevice, c->stream));
-    if (int e = gemm_batched(c->stream, c->dw.d_items, 2 * nb, maxn, maxn, 0, 1, 1.0, 0.0, 0))
-      return e;
-    if (int e = gemm_batched(c->stream, c->dw.d_items + 2 * nb, 2 * nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
-      return e;
+    if (nb == 1 && maxn >= kBlasMinN) {
+      // one large block: plain GEMMs through the library
+      for (int k = 0; k < 4; ++k) {
+        const GemmItem& g = it[k];
+        if (int e = blas_gemm(c, 0, k < 2 ? 1 : 0, g.M, g.N, g.K, 1.0, g.A, g.lda, g.B, g.ldb,
+                              0.0, g.C, g.ldc))
+          return e;
+      }
+    } else {
+      if (int e = c->dw.ensure_items(4 * nb)) return e;
+      SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * 4 * nb,
+                                    hipMemcpyHostToDevice, c->stream));
+      if (int e = gemm_batched(c->stream, c->dw.d_items, 2 * nb, maxn, maxn, 0, 1, 1.0, 0.0, 0))
+        return e;
+      if (int e = gemm_batched(c->stream, c->dw.d_items + 2 * nb, 2 * nb, maxn, maxn, 0, 0, 1.0,
+                               0.0, 0))
+        return e;
+    }
     hipLaunchKernelGGL(k_min_eig_small, dim3(nb, 2), dim3(64), 0, c->stream, c->d_w1, c->lpOff,
                        c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
     hipLaunchKernelGGL(k_lanczos, dim3(nb, 2), dim3(kLzThreads), 0, c->stream, c->d_w1, c->lpOff,

@@ -232,12 +232,21 @@ int triple_product(Ctx* c, const double* A, const double* B, const double* Cm, d
       it[nb + q] = GemmItem{c->d_w3 + o, Cm + o, R + o, n, n, n, n, n, n};
       maxn = n > maxn ? n : maxn;
     }
-    if (int e = c->dw.ensure_items(2 * nb)) return e;
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * 2 * nb,
-                                  hipMemcpyHostToDevice, c->stream));
-    if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0)) return e;
-    if (int e = gemm_batched(c->stream, c->dw.d_items + nb, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
-      return e;
+    if (nb == 1 && maxn >= kBlasMinN) {
+      for (int k = 0; k < 2; ++k) {  // one large block: plain GEMMs through the library
+        const GemmItem& g = it[k];
+        if (int e = blas_gemm(c, 0, 0, g.M, g.N, g.K, 1.0, g.A, g.lda, g.B, g.ldb, 0.0, g.C, g.ldc))
+          return e;
+      }
+    } else {
+      if (int e = c->dw.ensure_items(2 * nb)) return e;
+      SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * 2 * nb,
+                                    hipMemcpyHostToDevice, c->stream));
+      if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
+        return e;
+      if (int e = gemm_batched(c->stream, c->dw.d_items + nb, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
+        return e;
+    }
   }
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
@@ -415,10 +424,17 @@ int inverse_chol(Ctx* c) {
       it[q] = GemmItem{c->d_invCholZ + o, c->d_invCholZ + o, c->d_Zinv + o, n, n, n, n, n, n};
       maxn = n > maxn ? n : maxn;
     }
-    if (int e = c->dw.ensure_items(nb)) return e;
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * nb,
-                                  hipMemcpyHostToDevice, c->stream));
-    if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 1, 0, 1.0, 0.0, 0)) return e;
+    if (nb == 1 && maxn >= kBlasMinN) {
+      const GemmItem& g = it[0];  // one large block: a plain GEMM through the library
+      if (int e = blas_gemm(c, 1, 0, g.M, g.N, g.K, 1.0, g.A, g.lda, g.B, g.ldb, 0.0, g.C, g.ldc))
+        return e;
+    } else {
+      if (int e = c->dw.ensure_items(nb)) return e;
+      SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * nb,
+                                    hipMemcpyHostToDevice, c->stream));
+      if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 1, 0, 1.0, 0.0, 0))
+        return e;
+    }
   }
   if (c->nLP > 0) {
     int* bad = c->d_info + 2 * nb;

@@ -175,6 +175,16 @@ __global__ __launch_bounds__(256) void k_densify(const int* __restrict__ own, in
   }
 }
 
+// C[i] = sum over s of P[s * len + i], in order of s
+__global__ __launch_bounds__(256) void k_sum_slices(const double* __restrict__ P, int ns,
+                                                    size_t len, double* __restrict__ C) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= len) return;
+  double a = 0.0;
+  for (int s = 0; s < ns; ++s) a += P[(size_t)s * len + i];
+  C[i] = a;
+}
+
 __device__ __forceinline__ bool owns(int jloc, int iloc, const int* nzeff) {
   const int jnz = nzeff[jloc], inz = nzeff[iloc];
   return !((jnz < inz) || (inz == jnz && iloc < jloc));
@@ -285,26 +295,63 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
     SDPA_HIP_CHECK(hipMemcpyAsync(d_dlist, dl.data(), sizeof(int) * T.ndense,
                                   hipMemcpyHostToDevice, c->stream));
   }
-  std::vector<GemmItem> items(2 * chunk);
-  if (int e = c->dw.ensure_items(2 * chunk)) return e;
   for (int first = 0; first < T.nF1; first += chunk) {
     const int cnt = std::min(chunk, T.nF1 - first);
-    hipLaunchKernelGGL(k_densify, dim3(cnt), dim3(256), 0, c->stream, T.d_f1own, first,
-                       T.d_estart, T.d_ecnt, T.d_dpos, c->d_er, c->d_ec, c->d_ev, T.d_dense, n, Ad);
-    for (int t = 0; t < cnt; ++t) {
-      items[t] = GemmItem{X, Ad + t * nn, XA + t * nn, n, n, n, n, n, n};
-      items[cnt + t] = GemmItem{XA + t * nn, Zi, G + t * nn, n, n, n, n, n, n};
+    // the chunk's A_t: straight from the dense pool when its owners are
+    // consecutive pool columns, else densified into Ad
+    const double* At = Ad;
+    {
+      const int p0 = T.h_dpos[T.h_f1own[first]];
+      bool consecutive = p0 >= 0;
+      for (int t = 1; t < cnt && consecutive; ++t)
+        consecutive = T.h_dpos[T.h_f1own[first + t]] == p0 + t;
+      if (consecutive)
+        At = T.d_dense + (size_t)p0 * nn;
+      else
+        hipLaunchKernelGGL(k_densify, dim3(cnt), dim3(256), 0, c->stream, T.d_f1own, first,
+                           T.d_estart, T.d_ecnt, T.d_dpos, c->d_er, c->d_ec, c->d_ev, T.d_dense, n,
+                           Ad);
     }
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, items.data(), sizeof(GemmItem) * 2 * cnt,
-                                  hipMemcpyHostToDevice, c->stream));
-    if (int e = gemm_batched(c->stream, c->dw.d_items, cnt, n, n, 0, 0, 1.0, 0.0, 0)) return e;
-    if (int e = gemm_batched(c->stream, c->dw.d_items + cnt, cnt, n, n, 0, 0, 1.0, 0.0, 0))
+    // XA_t = X A_t and G_t = XA_t Z^-1 for the chunk: plain GEMMs (rocBLAS)
+    if (int e = blas_gemm_strided(c, 0, 0, n, n, n, 1.0, X, n, 0, At, n, (long)nn, 0.0, XA, n,
+                                  (long)nn, cnt))
+      return e;
+    if (int e = blas_gemm_strided(c, 0, 0, n, n, n, 1.0, XA, n, (long)nn, Zi, n, 0, 0.0, G, n,
+                                  (long)nn, cnt))
       return e;
     if (T.ndense > 0) {
-      // P = Abar^T * Gbar : (ndense x n^2) * (n^2 x cnt)
-      if (int e = gemm(c->stream, &c->dw, T.d_dense, (int)nn, G, (int)nn, c->d_F1C, T.ndense,
-                       T.ndense, cnt, (int)nn, 1, 0, 1.0, 0.0, 0))
-        return e;
+      // P = Abar^T * Gbar : (ndense x n^2) * (n^2 x cnt).  The output has few
+      // tiles for its long inner dimension: the n^2 rows are split into ks
+      // slices (one batched call, one partial product each) and the partials
+      // summed, so that the GEMM fills the chip.
+      const int M = T.ndense, N = cnt;
+      const long K = (long)nn;
+      const long tiles = (long)((M + 127) / 128) * ((N + 127) / 128);
+      int ks = (int)std::min<long>(32, std::max<long>(1, 1024 / tiles));
+      if (K / ks < 1024) ks = (int)std::max<long>(1, K / 1024);
+      if (ks <= 1) {
+        if (int e = blas_gemm(c, 1, 0, M, N, (int)K, 1.0, T.d_dense, (int)nn, G, (int)nn, 0.0,
+                              c->d_F1C, M))
+          return e;
+      } else {
+        const long Kc = K / ks, rem = K - Kc * ks;
+        const int nparts = ks + (rem > 0 ? 1 : 0);
+        const size_t len = (size_t)M * N;
+        if ((size_t)nparts * len > c->capF1P) {
+          if (c->d_F1P) SDPA_HIP_CHECK(hipFree(c->d_F1P));
+          c->capF1P = (size_t)nparts * len;
+          SDPA_HIP_CHECK(hipMalloc(&c->d_F1P, c->capF1P * sizeof(double)));
+        }
+        if (int e = blas_gemm_strided(c, 1, 0, M, N, (int)Kc, 1.0, T.d_dense, (int)nn, Kc, G,
+                                      (int)nn, Kc, 0.0, c->d_F1P, M, (long)len, ks))
+          return e;
+        if (rem > 0)
+          if (int e = blas_gemm(c, 1, 0, M, N, (int)rem, 1.0, T.d_dense + Kc * ks, (int)nn,
+                                G + Kc * ks, (int)nn, 0.0, c->d_F1P + (size_t)ks * len, M))
+            return e;
+        hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((len + 255) / 256)), dim3(256), 0,
+                           c->stream, c->d_F1P, nparts, len, c->d_F1C);
+      }
       hipLaunchKernelGGL(k_f1_scatter, dim3((T.ndense + 255) / 256, cnt), dim3(256), 0, c->stream,
                          T.d_f1own, first, d_dlist, T.d_con, T.d_nzeff, T.ndense, c->d_F1C,
                          c->d_B, c->m);

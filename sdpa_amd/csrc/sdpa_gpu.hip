@@ -60,14 +60,14 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_Dx, (void**)&c->d_Dz, (void**)&c->d_dy, (void**)&c->d_rzinv,
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
-                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C,
+                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
   for (void** p : ptrs) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
-  c->capG = c->capF1C = 0;
+  c->capG = c->capF1C = c->capF1P = 0;
   free_mixed(c);
   c->loaded = false;
 }
@@ -105,6 +105,7 @@ int sdpa_gpu_destroy(sdpa_gpu_ctx* h) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_problem(c);
+  blas_release(c);
   c->dw.release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   if (c->h_red) (void)hipHostFree(c->h_red);
@@ -282,6 +283,7 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     T.h_con = con;
     T.h_form = form;
     T.h_dpos = dpos;
+    T.h_f1own = f1own;
     T.h_ecnt = ecnt;
     T.h_estart = estart;
     if (T.ndense > 0) {

@@ -214,6 +214,10 @@ SOLVE_CASES = {
     "dense_n25_m40": lambda: PR.dense_random(n=25, m=40, seed=3),
     "multiblock_lp": lambda: PR.multi_block(n_sdp_blocks=8, nmin=3, nmax=25, lp=20, m=60,
                                             density=0.03, seed=5),
+    # one block of at least 128 rows: the library GEMM paths (F1 products,
+    # inverse and eigenvalue set-up)
+    "maxcut150": lambda: PR.maxcut(150, 0.1, 4),
+    "dense_n130_m60": lambda: PR.dense_random(n=130, m=60, seed=6),
 }
 
 
