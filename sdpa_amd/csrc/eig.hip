@@ -164,7 +164,9 @@ __device__ double jacobi_min_wave(int n, double* A) {
     off = wsum(off);
     double dv = lane < n ? A[lane + 65 * lane] : 0.0;
     dg = wsum(dv * dv);
-    if (off == 0.0 || off <= 1e-36 * dg) break;
+    // converged when the off-diagonal norm is below eps times the diagonal's
+    // (a tighter test is never met in FP64 and ran every call to the sweep cap)
+    if (off == 0.0 || off <= 1e-31 * dg) break;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
         __syncthreads();
