@@ -1,0 +1,42 @@
+/* sdpa_batch.h -- C ABI of the batched small-SDP solver: many independent
+ * single-block instances (n <= 64, m <= 72, sparse constraint matrices)
+ * solved by ONE persistent launch, one work-group per instance running
+ * SDPA's whole Mehrotra predictor-corrector loop -- Newton direction, step
+ * lengths and the control scalars -- on the device.
+ *
+ * Counterpart in the reference: SDPA::solve (sdpa_solve.cpp:59-233) with
+ * its phases (Newton::Mehrotra, StepLength::MehrotraCorrector,
+ * Solutions::update, Phase::updateCheck, SolveInfo::update) run once per
+ * instance; the formulas are those of the single-problem path (F3 bMat,
+ * rATL_dpotrfL pivot rule, Lal::getMinEigen).  Internal convention
+ * (reversed primal/dual, C = -F0), default initial point X0 = Z0 =
+ * lambda_k I, y0 = 0 with a per-instance lambda_k.
+ *
+ * Data of instance k (all arrays concatenated over instances):
+ *   n[k], m[k]; C: n[k]^2 doubles (column-major, the internal C) at coff[k];
+ *   b: m[k] doubles at boff[k]; constraints: entry pointers cptr at
+ *   pptr[k] .. pptr[k] + m[k] (m[k] + 1 values, global entry indices), and
+ *   entries er, ec (0-based, er <= ec), ev.
+ * param: maxIteration, epsilonStar, epsilonDash, betaStar, betaBar,
+ *   gammaStar, omegaStar, lowerBound, upperBound (9 doubles).
+ * out: 8 doubles per instance: phase (0 noINFO .. 9 dUNBD, internal
+ *   convention), iterations, objP (= C.X), objD (= b.y), normP (max |b -
+ *   A(X)|), normD (max |C - sum y A - Z|), mu, failure (0 none, 1 Schur
+ *   Cholesky, 2 step length, 3 inverse Cholesky).
+ * Returns 0, or < 0 on an argument or device error. */
+#ifndef SDPA_BATCH_H
+#define SDPA_BATCH_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int sdpa_batch_solve(int device, int count, const int* n, const int* m, const long* coff,
+                     const double* C, long nC, const long* boff, const double* b, long nb,
+                     const long* pptr, const int* cptr, long ncptr, const int* er, const int* ec,
+                     const double* ev, long nent, const double* lambda, const double* param,
+                     int threads, double* out, double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDPA_BATCH_H */

@@ -1,0 +1,187 @@
+// eig_dev.h -- device routines shared by the step-length kernels (eig.hip)
+// and the batched small-SDP solver (batch.hip): wave sums, an FP64
+// reciprocal, the smallest eigenvalue of a symmetric tridiagonal matrix with
+// the last component of its eigenvector (what the reference reads from
+// dsteqr, sdpa_linear.cpp:98-127), and cyclic Jacobi for small blocks (the
+// dsyev path, sdpa_linear.cpp:145-182).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sdpa_amd {
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// 1/x to full FP64 accuracy without the IEEE divide sequence: the hardware
+// reciprocal estimate and two Newton steps (each doubles the correct bits).
+// x is a nonzero normal number here (Sturm pivots are kept away from 0).
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Smallest eigenvalue theta of the symmetric tridiagonal T (diagonal d[0..N),
+// off-diagonal e[0..N-1), in LDS) and the last component of its unit
+// eigenvector -- what the reference reads from dsteqr's output (the first
+// eigenvalue in ascending order and Q[k][0], sdpa_linear.cpp:98-127).
+// All threads of the work-group (nthreads, a multiple of 64) take part:
+// multisection with Sturm counts, nthreads shifts per round (the interval of
+// the smallest eigenvalue shrinks nthreads+1-fold per round); then wave 0 runs
+// two steps of inverse iteration from a shift just below theta, where
+// T - sigma I is positive definite and its LDL^T needs no pivoting.
+// scratch: 2 * (nthreads / 64) doubles of LDS.  Call from all threads.
+static __device__ void tridiag_min_block(int N, const double* d, const double* e, double& theta,
+                                  double& zlast, int nthreads, double* scratch) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = nthreads >> 6;
+  const double eps = 2.220446049250313e-16;
+  double lo = 1.0e300, hi = -1.0e300;
+  for (int i = lane; i < N; i += 64) {
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < N - 1 ? fabs(e[i]) : 0.0);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  const double pivmin = 1.0e-290 + eps * eps * tnorm * tnorm;
+  lo -= 4.0 * eps * tnorm + pivmin;
+  hi += 4.0 * eps * tnorm + pivmin;
+  int* firsts = reinterpret_cast<int*>(scratch);
+  for (int round = 0; round < 20; ++round) {
+    const double step = (hi - lo) / (double)(nthreads + 1);
+    const double x = lo + step * (double)(tid + 1);
+    // Sturm count: eigenvalues below x
+    int cnt = 0;
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+    for (int i = 1; i < N; ++i) {
+      q = (d[i] - x) - (e[i - 1] * e[i - 1]) * recip(q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += q < 0.0;
+    }
+    // the first shift (over the whole work-group) with at least one
+    // eigenvalue below it brackets the smallest eigenvalue
+    const unsigned long long mask = __ballot(cnt >= 1);
+    if (lane == 0) firsts[wv] = mask ? (__ffsll((long long)mask) - 1) + 64 * wv : nthreads;
+    __syncthreads();
+    int first = nthreads;
+    for (int w = 0; w < nw; ++w) first = min(first, firsts[w]);
+    __syncthreads();
+    const double nlo = lo + step * (double)first;
+    const double nhi = (first < nthreads) ? lo + step * (double)(first + 1) : hi;
+    lo = nlo;
+    hi = nhi;
+    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+  }
+  theta = 0.5 * (lo + hi);
+  // inverse iteration on wave 0: LDL^T of T - sigma I (pivots piv_i), two
+  // solves from b = ones with a normalization between them.  The sequential
+  // sweeps run on every lane with the same scalars; lane i keeps entry i.
+  if (wv == 0) {
+    const double sigma = theta - (8.0 * eps * tnorm + pivmin);
+    double piv = 0.0, rpiv = 0.0, y = 0.0, zi = 0.0;
+    double rprev = 1.0, yprev = 0.0;
+    for (int i = 0; i < N; ++i) {  // pivots and forward sweep L y = 1
+      const double ei = (i > 0) ? e[i - 1] : 0.0;
+      double p = (d[i] - sigma) - ((i > 0) ? ei * ei * rprev : 0.0);
+      if (fabs(p) < pivmin) p = pivmin;
+      const double rp = recip(p);
+      const double l = (i > 0) ? ei * rprev : 0.0;
+      const double yi = 1.0 - l * yprev;
+      if (lane == i) { piv = p; rpiv = rp; y = yi; }
+      rprev = rp;
+      yprev = yi;
+    }
+    (void)piv;
+    // backward: z_i = (y_i - e_i z_{i+1}) / piv_i
+    double znext = 0.0;
+    for (int i = N - 1; i >= 0; --i) {
+      const double ri = __shfl(rpiv, i, 64), yi = __shfl(y, i, 64);
+      const double ei = (i < N - 1) ? e[i] : 0.0;
+      const double zi2 = (yi - ei * znext) * ri;
+      if (lane == i) zi = zi2;
+      znext = zi2;
+    }
+    // normalize and second solve with b = z
+    double s2 = (lane < N) ? zi * zi : 0.0;
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
+    const double b = zi / sqrt(s2);
+    rprev = 1.0;
+    yprev = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const double ei = (i > 0) ? e[i - 1] : 0.0;
+      const double l = (i > 0) ? ei * rprev : 0.0;
+      const double yi = __shfl(b, i, 64) - l * yprev;
+      if (lane == i) y = yi;
+      rprev = __shfl(rpiv, i, 64);
+      yprev = yi;
+    }
+    znext = 0.0;
+    for (int i = N - 1; i >= 0; --i) {
+      const double ri = __shfl(rpiv, i, 64), yi = __shfl(y, i, 64);
+      const double ei = (i < N - 1) ? e[i] : 0.0;
+      const double zi2 = (yi - ei * znext) * ri;
+      if (lane == i) zi = zi2;
+      znext = zi2;
+    }
+    double nrm2 = (lane < N) ? zi * zi : 0.0;
+    for (int o = 32; o > 0; o >>= 1) nrm2 += __shfl_xor(nrm2, o, 64);
+    const double last = __shfl(zi, N - 1, 64);
+    zlast = last / sqrt(nrm2);
+  }
+}
+
+// Smallest eigenvalue of a symmetric n x n (n <= 64) matrix in LDS (ld 65),
+// cyclic Jacobi; lane k updates row/column k.
+static __device__ double jacobi_min_wave(int n, double* A) {
+  const int lane = threadIdx.x;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double v = (lane < n && lane > j) ? A[lane + 65 * j] : 0.0;
+      off += v * v;
+    }
+    off = wsum(off);
+    double dv = lane < n ? A[lane + 65 * lane] : 0.0;
+    dg = wsum(dv * dv);
+    // converged when the off-diagonal norm is below eps times the diagonal's
+    // (a tighter test is never met in FP64 and ran every call to the sweep cap)
+    if (off == 0.0 || off <= 1e-31 * dg) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        __syncthreads();
+        double apq = A[p + 65 * q];
+        if (apq == 0.0) continue;
+        double app = A[p + 65 * p], aqq = A[q + 65 * q];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        __syncthreads();
+        if (lane < n) {
+          double akp = A[lane + 65 * p], akq = A[lane + 65 * q];
+          A[lane + 65 * p] = c * akp - s * akq;
+          A[lane + 65 * q] = s * akp + c * akq;
+        }
+        __syncthreads();
+        if (lane < n) {
+          double apk = A[p + 65 * lane], aqk = A[q + 65 * lane];
+          A[p + 65 * lane] = c * apk - s * aqk;
+          A[q + 65 * lane] = s * apk + c * aqk;
+        }
+      }
+  }
+  __syncthreads();
+  double dv = lane < n ? A[lane + 65 * lane] : 1.0e300;
+  for (int o = 32; o > 0; o >>= 1) dv = fmin(dv, __shfl_xor(dv, o, 64));
+  return dv;
+}
+
+}  // namespace sdpa_amd

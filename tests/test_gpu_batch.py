@@ -1,0 +1,121 @@
+"""Batched small-SDP solver (include/sdpa_batch.h): every instance of one
+launch against the oracle's SDPA::solve on the same instance and lambda.
+
+Tolerances (FP64 on both sides, different summation orders): the oracle's
+phase; iteration count within one; objectives within 1e-6 relative (1e-8
+absolute), except on the unbounded side of a pUNBD/dUNBD instance, where
+only the bound crossing is fixed; at pdOPT the relative gap and both feasibility errors at the 1e-7
+stopping level.  Instances cover the Jacobi (n <= 20) and Lanczos (n > 20)
+step-length branches, trace rows (m = n + 1), dense C (Lovasz), ownership
+ties, the reference's example1, a pUNBD and a dUNBD case, n and m at the
+limits (64, 72), and per-instance lambda.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sdpa_amd import problems as PR
+
+from test_gpu_parity import sparse_mixed
+
+pytestmark = pytest.mark.gpu
+
+sa = pytest.importorskip("sdpa_amd")
+B = pytest.importorskip("sdpa_amd.batch")
+
+
+def unbounded():
+    # the user dual asks F_1.Y = 1 and F_2.Y = 2 with F_1 = F_2 = E11
+    recs = [(0, 1, 1, 1, 1.0), (0, 1, 2, 2, 1.0), (0, 1, 3, 3, 1.0), (1, 1, 1, 1, 1.0),
+            (2, 1, 1, 1, 1.0), (3, 1, 2, 2, 1.0), (3, 1, 3, 3, 1.0), (4, 1, 2, 3, 1.0)]
+    K, L, I, J, V = PR._records(recs)
+    return PR._make(4, [3], [1, 2, 1, 0], K, L, I, J, V, "unbounded_n3_m4")
+
+
+def scaled_maxcut():
+    d = PR.maxcut(20, 0.3, 7)
+    d.v = d.v * np.where(d.k == 0, 1e3, 1.0)
+    return d
+
+
+def at_limits():
+    # n = 64 with 8 extra rows: m = 72
+    d = PR.maxcut(64, 0.2, 9)
+    rng = np.random.default_rng(9)
+    K, I, J, V = [d.k], [d.i], [d.j], [d.v]
+    for t in range(8):
+        a = rng.choice(64, 6, replace=False) + 1
+        K.append(np.full(6, 65 + t, np.int32)); I.append(a.astype(np.int32))
+        J.append(a.astype(np.int32)); V.append(np.ones(6))
+    c = np.concatenate([d.c, np.full(8, 6.0)])
+    K = np.concatenate(K); I = np.concatenate(I); J = np.concatenate(J); V = np.concatenate(V)
+    return PR._make(72, [64], c, K, np.ones_like(K), I, J, V, "maxcut_n64_m72")
+
+
+def check(d, o, r, t):
+    msg = f"{d.name} n={d.block_struct[0]} m={d.m}"
+    assert r.phase[t] == o.phase, msg
+    assert abs(int(r.iterations[t]) - o.iterations) <= 1, msg
+    if o.phase in ("pUNBD", "dUNBD"):
+        # a diverging iterate: the objective on the unbounded side is only
+        # fixed to be past the bound (1e5) at the iteration that detects it
+        if o.phase == "pUNBD":
+            assert r.primal_obj[t] <= -1e5 and o.primal_obj <= -1e5, msg
+        else:
+            assert r.dual_obj[t] >= 1e5 and o.dual_obj >= 1e5, msg
+        return
+    assert r.primal_obj[t] == pytest.approx(o.primal_obj, rel=1e-6, abs=1e-8), msg
+    assert r.dual_obj[t] == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8), msg
+    if o.phase == "pdOPT":
+        assert r.relative_gap[t] <= 1e-7, msg
+        assert r.primal_error[t] <= 1e-7 and r.dual_error[t] <= 1e-7, msg
+        assert r.failure[t] == 0, msg
+
+
+def instances():
+    insts = PR.maxcut_batch(16, nmin=8, nmax=64, p=0.2, seed=4)
+    insts += [PR.lovasz(12, 0.3, 5), PR.lovasz(16, 0.25, 6), PR.example1(),
+              sparse_mixed(30, 60, 20, 5), unbounded(), scaled_maxcut(), at_limits()]
+    return insts
+
+
+@pytest.mark.parametrize("threads", [256, 128])
+def test_batch_matches_oracle(threads):
+    insts = instances()
+    print(f"batch of {len(insts)} instances, {threads} threads", flush=True)
+    r = B.solve(insts, threads=threads)
+    for t, d in enumerate(insts):
+        o = O.Problem(d).solve()
+        print(d.name, r.phase[t], o.phase, r.iterations[t], o.iterations, r.primal_obj[t],
+              o.primal_obj)
+        check(d, o, r, t)
+    assert r.kernel_ms > 0
+
+
+def test_batch_per_instance_lambda():
+    insts = [scaled_maxcut(), PR.maxcut(30, 0.2, 3), PR.lovasz(14, 0.3, 8)]
+    lams = np.array([1e4, 10.0, 1e3])
+    r = B.solve(insts, lambdas=lams)
+    for t, d in enumerate(insts):
+        o = O.Problem(d).solve(O.default_param(lambdaStar=float(lams[t])))
+        check(d, o, r, t)
+
+
+def test_batch_parameters_and_repeat():
+    # maxIteration caps the loop (noINFO-or-feasible phase, iterations = cap);
+    # a larger batch (instances repeated, more than one per work-group) gives
+    # every copy the same answer
+    insts = PR.maxcut_batch(6, nmin=10, nmax=40, seed=12)
+    r = B.solve(insts, maxIteration=4)
+    for t, d in enumerate(insts):
+        o = O.Problem(d).solve(O.default_param(maxIteration=4))
+        assert r.iterations[t] == o.iterations == 4
+        assert r.phase[t] == o.phase
+        assert r.primal_obj[t] == pytest.approx(o.primal_obj, rel=1e-9)
+    many = insts * 100  # 600 instances
+    rm = B.solve(many)
+    r1 = B.solve(insts)
+    for t in range(len(many)):
+        assert rm.iterations[t] == r1.iterations[t % 6]
+        assert rm.primal_obj[t] == r1.primal_obj[t % 6]
+        assert rm.dual_obj[t] == r1.dual_obj[t % 6]
