@@ -2,17 +2,30 @@
 // work-group per instance runs SDPA's whole Mehrotra predictor-corrector
 // loop (SDPA::solve, sdpa_solve.cpp:59-233): the Newton direction (Make_gVec
 // sdpa_newton.cpp:817-883, F3 bMat 1319-1403, rATL_dpotrfL sdpa_dpotrf.cpp:
-// 218-265, compute_DzMat/DxMat 2718-2752), the step lengths (Lal::
-// getMinEigen, sdpa_linear.cpp:29-182) and the control scalars
-// (sdpa_parts.cpp:493-1242) -- the same formulas as the single-problem path,
-// with every scalar decision taken on the device.  Instances: one SDP block
-// of n <= 64, m <= 72 constraints with sparse data (all F3), no LP block.
+// 218-265, compute_DzMat/DxMat 2718-2752), the step lengths
+// (Jal::getMinEigen, sdpa_jordan.cpp:55-90: Lanczos for n > 20, all
+// eigenvalues for n <= 20) and the control scalars (sdpa_parts.cpp:
+// 493-1242) -- the formulas of the single-problem path, with every scalar
+// decision taken on the device.  Instances: one SDP block of n <= 64,
+// m <= 72 constraints with at most kEnt stored entries, no LP block.
 //
-// Every thread of a work-group evaluates the control scalars from the same
-// block reductions, so control flow is uniform and needs no broadcast.
+// Layout per work-group: the instance's constraint entries, Schur matrix and
+// GEMM operands live in LDS (about 137 KB, one work-group per CU); the n x n
+// iterates and directions live in a per-instance global slab (L2-resident).
+// Work-groups take instances from a queue, largest first.
+//
+// Every thread evaluates the control scalars from the same block
+// reductions, so control flow is uniform and needs no broadcast.  The
+// sequential kernels of one iteration use one wave where a chain of
+// dependent steps would otherwise pay a barrier per step: the Cholesky
+// factors and inverses of X and Z (one wave each, concurrently), the
+// triangular solves with the Schur factor, and the Householder reduction of
+// small blocks.  Products of n x n matrices are FP64 MFMA tiles.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <vector>
 
 #include "../../include/sdpa_batch.h"
@@ -21,8 +34,11 @@
 namespace sdpa_amd {
 namespace {
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr int NMAX = 64;        // block size limit
 constexpr int MMAX = 72;        // constraint limit
+constexpr int kEnt = 1536;      // stored constraint entries per instance
 constexpr int LDA = NMAX + 1;   // LDS leading dimension of block matrices
 constexpr int LDB = MMAX + 1;   // of the Schur matrix
 constexpr int kMats = 12;       // n x n work matrices per instance
@@ -49,57 +65,86 @@ struct BData {
   const int* ec;
   const double* ev;
   const double* lambda;
-  const long* woff;  // work slab of each instance (kMats n^2 doubles)
+  const long* woff;   // work slab of each instance (kMats n^2 doubles)
+  const int* order;   // instances, largest first
+  unsigned* next;     // queue head (zero at launch)
   double* work;
   double* out;
 };
 
 template <int BT>
 struct Shared {
-  double A[NMAX * LDA];   // block-matrix scratch (GEMM operand, factor)
-  double B2[NMAX * LDA];  // second operand / inverse
-  double S[MMAX * LDB];   // Schur matrix and its factor
-  double y[MMAX], dy[MMAX], g[MMAX], p[MMAX], bv[MMAX], tv[MMAX];
+  double A[NMAX * LDA];   // GEMM operand / X factor / step-length matrix
+  double B2[NMAX * LDA];  // GEMM operand / Z factor
+  double S[MMAX * LDB];   // Schur matrix and its factor (lower)
+  double ev[kEnt];        // constraint entries: value, row, column, constraint
+  double y[MMAX], dy[MMAX], g[MMAX], p[MMAX], bv[MMAX], piv[MMAX];
   double lq[NMAX], lqo[NMAX], lr[NMAX], lw[NMAX], dg[kLz], od[kLz];
-  double red[BT / 64 + 1];
+  double red[8 * (BT / 64)];
   double trisc[2 * (BT / 64)];
   double sc[4];
-  int info;
+  unsigned short perm[kEnt];  // entries in position-major order
+  unsigned short cp[MMAX + 1];
+  unsigned char er[kEnt], ec[kEnt], con[kEnt], head[kEnt];
+  int nz[MMAX];
+  int info[2];
+  int task;
 };
+
+// the same value in every lane, as the compiler's uniform (scalar) value
+__device__ __forceinline__ double uni(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// block sums of K values; every thread receives all K
+template <int BT, int K>
+__device__ __forceinline__ void bsums(double (&v)[K], Shared<BT>& sh) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wsum(v[k]);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh.red[(threadIdx.x >> 6) * K + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < BT / 64; ++w) s += sh.red[w * K + k];
+    v[k] = s;
+  }
+}
 
 template <int BT>
 __device__ __forceinline__ double bsum(double v, Shared<BT>& sh) {
-  v = wsum(v);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double s = 0.0;
-  for (int i = 0; i < BT / 64; ++i) s += sh.red[i];
-  return s;
+  double a[1] = {v};
+  bsums<BT, 1>(a, sh);
+  return a[0];
 }
 
 template <int BT>
-__device__ __forceinline__ double bmax(double v, Shared<BT>& sh) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+__device__ __forceinline__ void bmax2(double& a, double& b, Shared<BT>& sh) {
+  a = wmax(a);
+  b = wmax(b);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = v;
+  if ((threadIdx.x & 63) == 0) {
+    sh.red[2 * (threadIdx.x >> 6)] = a;
+    sh.red[2 * (threadIdx.x >> 6) + 1] = b;
+  }
   __syncthreads();
-  double s = 0.0;
-  for (int i = 0; i < BT / 64; ++i) s = fmax(s, sh.red[i]);
-  return s;
-}
-
-// sum of a[e] b[e] over len elements (global or LDS)
-template <int BT>
-__device__ double dot(const double* a, const double* b, int len, Shared<BT>& sh) {
-  double s = 0.0;
-  for (int e = threadIdx.x; e < len; e += BT) s += a[e] * b[e];
-  return bsum(s, sh);
+  a = b = 0.0;
+  for (int w = 0; w < BT / 64; ++w) {
+    a = fmax(a, sh.red[2 * w]);
+    b = fmax(b, sh.red[2 * w + 1]);
+  }
 }
 
 // out = a A + b B + c Cm (B, Cm may be null)
 template <int BT>
-__device__ void axpby(double* out, double a, const double* A, double b, const double* B, double c,
+__device__ __forceinline__ void axpby(double* out, double a, const double* A, double b, const double* B, double c,
                       const double* Cm, int len) {
   for (int e = threadIdx.x; e < len; e += BT) {
     double v = a * A[e];
@@ -110,156 +155,318 @@ __device__ void axpby(double* out, double a, const double* A, double b, const do
   __syncthreads();
 }
 
-struct Prob {
-  int n, m;
-  const double* C;
-  const double* b;
-  const int* cp;  // m + 1 global entry indices
-  const int* er;
-  const int* ec;
-  const double* ev;
-};
-
-// out_i = A_i . D (symmetric inner product of the stored upper entries)
+// The instance's constraint data into LDS: entries, per-constraint pointers
+// and NonZeroEffect (n^2 for a dense A_i, >= 20% of the entries stored,
+// sdpa_struct.cpp:518; else the entry count with off-diagonals twice), and
+// the entries in position-major order (position r + n c, then constraint)
+// with run heads, for the deterministic sum_i y_i A_i.
 template <int BT>
-__device__ void adot(const Prob& P, const double* D, double* out) {
-  for (int i = threadIdx.x; i < P.m; i += BT) {
+__device__ __forceinline__ int stage(const BData& Dt, int k, int n, int m, Shared<BT>& sh) {
+  const int tid = threadIdx.x;
+  const int* gcp = Dt.cptr + Dt.pptr[k];
+  const int e0 = gcp[0];
+  const int ne = gcp[m] - e0;
+  for (int i = tid; i <= m; i += BT) sh.cp[i] = (unsigned short)(gcp[i] - e0);
+  for (int e = tid; e < ne; e += BT) {
+    sh.er[e] = (unsigned char)Dt.er[e0 + e];
+    sh.ec[e] = (unsigned char)Dt.ec[e0 + e];
+    sh.ev[e] = Dt.ev[e0 + e];
+  }
+  const double* b = Dt.b + Dt.boff[k];
+  for (int i = tid; i < m; i += BT) {
+    sh.bv[i] = b[i];
+    sh.y[i] = 0.0;
+  }
+  __syncthreads();
+  for (int i = tid; i < m; i += BT) {
+    const int a = sh.cp[i], z = sh.cp[i + 1];
+    int s = 0;
+    for (int e = a; e < z; ++e) {
+      sh.con[e] = (unsigned char)i;
+      s += (sh.er[e] == sh.ec[e]) ? 1 : 2;
+    }
+    sh.nz[i] = ((double)(z - a) >= (double)n * n * 0.20) ? n * n : s;
+  }
+  __syncthreads();
+  // sort keys (position, constraint, entry) are distinct: the ranks are a
+  // permutation whatever the data (repeated entries just add up)
+  int* key = reinterpret_cast<int*>(sh.S);
+  for (int e = tid; e < ne; e += BT)
+    key[e] = ((sh.er[e] + n * sh.ec[e]) * MMAX + sh.con[e]) * kEnt + e;
+  __syncthreads();
+  for (int e = tid; e < ne; e += BT) {
+    const int ke = key[e];
+    int rank = 0;
+    for (int f = 0; f < ne; ++f) rank += key[f] < ke;
+    sh.perm[rank] = (unsigned short)e;
+  }
+  __syncthreads();
+  for (int s = tid; s < ne; s += BT) {
+    const int e = sh.perm[s];
+    int h = 1;
+    if (s > 0) {
+      const int f = sh.perm[s - 1];
+      h = (sh.er[f] != sh.er[e] || sh.ec[f] != sh.ec[e]) ? 1 : 0;
+    }
+    sh.head[s] = (unsigned char)h;
+  }
+  __syncthreads();
+  return ne;
+}
+
+// out_i = A_i . D (D global n x n; off-diagonal entries count twice)
+template <int BT>
+__device__ __forceinline__ void adot(int n, int m, const double* D, double* out, Shared<BT>& sh) {
+  for (int i = threadIdx.x; i < m; i += BT) {
     double s = 0.0;
-    for (int e = P.cp[i]; e < P.cp[i + 1]; ++e) {
-      const int r = P.er[e], c = P.ec[e];
-      double d = D[r + P.n * c];
-      if (r != c) d += D[c + P.n * r];
-      s += P.ev[e] * d;
+    for (int e = sh.cp[i]; e < sh.cp[i + 1]; ++e) {
+      const int r = sh.er[e], c = sh.ec[e];
+      double d = D[r + n * c];
+      if (r != c) d += D[c + n * r];
+      s += sh.ev[e] * d;
     }
     out[i] = s;
   }
   __syncthreads();
 }
 
-// out = sum_i y_i A_i (entries scattered by one thread, in order)
+// out = sg sum_i yv_i A_i + c1 M1 + c2 M2 (global n x n; M1, M2 may be
+// null).  Each position is summed by one thread in constraint order.
 template <int BT>
-__device__ void sum_yA(const Prob& P, const double* y, double* out) {
-  const int nn = P.n * P.n;
-  for (int e = threadIdx.x; e < nn; e += BT) out[e] = 0.0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < P.m; ++i)
-      for (int e = P.cp[i]; e < P.cp[i + 1]; ++e) {
-        const int r = P.er[e], c = P.ec[e];
-        out[r + P.n * c] += y[i] * P.ev[e];
-        if (r != c) out[c + P.n * r] += y[i] * P.ev[e];
-      }
-  }
-  __syncthreads();
-}
-
-// C = op(A) op(B) (global n x n, ld n), through LDS
-template <int BT>
-__device__ void gemm(int n, const double* A, int ta, const double* B, int tb, double* C,
-                     Shared<BT>& sh) {
+__device__ __forceinline__ void yA_comb(int n, int ne, const double* yv, double sg, const double* M1, double c1,
+                        const double* M2, double c2, double* out, Shared<BT>& sh) {
   const int nn = n * n;
   for (int e = threadIdx.x; e < nn; e += BT) {
-    const int i = e % n, j = e / n;
-    sh.A[(ta ? j : i) + LDA * (ta ? i : j)] = A[e];
-    sh.B2[(tb ? j : i) + LDA * (tb ? i : j)] = B[e];
+    double v = 0.0;
+    if (M1) v += c1 * M1[e];
+    if (M2) v += c2 * M2[e];
+    out[e] = v;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < nn; e += BT) {
-    const int i = e % n, j = e / n;
-    double s0 = 0.0, s1 = 0.0;
-    int k = 0;
-    for (; k + 1 < n; k += 2) {
-      s0 += sh.A[i + LDA * k] * sh.B2[k + LDA * j];
-      s1 += sh.A[i + LDA * (k + 1)] * sh.B2[(k + 1) + LDA * j];
+  for (int s = threadIdx.x; s < ne; s += BT) {
+    if (!sh.head[s]) continue;
+    const int e = sh.perm[s];
+    const int r = sh.er[e], c = sh.ec[e];
+    double v = 0.0;
+    for (int t = s; t < ne; ++t) {
+      if (t > s && sh.head[t]) break;
+      const int f = sh.perm[t];
+      v += yv[sh.con[f]] * sh.ev[f];
     }
-    if (k < n) s0 += sh.A[i + LDA * k] * sh.B2[k + LDA * j];
-    C[e] = s0 + s1;
+    out[r + n * c] += sg * v;
+    if (r != c) out[c + n * r] += sg * v;
   }
   __syncthreads();
 }
 
-// in-place lower Cholesky of the n x n matrix M (LDS, ld) -- mode 0: LAPACK
-// (a pivot that is not > 0 fails), mode 1: rATL_dpotrfL's rule (pivot < -1e-6
-// fails, pivot < 1e-14 becomes 1e100).  Returns 0 or the 1-based column.
+// acc = op(A) op(B) on the 32x32 quarter at (r0, c0) of a 64x64 product,
+// A and B in LDS (ld LDA, zero beyond the data up to kd).  The MFMA's A
+// operand is fed from B and its B operand from A, so the 16x16 result is the
+// transpose: acc[ti][tj][v] is row r0 + 16 ti + lane%16, column
+// c0 + 16 tj + 4 v + lane/16 (f64 16x16x4: D[i][j] in lane j + 16 (i%4),
+// register i/4).
+__device__ __forceinline__ void mma_quarter(v4d acc[2][2], const double* As, int ta,
+                                            const double* Bs, int tb, int kd, int r0, int c0) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int kk = 0; kk < kd; kk += 4) {
+    const int k = kk + lk;
+    double a[2], b[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int col = c0 + 16 * t + li, row = r0 + 16 * t + li;
+      a[t] = tb ? Bs[col + LDA * k] : Bs[k + LDA * col];
+      b[t] = ta ? As[k + LDA * row] : As[row + LDA * k];
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+        acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tj], b[ti], acc[ti][tj], 0, 0, 0);
+  }
+}
+
+// C = op(As) op(Bs) (LDS operands, n x n) into global C (ld n), or into sh.A
+// (ld LDA) when C is null.  Operands must be zero in rows/columns [n, 64).
 template <int BT>
-__device__ int chol(double* M, int ld, int n, int mode, Shared<BT>& sh) {
-  if (threadIdx.x == 0) sh.info = 0;
+__device__ __forceinline__ void mma_lds(int n, const double* As, int ta, const double* Bs, int tb, double* C,
+                        Shared<BT>& sh) {
+  constexpr int NW = BT / 64, NQ = 4 / NW;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int kd = (n + 3) & ~3;
+  v4d acc[NQ][2][2];
+#pragma unroll
+  for (int s = 0; s < NQ; ++s) {
+    const int q = w + NW * s, r0 = 32 * (q & 1), c0 = 32 * (q >> 1);
+    if (r0 < n && c0 < n) {
+      mma_quarter(acc[s], As, ta, Bs, tb, kd, r0, c0);
+    } else {
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) acc[s][ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+  }
+  if (!C) __syncthreads();  // the operands may be sh.A
+  double* out = C ? C : sh.A;
+  const int ldo = C ? n : LDA;
+#pragma unroll
+  for (int s = 0; s < NQ; ++s) {
+    const int q = w + NW * s, r0 = 32 * (q & 1), c0 = 32 * (q >> 1);
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + 16 * ti + (lane & 15), col = c0 + 16 * tj + 4 * v + (lane >> 4);
+          if (row < n && col < n) out[row + ldo * col] = acc[s][ti][tj][v];
+        }
+  }
   __syncthreads();
-  for (int k = 0; k < n; ++k) {
+}
+
+// an n x n global matrix into LDS (ld LDA), zero outside it up to the extent
+// the product's quarters read (32 or 64)
+template <int BT>
+__device__ __forceinline__ void load_pad(int n, const double* G, double* L) {
+  const int R = n <= 32 ? 32 : 64;
+  for (int e = threadIdx.x; e < R * R; e += BT) {
+    const int i = e % R, j = e / R;
+    L[i + LDA * j] = (i < n && j < n) ? G[i + n * j] : 0.0;
+  }
+}
+
+// C = op(A) op(B), A, B global n x n; C global or (null) sh.A
+template <int BT>
+__device__ __forceinline__ void gemm(int n, const double* A, int ta, const double* B, int tb, double* C,
+                     Shared<BT>& sh) {
+  load_pad<BT>(n, A, sh.A);
+  load_pad<BT>(n, B, sh.B2);
+  __syncthreads();
+  mma_lds(n, sh.A, ta, sh.B2, tb, C, sh);
+}
+
+// Cholesky of the Schur matrix in sh.S with rATL_dpotrfL's pivot rule
+// (sdpa_dpotrf.cpp:218-265: a pivot below -1e-6 fails, one below 1e-14
+// becomes 1e100) and its operations (L21 = A21 / L11, A22 -= L21 L21^T):
+// right-looking, the column scaled before the trailing update.  On success
+// S holds L (lower) and sh.piv[k] = 1 / L_kk.  Returns 0 or k + 1.
+template <int BT>
+__device__ __forceinline__ int chol_schur(int m, Shared<BT>& sh) {
+  double* S = sh.S;
+  for (int k = 0; k < m; ++k) {
+    double d = uni(S[k + LDB * k]);
+    if (d < -1.0e-6) return k + 1;
+    if (d < 1.0e-14) d = 1.0e100;
+    const double s = sqrt(d);
+    for (int i = k + 1 + threadIdx.x; i < m; i += BT) S[i + LDB * k] /= s;
     if (threadIdx.x == 0) {
-      double d = M[k + ld * k];
-      bool bad;
-      if (mode == 0) {
-        bad = !(d > 0.0);
-      } else {
-        bad = d < -1.0e-6;
-        if (!bad && d < 1.0e-14) d = 1.0e100;
-      }
-      if (bad) sh.info = k + 1;
-      M[k + ld * k] = bad ? 1.0 : sqrt(d);
+      S[k + LDB * k] = s;
+      sh.piv[k] = 1.0 / s;
     }
     __syncthreads();
-    if (sh.info) return sh.info;
-    const double piv = M[k + ld * k];
-    for (int i = k + 1 + threadIdx.x; i < n; i += BT) M[i + ld * k] /= piv;
-    __syncthreads();
-    const int rem = n - k - 1;
+    const int rem = m - k - 1;
     for (int e = threadIdx.x; e < rem * rem; e += BT) {
       const int i = k + 1 + e % rem, j = k + 1 + e / rem;
-      if (i >= j) M[i + ld * j] -= M[i + ld * k] * M[j + ld * k];
+      // the upper triangle (not read again) takes a zero update: no branch
+      S[i + LDB * j] -= (i >= j) ? S[i + LDB * k] * S[j + LDB * k] : 0.0;
     }
     __syncthreads();
   }
   return 0;
 }
 
-// W = L^-1 (LDS, lower, zero above) from the lower factor L (LDS)
+// x := L^-T L^-1 x with the Schur factor (one wave; lane l keeps x_l and
+// x_{l+64} in registers, the step's pivot value comes by readlane)
 template <int BT>
-__device__ void trinv(const double* L, double* W, int n) {
-  const int j = threadIdx.x;
-  if (j < n) {
-    for (int i = 0; i < j; ++i) W[i + LDA * j] = 0.0;
-    for (int i = j; i < n; ++i) {
-      double s = (i == j) ? 1.0 : 0.0;
-      for (int k = j; k < i; ++k) s -= L[i + LDA * k] * W[k + LDA * j];
-      W[i + LDA * j] = s / L[i + LDA * i];
+__device__ __forceinline__ void schur_solve(int m, double* x, Shared<BT>& sh) {
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
+    const double* S = sh.S;
+    double x0 = l < m ? x[l] : 0.0, x1 = l + 64 < m ? x[l + 64] : 0.0;
+    const double p0 = l < m ? sh.piv[l] : 0.0, p1 = l + 64 < m ? sh.piv[l + 64] : 0.0;
+    // the column (row) of L each step needs is loaded unconditionally (rows
+    // past m read the padded array and are masked by a select), so the loads
+    // do not wait on the step's pivot value
+    const int l1 = (l + 64 < MMAX) ? l + 64 : MMAX - 1;
+    for (int k = 0; k < m; ++k) {
+      const double t0 = S[l + LDB * k], t1 = S[l1 + LDB * k];
+      const double xk = k < 64 ? rdlane(x0, k) * rdlane(p0, k) : rdlane(x1, k - 64) * rdlane(p1, k - 64);
+      x0 = (l == k) ? xk : ((l > k && l < m) ? x0 - t0 * xk : x0);
+      x1 = (l + 64 == k) ? xk : ((l + 64 > k && l + 64 < m) ? x1 - t1 * xk : x1);
     }
+    for (int k = m - 1; k >= 0; --k) {
+      const double t0 = S[k + LDB * l], t1 = S[k + LDB * l1];
+      const double xk = k < 64 ? rdlane(x0, k) * rdlane(p0, k) : rdlane(x1, k - 64) * rdlane(p1, k - 64);
+      x0 = (l == k) ? xk : ((l < k) ? x0 - t0 * xk : x0);
+      x1 = (l + 64 == k) ? xk : ((l + 64 < k) ? x1 - t1 * xk : x1);
+    }
+    if (l < m) x[l] = x0;
+    if (l + 64 < m) x[l + 64] = x1;
   }
   __syncthreads();
 }
 
-// x := L^-T L^-1 x with the Schur factor in sh.S (x in LDS)
-template <int BT>
-__device__ void schur_solve(int m, double* x, Shared<BT>& sh) {
-  for (int k = 0; k < m; ++k) {
-    if (threadIdx.x == 0) x[k] /= sh.S[k + LDB * k];
-    __syncthreads();
-    for (int i = k + 1 + threadIdx.x; i < m; i += BT) x[i] -= sh.S[i + LDB * k] * x[k];
-    __syncthreads();
-  }
-  for (int k = m - 1; k >= 0; --k) {
-    if (threadIdx.x == 0) {
-      double s = x[k];
-      for (int i = k + 1; i < m; ++i) s -= sh.S[i + LDB * k] * x[i];
-      x[k] = s / sh.S[k + LDB * k];
+// One wave: M (LDS, ld LDA, symmetric n x n, lower part read) := L^-1
+// (lower) where M = L L^T, with LAPACK's rule (a pivot that is not > 0
+// fails).  Lane l owns row l; column values of the step come by readlane.
+// The inverse is dtrti2's column sweep from the right: column j of L^-1 is
+// -(L^-1)_{22} L_{21} / L_jj with the trailing inverse already in place.
+// Returns 0 or k + 1.
+__device__ __forceinline__ int chol_inv_wave(int n, double* M) {
+  const int l = threadIdx.x & 63;
+  for (int k = 0; k < n; ++k) {
+    const double d = uni(M[k + LDA * k]);
+    if (!(d > 0.0)) return k + 1;
+    const double s = sqrt(d), r = 1.0 / s;
+    const bool below = l > k && l < n;
+    const double li = below ? M[l + LDA * k] * r : 0.0;
+    if (below) M[l + LDA * k] = li;
+    if (l == k) M[k + LDA * k] = s;
+    // rows l < j (and l >= n) subtract zero: no branch, the loads of
+    // successive columns can be in flight together
+#pragma unroll 4
+    for (int j = k + 1; j < n; ++j) {
+      const double lj = rdlane(li, j);
+      M[l + LDA * j] -= (l >= j) ? li * lj : 0.0;
     }
-    __syncthreads();
   }
+  for (int j = n - 1; j >= 0; --j) {
+    const double ajj = 1.0 / M[j + LDA * j];
+    const double x = (l > j && l < n) ? M[l + LDA * j] : 0.0;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = j + 1; k < n; ++k) {
+      const double xk = rdlane(x, k);
+      const double t = M[l + LDA * k];
+      acc += (l >= k && l < n) ? t * xk : 0.0;
+    }
+    if (l > j && l < n) M[l + LDA * j] = -ajj * acc;
+    if (l == j) M[j + LDA * j] = ajj;
+  }
+  return 0;
 }
 
-// calF3 with A_j the owner (outer loop), sdpa_newton.cpp:1319-1403
-__device__ double calF3(const Prob& P, const double* X, const double* Zi, int i, int j) {
-  const int n = P.n;
+// calF3 with A_j the owner (outer loop), sdpa_newton.cpp:1319-1403, over
+// the owner's entries e1 = first, first + step, ...; X and Z^-1 in LDS
+template <int BT>
+__device__ __forceinline__ double calF3(const Shared<BT>& sh, int i, int j, int first, int step) {
+  const double* X = sh.A;
+  const double* Zi = sh.B2;
   double ret = 0.0;
-  for (int e1 = P.cp[j]; e1 < P.cp[j + 1]; ++e1) {
-    const int a = P.er[e1], b = P.ec[e1];
-    const double* Xa = X + n * a;
-    const double* Xb = X + n * b;
-    const double* Za = Zi + n * a;
-    const double* Zb = Zi + n * b;
+  for (int e1 = sh.cp[j] + first; e1 < sh.cp[j + 1]; e1 += step) {
+    const int a = sh.er[e1], b = sh.ec[e1];
+    const double* Xa = X + LDA * a;
+    const double* Xb = X + LDA * b;
+    const double* Za = Zi + LDA * a;
+    const double* Zb = Zi + LDA * b;
     double sum = 0.0;
-    for (int e2 = P.cp[i]; e2 < P.cp[i + 1]; ++e2) {
-      const int g = P.er[e2], d = P.ec[e2];
+    for (int e2 = sh.cp[i]; e2 < sh.cp[i + 1]; ++e2) {
+      const int g = sh.er[e2], d = sh.ec[e2];
       double plu;
       if (a != b) {
         plu = (g != d) ? Zb[d] * Xa[g] + Zb[g] * Xa[d] + Za[d] * Xb[g] + Za[g] * Xb[d]
@@ -267,103 +474,138 @@ __device__ double calF3(const Prob& P, const double* X, const double* Zi, int i,
       } else {
         plu = (g != d) ? Zb[d] * Xa[g] + Zb[g] * Xa[d] : Zb[d] * Xa[g];
       }
-      sum += P.ev[e2] * plu;
+      sum += sh.ev[e2] * plu;
     }
-    ret += P.ev[e1] * sum;
+    ret += sh.ev[e1] * sum;
   }
   return ret;
 }
 
-// NonZeroEffect: n^2 for a dense A_i (>= 20% of the entries stored,
-// sdpa_struct.cpp:518), else the entry count with off-diagonals twice
-__device__ __forceinline__ int nzeff(const Prob& P, int i) {
-  const int cnt = P.cp[i + 1] - P.cp[i];
-  if ((double)cnt >= (double)P.n * P.n * 0.20) return P.n * P.n;
-  int s = 0;
-  for (int e = P.cp[i]; e < P.cp[i + 1]; ++e) s += (P.er[e] == P.ec[e]) ? 1 : 2;
-  return s;
+constexpr int kHeavy = 64;  // pairs with more entry products than this: one wave each
+
+// the owner of pair (i, j): the larger NonZeroEffect (ties: the smaller index)
+template <int BT>
+__device__ __forceinline__ bool j_owns(const Shared<BT>& sh, int i, int j) {
+  const int jnz = sh.nz[j], inz = sh.nz[i];
+  return !((jnz < inz) || (inz == jnz && i < j));
 }
 
-// the Schur matrix into sh.S (both triangles), F3 with SDPA's ownership rule
+// the Schur matrix into sh.S (both triangles), F3 with SDPA's ownership
+// rule: light pairs one thread each, heavy pairs one wave each (the lanes
+// split the owner's entries)
 template <int BT>
-__device__ void make_bmat(const Prob& P, const double* X, const double* Zi, Shared<BT>& sh) {
-  const int m = P.m, np = m * (m + 1) / 2;
-  for (int t = threadIdx.x; t < np; t += BT) {
-    // pair (i, j), i >= j, column by column
-    int j = 0, r = t;
-    while (r >= m - j) { r -= m - j; ++j; }
-    const int i = j + r;
-    const int jnz = nzeff(P, j), inz = nzeff(P, i);
-    // the owner is the one with the larger NonZeroEffect (ties: the smaller index)
-    const bool jown = !((jnz < inz) || (inz == jnz && i < j));
-    const double v = jown ? calF3(P, X, Zi, i, j) : calF3(P, X, Zi, j, i);
+__device__ __forceinline__ void make_bmat(int n, int m, const double* X, const double* Zi,
+                                          Shared<BT>& sh) {
+  for (int e = threadIdx.x; e < n * n; e += BT) {
+    const int i = e % n, j = e / n;
+    sh.A[i + LDA * j] = X[e];
+    sh.B2[i + LDA * j] = Zi[e];
+  }
+  __syncthreads();
+  const int mm = m * m;
+  for (int t = threadIdx.x; t < mm; t += BT) {
+    const int i = t % m, j = t / m;
+    if (i < j) continue;
+    if ((sh.cp[i + 1] - sh.cp[i]) * (sh.cp[j + 1] - sh.cp[j]) > kHeavy) continue;
+    const double v = j_owns(sh, i, j) ? calF3(sh, i, j, 0, 1) : calF3(sh, j, i, 0, 1);
     sh.S[i + LDB * j] = v;
     sh.S[j + LDB * i] = v;
   }
-  __syncthreads();
-}
-
-// smallest eigenvalue of the symmetric n x n matrix in sh.A by cyclic Jacobi
-// (the dsyev branch of Lal::getMinEigen for n <= 20, sdpa_linear.cpp:145-182):
-// thread k rotates row/column k; every thread takes the same decisions.
-template <int BT>
-__device__ double jacobi_min(int n, Shared<BT>& sh) {
-  const int t = threadIdx.x;
-  double* A = sh.A;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    __syncthreads();  // the last rotation's writes before the norms
-    double off = 0.0, dv = 0.0;
-    if (t < n) {
-      for (int j = 0; j < t; ++j) off += A[t + LDA * j] * A[t + LDA * j];
-      dv = A[t + LDA * t] * A[t + LDA * t];
+  const int lane = threadIdx.x & 63;
+  for (int t0 = (threadIdx.x >> 6) * 64; t0 < mm; t0 += BT) {
+    const int t = t0 + lane;
+    bool heavy = false;
+    if (t < mm) {
+      const int i = t % m, j = t / m;
+      heavy = i >= j && (sh.cp[i + 1] - sh.cp[i]) * (sh.cp[j + 1] - sh.cp[j]) > kHeavy;
     }
-    off = bsum(off, sh);
-    dv = bsum(dv, sh);
-    // converged at an eps-relative off-diagonal norm
-    if (off == 0.0 || off <= 1e-31 * dv) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        __syncthreads();
-        const double apq = A[p + LDA * q];
-        if (apq == 0.0) continue;
-        const double app = A[p + LDA * p], aqq = A[q + LDA * q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
-        __syncthreads();
-        if (t < n) {
-          const double akp = A[t + LDA * p], akq = A[t + LDA * q];
-          A[t + LDA * p] = c * akp - s * akq;
-          A[t + LDA * q] = s * akp + c * akq;
-        }
-        __syncthreads();
-        if (t < n) {
-          const double apk = A[p + LDA * t], aqk = A[q + LDA * t];
-          A[p + LDA * t] = c * apk - s * aqk;
-          A[q + LDA * t] = s * apk + c * aqk;
-        }
+    unsigned long long mask = __ballot(heavy);
+    while (mask) {
+      const int tt = t0 + __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const int i = tt % m, j = tt / m;
+      const double part = j_owns(sh, i, j) ? calF3(sh, i, j, lane, 64) : calF3(sh, j, i, lane, 64);
+      const double v = wsum(part);
+      if (lane == 0) {
+        sh.S[i + LDB * j] = v;
+        sh.S[j + LDB * i] = v;
       }
+    }
   }
   __syncthreads();
-  double mn = 1.0e300;
-  for (int i = 0; i < n; ++i) mn = fmin(mn, A[i + LDA * i]);
-  __syncthreads();
-  return mn;
 }
 
-// Lal::getMinEigen on the symmetric M (global, n x n): the truncated Lanczos
-// (n > 20) or all eigenvalues by Jacobi (n <= 20).
+// smallest eigenvalue of the symmetric n x n (n <= 20) matrix in sh.A, as
+// the dsyev branch of Jal::getMinEigen (sdpa_jordan.cpp:80-87, lower
+// triangle): Householder reduction to tridiagonal form on wave 0 (lane l
+// owns row l), then the tridiagonal's smallest eigenvalue by Sturm
+// multisection on the whole work-group.
 template <int BT>
-__device__ double min_eig(const double* M, int n, Shared<BT>& sh) {
-  const int tid = threadIdx.x;
-  if (n <= 20) {
-    for (int e = tid; e < n * n; e += BT) sh.A[(e % n) + LDA * (e / n)] = M[e];
-    __syncthreads();
-    return jacobi_min(n, sh);
+__device__ __forceinline__ double small_min_eig(int n, Shared<BT>& sh) {
+  double* A = sh.A;
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
+    for (int j = 0; j < n; ++j)
+      if (l < j) A[l + LDA * j] = A[j + LDA * l];
+    for (int k = 0; k + 2 < n; ++k) {
+      const bool act = l > k && l < n;
+      const double xl = act ? A[l + LDA * k] : 0.0;
+      const double x0 = rdlane(xl, k + 1);
+      const double tail = wsum((l > k + 1 && l < n) ? xl * xl : 0.0);
+      if (tail == 0.0) {  // already tridiagonal in this column
+        if (l == 0) {
+          sh.dg[k] = A[k + LDA * k];
+          sh.od[k] = x0;
+        }
+        continue;
+      }
+      const double alpha = -copysign(sqrt(x0 * x0 + tail), x0);
+      const double vl = (l == k + 1) ? x0 - alpha : xl;
+      const double tau = 2.0 / (tail + (x0 - alpha) * (x0 - alpha));
+      double pl = 0.0;
+      for (int j = k + 1; j < n; ++j) {
+        const double vj = rdlane(vl, j);
+        if (act) pl += A[l + LDA * j] * vj;
+      }
+      pl *= tau;
+      const double K = 0.5 * tau * wsum(pl * vl);
+      const double wl = pl - K * vl;
+      for (int j = k + 1; j < n; ++j) {
+        const double vj = rdlane(vl, j), wj = rdlane(wl, j);
+        if (act) A[l + LDA * j] -= vl * wj + wl * vj;
+      }
+      if (l == 0) {
+        sh.dg[k] = A[k + LDA * k];
+        sh.od[k] = alpha;
+      }
+    }
+    if (l == 0) {
+      if (n >= 2) {
+        sh.dg[n - 2] = A[(n - 2) + LDA * (n - 2)];
+        sh.od[n - 2] = A[(n - 1) + LDA * (n - 2)];
+      }
+      sh.dg[n - 1] = A[(n - 1) + LDA * (n - 1)];
+    }
   }
+  __syncthreads();
+  double th = 0.0, zl = 0.0;
+  tridiag_min_block(n, sh.dg, sh.od, th, zl, BT, sh.trisc, false);
+  __syncthreads();
+  return th;
+}
+
+// Lal::getMinEigen (sdpa_linear.cpp:29-142) on the symmetric n x n (n > 20)
+// matrix in sh.A: the truncated Lanczos recurrence with its stopping rules.
+template <int BT>
+__device__ __forceinline__ double lanczos(int n, Shared<BT>& sh) {
+  const int tid = threadIdx.x;
+  const double* M = sh.A;
   double mn = 1.0e+51, mn_old = 1.0e+52, error = 1.0e+10;
   int k = 0, kk = 0;
-  for (int i = tid; i < n; i += BT) { sh.lq[i] = 0.0; sh.lr[i] = 1.0; }
+  for (int i = tid; i < n; i += BT) {
+    sh.lq[i] = 0.0;
+    sh.lr[i] = 1.0;
+  }
   double beta = sqrt((double)n);
   const double sqn = sqrt((double)n);
   while (k < n && k < sqn + 10 && k < kLz && beta > 1.0e-16 &&
@@ -371,12 +613,21 @@ __device__ double min_eig(const double* M, int n, Shared<BT>& sh) {
           fabs(error * beta) > (1.0e-2) * fabs(mn) + (1.0e-4))) {
     const double inv = 1.0 / beta;
     __syncthreads();
-    for (int i = tid; i < n; i += BT) { sh.lqo[i] = sh.lq[i]; sh.lq[i] = sh.lr[i] * inv; }
+    for (int i = tid; i < n; i += BT) {
+      sh.lqo[i] = sh.lq[i];
+      sh.lq[i] = sh.lr[i] * inv;
+    }
     __syncthreads();
     for (int i = tid; i < n; i += BT) {
-      double a = 0.0;
-      for (int j = 0; j < n; ++j) a += M[i + n * j] * sh.lq[j];
-      sh.lw[i] = a;
+      double a0 = 0.0, a1 = 0.0;
+      int j = 0;
+#pragma unroll 4
+      for (; j + 1 < n; j += 2) {
+        a0 += M[i + LDA * j] * sh.lq[j];
+        a1 += M[i + LDA * (j + 1)] * sh.lq[j + 1];
+      }
+      if (j < n) a0 += M[i + LDA * j] * sh.lq[j];
+      sh.lw[i] = a0 + a1;
     }
     __syncthreads();
     double a = 0.0;
@@ -389,7 +640,10 @@ __device__ double min_eig(const double* M, int n, Shared<BT>& sh) {
       __syncthreads();
       double th = 0.0, zl = 0.0;
       tridiag_min_block(k + 1, sh.dg, sh.od, th, zl, BT, sh.trisc);
-      if (tid == 0) { sh.sc[0] = th; sh.sc[1] = zl; }
+      if (tid == 0) {
+        sh.sc[0] = th;
+        sh.sc[1] = zl;
+      }
       __syncthreads();
       mn_old = mn;
       mn = sh.sc[0];
@@ -406,26 +660,46 @@ __device__ double min_eig(const double* M, int n, Shared<BT>& sh) {
   return mn - fabs(error * beta);
 }
 
-// invChol (LXi, LZi) and Zi from X, Z; 0 or 1 (a factorization failed)
+// minimum eigenvalue of L D L^T (StepLength::MehrotraCorrector's
+// Jal::getMinEigen(invChol, D)); T is global scratch
 template <int BT>
-__device__ int inv_chol(int n, const double* X, const double* Z, double* LXi, double* LZi,
+__device__ __forceinline__ double min_eig(int n, const double* L, const double* D, double* T, Shared<BT>& sh) {
+  gemm(n, D, 0, L, 1, T, sh);        // T = D L^T
+  gemm(n, L, 0, T, 0, nullptr, sh);  // sh.A = L T
+  return n <= 20 ? small_min_eig(n, sh) : lanczos(n, sh);
+}
+
+// invChol of X and Z (LXi, LZi, both lower) and Zi = LZi^T LZi: the two
+// factorizations run on waves 0 and 1 at once.  0, or 1 if one failed.
+template <int BT>
+__device__ __forceinline__ int inv_chol(int n, const double* X, const double* Z, double* LXi, double* LZi,
                         double* Zi, Shared<BT>& sh) {
-  for (int w = 0; w < 2; ++w) {
-    const double* S = w ? Z : X;
-    for (int e = threadIdx.x; e < n * n; e += BT) sh.A[(e % n) + LDA * (e / n)] = S[e];
-    __syncthreads();
-    if (chol(sh.A, LDA, n, 0, sh)) return 1;
-    for (int e = threadIdx.x; e < n * n; e += BT) {  // zero the upper part
-      const int i = e % n, j = e / n;
-      if (i < j) sh.A[i + LDA * j] = 0.0;
-    }
-    __syncthreads();
-    trinv<BT>(sh.A, sh.B2, n);
-    double* out = w ? LZi : LXi;
-    for (int e = threadIdx.x; e < n * n; e += BT) out[e] = sh.B2[(e % n) + LDA * (e / n)];
-    __syncthreads();
+  const int tid = threadIdx.x, w = tid >> 6;
+  for (int e = tid; e < n * n; e += BT) {
+    const int i = e % n, j = e / n;
+    sh.A[i + LDA * j] = X[e];
+    sh.B2[i + LDA * j] = Z[e];
   }
-  gemm(n, LZi, 1, LZi, 0, Zi, sh);  // Zi = L^-T L^-1
+  __syncthreads();
+  if (w < 2) {
+    const int r = chol_inv_wave(n, w ? sh.B2 : sh.A);
+    if ((tid & 63) == 0) sh.info[w] = r;
+  }
+  __syncthreads();
+  if (sh.info[0] || sh.info[1]) return 1;
+  for (int e = tid; e < NMAX * NMAX; e += BT) {
+    const int i = e & 63, j = e >> 6;
+    const bool in = i < n && j < n;
+    const double vx = (in && i >= j) ? sh.A[i + LDA * j] : 0.0;
+    const double vz = (in && i >= j) ? sh.B2[i + LDA * j] : 0.0;
+    if (in) {
+      LXi[i + n * j] = vx;
+      LZi[i + n * j] = vz;
+    }
+    sh.B2[i + LDA * j] = vz;
+  }
+  __syncthreads();
+  mma_lds(n, sh.B2, 1, sh.B2, 0, Zi, sh);  // Zi = LZi^T LZi
   return 0;
 }
 
@@ -433,17 +707,16 @@ template <int BT>
 __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
   __shared__ Shared<BT> sh;
   const int tid = threadIdx.x;
-  for (int k = blockIdx.x; k < Dt.count; k += gridDim.x) {
-    Prob P;
-    P.n = Dt.n[k];
-    P.m = Dt.m[k];
-    P.C = Dt.C + Dt.coff[k];
-    P.b = Dt.b + Dt.boff[k];
-    P.cp = Dt.cptr + Dt.pptr[k];
-    P.er = Dt.er;
-    P.ec = Dt.ec;
-    P.ev = Dt.ev;
-    const int n = P.n, m = P.m, nn = n * n;
+  for (;;) {
+    if (tid == 0) sh.task = (int)atomicAdd(Dt.next, 1u);
+    __syncthreads();
+    const int slot = __builtin_amdgcn_readfirstlane(sh.task);
+    __syncthreads();
+    if (slot >= Dt.count) break;
+    const int k = Dt.order[slot];
+    const int n = Dt.n[k], m = Dt.m[k], nn = n * n;
+    const double* Cm = Dt.C + Dt.coff[k];
+    const int ne = stage(Dt, k, n, m, sh);
     double* wk = Dt.work + Dt.woff[k];
     double* X = wk;
     double* Z = wk + nn;
@@ -468,22 +741,20 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
       LZi[e] = d ? 1.0 / sqrt(lam) : 0.0;
       Zi[e] = d ? 1.0 / lam : 0.0;
     }
-    for (int i = tid; i < m; i += BT) { sh.y[i] = 0.0; sh.bv[i] = P.b[i]; }
     __syncthreads();
     const double muInit = lam * lam;
     double muCur = muInit;
-    // residuals: p = b - A(X), D = C - sum y A - Z
+    // residuals: p = b - A(X), D = C - sum y A - Z, and their max norms
     auto residuals = [&](double& nP, double& nD) {
-      adot<BT>(P, X, sh.p);
+      adot(n, m, X, sh.p, sh);
       for (int i = tid; i < m; i += BT) sh.p[i] = sh.bv[i] - sh.p[i];
-      __syncthreads();
-      sum_yA<BT>(P, sh.y, Dm);
-      axpby<BT>(Dm, -1.0, Dm, 1.0, P.C, -1.0, Z, nn);
+      yA_comb(n, ne, sh.y, -1.0, Cm, 1.0, Z, -1.0, Dm, sh);
       double a = 0.0, c = 0.0;
       for (int i = tid; i < m; i += BT) a = fmax(a, fabs(sh.p[i]));
       for (int e = tid; e < nn; e += BT) c = fmax(c, fabs(Dm[e]));
-      nP = bmax(a, sh);
-      nD = bmax(c, sh);
+      bmax2(a, c, sh);
+      nP = a;
+      nD = c;
     };
     double normP, normD;
     residuals(normP, normD);
@@ -493,7 +764,15 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
     double thetaD = (initNormD < Pm.epsilonDash) ? 0.0 : 1.0;
     double rho = 1.0;
     double etaP = Pm.omegaStar * nDim * muInit, etaD = etaP;
-    double objP = dot(P.C, X, nn, sh), objD = dot(sh.bv, sh.y, m, sh);
+    double objP, objD;
+    {
+      double o[2] = {0.0, 0.0};
+      for (int e = tid; e < nn; e += BT) o[0] += Cm[e] * X[e];
+      for (int i = tid; i < m; i += BT) o[1] += sh.bv[i] * sh.y[i];
+      bsums<BT, 2>(o, sh);
+      objP = o[0];
+      objD = o[1];
+    }
     int phase = noINFO, failure = 0;
     double alphaP = 0.0, alphaD = 0.0;
     // Phase::updateCheck (sdpa_parts.cpp:1150-1242)
@@ -507,22 +786,40 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
         const double mean = (fabs(objP) + fabs(objD)) / 2.0;
         const double gap = fabs(objP - objD);
         const double dom = mean < 1.0 ? 1.0 : mean;
-        if (gap / dom <= Pm.epsilonStar) { phase = pdOPT; return false; }
+        if (gap / dom <= Pm.epsilonStar) {
+          phase = pdOPT;
+          return false;
+        }
       }
-      if (v == noINFO && rho > 1.0 + NONZERO) { phase = pdINF; return false; }
+      if (v == noINFO && rho > 1.0 + NONZERO) {
+        phase = pdINF;
+        return false;
+      }
       if (v == pFEAS) {
-        if (objP <= -Pm.upperBound) { phase = pUNBD; return false; }
-        if (rho > 1.0 + NONZERO) { phase = pFEAS_dINF; return false; }
+        if (objP <= -Pm.upperBound) {
+          phase = pUNBD;
+          return false;
+        }
+        if (rho > 1.0 + NONZERO) {
+          phase = pFEAS_dINF;
+          return false;
+        }
       }
       if (v == dFEAS) {
-        if (objD >= -Pm.lowerBound) { phase = dUNBD; return false; }
-        if (rho > 1.0 + NONZERO) { phase = pINF_dFEAS; return false; }
+        if (objD >= -Pm.lowerBound) {
+          phase = dUNBD;
+          return false;
+        }
+        if (rho > 1.0 + NONZERO) {
+          phase = pINF_dFEAS;
+          return false;
+        }
       }
       return true;
     };
     update_phase();
-    // r_zinv = target Zinv - X (- Dx Dz Zinv), minus X D Zinv when infeasible;
-    // g = -A(.) + p   (Newton::Make_gVec)
+    // r_zinv = target Zinv - X (- Dx Dz Zinv), minus X D Zinv when
+    // infeasible; g = -A(.) + p   (Newton::Make_gVec)
     auto make_gvec = [&](int corrector, double target, int infeasible) {
       axpby<BT>(R, target, Zi, -1.0, X, 0.0, nullptr, nn);
       if (corrector) {
@@ -539,26 +836,44 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
         axpby<BT>(W, 1.0, R, -1.0, XDZi, 0.0, nullptr, nn);
         Dd = W;
       }
-      adot<BT>(P, Dd, sh.g);
+      adot(n, m, Dd, sh.g, sh);
       for (int i = tid; i < m; i += BT) sh.g[i] = -sh.g[i] + sh.p[i];
       __syncthreads();
     };
     // Dz = -sum dy A (+ D); Dx = sym(r_zinv - X Dz Zinv)
     auto make_direction = [&](int infeasible) {
-      sum_yA<BT>(P, sh.dy, Dz);
-      axpby<BT>(Dz, -1.0, Dz, infeasible ? 1.0 : 0.0, infeasible ? Dm : nullptr, 0.0, nullptr, nn);
+      yA_comb(n, ne, sh.dy, -1.0, infeasible ? Dm : nullptr, 1.0, nullptr, 0.0, Dz, sh);
       gemm(n, X, 0, Dz, 0, T, sh);
       gemm(n, T, 0, Zi, 0, W, sh);
-      axpby<BT>(Dx, 1.0, R, -1.0, W, 0.0, nullptr, nn);
       for (int e = tid; e < nn; e += BT) {
         const int i = e % n, j = e / n;
         if (i > j) {
-          const double s = (Dx[i + n * j] + Dx[j + n * i]) * 0.5;
+          const double s = ((R[i + n * j] - W[i + n * j]) + (R[j + n * i] - W[j + n * i])) * 0.5;
           Dx[i + n * j] = s;
           Dx[j + n * i] = s;
+        } else if (i == j) {
+          Dx[e] = R[e] - W[e];
         }
       }
       __syncthreads();
+    };
+    // q = C.Dx, b.dy, X.Dz, Dx.Z, Dx.Dz, b.y, C.X (the last two when all)
+    auto direction_dots = [&](double (&q)[7], bool all) {
+      double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int e = tid; e < nn; e += BT) {
+        const double dx = Dx[e], dz = Dz[e];
+        v[0] += Cm[e] * dx;
+        v[2] += X[e] * dz;
+        v[3] += dx * Z[e];
+        v[4] += dx * dz;
+        if (all) v[6] += Cm[e] * X[e];
+      }
+      for (int i = tid; i < m; i += BT) {
+        v[1] += sh.bv[i] * sh.dy[i];
+        if (all) v[5] += sh.bv[i] * sh.y[i];
+      }
+      bsums<BT, 7>(v, sh);
+      for (int t = 0; t < 7; ++t) q[t] = v[t];
     };
     int it = 0;
     while (update_phase() && it < Pm.maxIteration) {
@@ -568,18 +883,17 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
       const int infeasible = (phase == pFEAS || phase == noINFO) ? 1 : 0;
       // Newton::Mehrotra(PREDICTOR)
       make_gvec(0, betaVal * muCur, infeasible);
-      make_bmat(P, X, Zi, sh);
-      if (chol(sh.S, LDB, m, 1, sh)) { failure = 1; break; }
+      make_bmat(n, m, X, Zi, sh);
+      if (chol_schur(m, sh)) {
+        failure = 1;
+        break;
+      }
       for (int i = tid; i < m; i += BT) sh.dy[i] = sh.g[i];
       __syncthreads();
       schur_solve(m, sh.dy, sh);
       make_direction(infeasible);
       double q[7];
-      q[0] = dot(P.C, Dx, nn, sh);
-      q[1] = dot(sh.bv, sh.dy, m, sh);
-      q[2] = dot(X, Dz, nn, sh);
-      q[3] = dot(Dx, Z, nn, sh);
-      q[4] = dot(Dx, Dz, nn, sh);
+      direction_dots(q, false);
       alphaP = alphaD = 0.9;
       if (phase == noINFO || phase == dFEAS) {
         if (alphaP > 1.0) alphaP = 1.0;
@@ -612,24 +926,14 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
       schur_solve(m, sh.dy, sh);
       make_direction(infeasible);
       // StepLength::MehrotraCorrector
-      gemm(n, Dx, 0, LXi, 1, T, sh);
-      gemm(n, LXi, 0, T, 0, W, sh);
-      const double lamX = min_eig(W, n, sh);
-      gemm(n, Dz, 0, LZi, 1, T, sh);
-      gemm(n, LZi, 0, T, 0, W, sh);
-      const double lamZ = min_eig(W, n, sh);
+      const double lamX = min_eig(n, LXi, Dx, T, sh);
+      const double lamZ = min_eig(n, LZi, Dz, T, sh);
       const double alphaBD = 100.0;
       alphaP = (-lamX > 1.0 / alphaBD) ? -1.0 / lamX : alphaBD;
       alphaD = (-lamZ > 1.0 / alphaBD) ? -1.0 / lamZ : alphaBD;
       alphaP *= Pm.gammaStar;
       alphaD *= Pm.gammaStar;
-      q[0] = dot(P.C, Dx, nn, sh);
-      q[1] = dot(sh.bv, sh.dy, m, sh);
-      q[2] = dot(X, Dz, nn, sh);
-      q[3] = dot(Dx, Z, nn, sh);
-      q[4] = dot(Dx, Dz, nn, sh);
-      q[5] = dot(sh.bv, sh.y, m, sh);
-      q[6] = dot(P.C, X, nn, sh);
+      direction_dots(q, true);
       if (phase == noINFO || phase == dFEAS) {
         if (alphaP > 1.0) alphaP = 1.0;
       } else if (q[0] > 0.0) {
@@ -658,33 +962,51 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
       if (phase == pdFEAS) {
         const double incD = q[1] * alphaD, incP = q[0] * alphaP;
         const double maxRatio = (q[5] - q[6]) / (incP - incD);
-        if (maxRatio > 0.0 && maxRatio < 1.0) { alphaP *= maxRatio; alphaD *= maxRatio; }
+        if (maxRatio > 0.0 && maxRatio < 1.0) {
+          alphaP *= maxRatio;
+          alphaD *= maxRatio;
+        }
       }
       // Solutions::update (sdpa_dataset.cpp:145-171)
-      axpby<BT>(X, 1.0, X, alphaP, Dx, 0.0, nullptr, nn);
+      for (int e = tid; e < nn; e += BT) {
+        X[e] += alphaP * Dx[e];
+        Z[e] += alphaD * Dz[e];
+      }
       for (int i = tid; i < m; i += BT) sh.y[i] += alphaD * sh.dy[i];
-      axpby<BT>(Z, 1.0, Z, alphaD, Dz, 0.0, nullptr, nn);
-      if (alphaP < 1.0e-4 && alphaD < 1.0e-4) { failure = 2; break; }
-      if (inv_chol(n, X, Z, LXi, LZi, Zi, sh)) { failure = 3; break; }
+      __syncthreads();
+      if (alphaP < 1.0e-4 && alphaD < 1.0e-4) {
+        failure = 2;
+        break;
+      }
+      if (inv_chol(n, X, Z, LXi, LZi, Zi, sh)) {
+        failure = 3;
+        break;
+      }
       if (!affine) {
         thetaP = fabs((1.0 - alphaP) * thetaP);
         thetaD = fabs((1.0 - alphaD) * thetaD);
       }
-      double trZ = 0.0, trX = 0.0;
-      for (int i = tid; i < n; i += BT) { trZ += Z[i + n * i]; trX += X[i + n * i]; }
-      trZ = bsum(trZ, sh);
-      trX = bsum(trX, sh);
-      const double xz = dot(X, Z, nn, sh);
-      objP = dot(P.C, X, nn, sh);
-      objD = dot(sh.bv, sh.y, m, sh);
-      muCur = xz / nDim;
+      double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // trZ, trX, X.Z, C.X, b.y
+      for (int i = tid; i < n; i += BT) {
+        o[0] += Z[i + n * i];
+        o[1] += X[i + n * i];
+      }
+      for (int e = tid; e < nn; e += BT) {
+        o[2] += X[e] * Z[e];
+        o[3] += Cm[e] * X[e];
+      }
+      for (int i = tid; i < m; i += BT) o[4] += sh.bv[i] * sh.y[i];
+      bsums<BT, 5>(o, sh);
+      objP = o[3];
+      objD = o[4];
+      muCur = o[2] / nDim;
       residuals(normP, normD);
       thetaP = (initNormP > Pm.epsilonDash * 1.0e-2) ? normP / initNormP : 0.0;
       thetaD = (initNormD > Pm.epsilonDash * 1.0e-2) ? normD / initNormD : 0.0;
       // SolveInfo::update (sdpa_parts.cpp:914-1060)
       rho = 0.0;
       const double x0z0 = nDim * muInit, xzn = nDim * muCur;
-      const double x0z = lam * trZ, xz0 = lam * trX;
+      const double x0z = lam * o[0], xz0 = lam * o[1];
       const double acc = Pm.epsilonDash, omega = Pm.omegaStar;
       if (normP <= acc && xz0 < etaP) etaP = xz0;
       if (normD <= acc && x0z < etaD) etaD = x0z;
@@ -719,11 +1041,6 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
 
 using namespace sdpa_amd;
 
-#define BCHK(x)                          \
-  do {                                   \
-    if ((x) != hipSuccess) return -100;  \
-  } while (0)
-
 extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* m,
                                 const long* coff, const double* C, long nC, const long* boff,
                                 const double* b, long nb, const long* pptr, const int* cptr,
@@ -745,20 +1062,32 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
       const int e = cptr[pptr[k] + i];
       if (e < 0 || e > nent || (i > 0 && e < cptr[pptr[k] + i - 1])) return -5;
     }
+    if (cptr[pptr[k] + m[k]] - cptr[pptr[k]] > kEnt) return -7;
+    if (!(lambda[k] > 0.0)) return -8;
     woff[k] = wtot;
     wtot += (long)kMats * n[k] * n[k];
   }
-  for (long e = 0; e < nent; ++e)
-    if (er[e] < 0 || ec[e] < er[e]) return -6;
   for (int k = 0; k < count; ++k)
     for (int e = cptr[pptr[k]]; e < cptr[pptr[k] + m[k]]; ++e)
-      if (ec[e] >= n[k]) return -6;
-  BCHK(hipSetDevice(device));
+      if (er[e] < 0 || ec[e] < er[e] || ec[e] >= n[k]) return -6;
+  // largest instances first (work ~ m^3 / 3 + n^3 per iteration): the
+  // queue then ends on small ones and the work-groups finish together
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    const double wa = (double)m[a] * m[a] * m[a] / 3.0 + (double)n[a] * n[a] * n[a];
+    const double wc = (double)m[c] * m[c] * m[c] / 3.0 + (double)n[c] * n[c] * n[c];
+    return wa > wc;
+  });
+  if (hipSetDevice(device) != hipSuccess) return -100;
   hipStream_t s;
-  BCHK(hipStreamCreate(&s));
+  if (hipStreamCreate(&s) != hipSuccess) return -100;
   int *dn = nullptr, *dm = nullptr, *dcp = nullptr, *der = nullptr, *dec = nullptr;
+  int* dord = nullptr;
+  unsigned* dnext = nullptr;
   long *dco = nullptr, *dbo = nullptr, *dpp = nullptr, *dwo = nullptr;
   double *dC = nullptr, *db = nullptr, *dev = nullptr, *dl = nullptr, *dw = nullptr, *dout = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
   auto up = [&](void** dst, const void* src, size_t bytes) -> int {
     if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return -100;
@@ -780,9 +1109,22 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
     if ((rc = up((void**)&dev, ev, sizeof(double) * nent))) break;
     if ((rc = up((void**)&dl, lambda, sizeof(double) * count))) break;
     if ((rc = up((void**)&dwo, woff.data(), sizeof(long) * count))) break;
-    if (hipMalloc(&dw, sizeof(double) * (wtot ? wtot : 1)) != hipSuccess) { rc = -100; break; }
-    if (hipMalloc(&dout, sizeof(double) * 8 * count) != hipSuccess) { rc = -100; break; }
-    BData D{count, dn, dm, dco, dC, dbo, db, dpp, dcp, der, dec, dev, dl, dwo, dw, dout};
+    if ((rc = up((void**)&dord, order.data(), sizeof(int) * count))) break;
+    if (hipMalloc(&dnext, sizeof(unsigned)) != hipSuccess ||
+        hipMemsetAsync(dnext, 0, sizeof(unsigned), s) != hipSuccess) {
+      rc = -100;
+      break;
+    }
+    if (hipMalloc(&dw, sizeof(double) * (wtot ? wtot : 1)) != hipSuccess) {
+      rc = -100;
+      break;
+    }
+    if (hipMalloc(&dout, sizeof(double) * 8 * count) != hipSuccess) {
+      rc = -100;
+      break;
+    }
+    BData D{count, dn, dm, dco, dC, dbo, db, dpp, dcp, der, dec, dev, dl, dwo, dord, dnext, dw,
+            dout};
     BParams P{(int)param[0], param[1], param[2], param[3], param[4], param[5], param[6], param[7],
               param[8]};
     int ncu = 0;
@@ -790,11 +1132,13 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
       rc = -100;
       break;
     }
-    // persistent work-groups: one per CU (LDS of ~110 KB allows one), each
-    // looping over instances
+    // persistent work-groups, one per CU (the LDS use allows one), taking
+    // instances from the queue until it is empty
     const int grid = count < ncu ? count : ncu;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { rc = -100; break; }
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      rc = -100;
+      break;
+    }
     (void)hipEventRecord(e0, s);
     if (threads == 256)
       hipLaunchKernelGGL(k_batch_solve<256>, dim3(grid), dim3(256), 0, s, D, P);
@@ -808,14 +1152,14 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     if (kernel_ms) *kernel_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (hipMemcpy(out, dout, sizeof(double) * 8 * count, hipMemcpyDeviceToHost) != hipSuccess)
       rc = -100;
   } while (0);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   for (void* p : {(void*)dn, (void*)dm, (void*)dcp, (void*)der, (void*)dec, (void*)dco, (void*)dbo,
                   (void*)dpp, (void*)dwo, (void*)dC, (void*)db, (void*)dev, (void*)dl, (void*)dw,
-                  (void*)dout})
+                  (void*)dout, (void*)dord, (void*)dnext})
     if (p) (void)hipFree(p);
   (void)hipStreamDestroy(s);
   return rc;

@@ -9,9 +9,50 @@
 
 namespace sdpa_amd {
 
+// lane l's value of v (l the same in every lane): two v_readlane, no LDS
+// round trip as a general shuffle takes
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// v from another lane of the same row of 16 by a DPP pattern (both halves)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Reductions over a full wave (all 64 lanes active), the same value in every
+// lane: DPP within rows of 16 (quad swaps [1,0,3,2] and [2,3,0,1], half-row
+// mirror, row mirror), then the four row results by readlane -- no LDS
+// round trips, as a shuffle butterfly takes.
 __device__ __forceinline__ double wsum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
+}
+
+__device__ __forceinline__ double wmin(double v) {
+  v = fmin(v, dpp_mov<0xB1>(v));
+  v = fmin(v, dpp_mov<0x4E>(v));
+  v = fmin(v, dpp_mov<0x141>(v));
+  v = fmin(v, dpp_mov<0x140>(v));
+  return fmin(fmin(rdlane(v, 0), rdlane(v, 16)), fmin(rdlane(v, 32), rdlane(v, 48)));
+}
+
+__device__ __forceinline__ double wmax(double v) {
+  v = fmax(v, dpp_mov<0xB1>(v));
+  v = fmax(v, dpp_mov<0x4E>(v));
+  v = fmax(v, dpp_mov<0x141>(v));
+  v = fmax(v, dpp_mov<0x140>(v));
+  return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
 }
 
 // 1/x to full FP64 accuracy without the IEEE divide sequence: the hardware
@@ -34,9 +75,15 @@ __device__ __forceinline__ double recip(double x) {
 // the smallest eigenvalue shrinks nthreads+1-fold per round); then wave 0 runs
 // two steps of inverse iteration from a shift just below theta, where
 // T - sigma I is positive definite and its LDL^T needs no pivoting.
+// The multisection stops at a relative width of 2 eps, or at an absolute
+// width of 1e-3 eps ||T|| (far below the eps ||T|| error of a backward-stable
+// QR step, and reached in about 8 rounds when theta is near zero).
 // scratch: 2 * (nthreads / 64) doubles of LDS.  Call from all threads.
-static __device__ void tridiag_min_block(int N, const double* d, const double* e, double& theta,
-                                  double& zlast, int nthreads, double* scratch) {
+// want_z = false skips the inverse iteration (zlast is then 0).
+static __device__ __forceinline__ void tridiag_min_block(int N, const double* d, const double* e,
+                                                         double& theta, double& zlast,
+                                                         int nthreads, double* scratch,
+                                                         bool want_z = true) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = nthreads >> 6;
   const double eps = 2.220446049250313e-16;
   double lo = 1.0e300, hi = -1.0e300;
@@ -45,10 +92,8 @@ static __device__ void tridiag_min_block(int N, const double* d, const double* e
     lo = fmin(lo, d[i] - r);
     hi = fmax(hi, d[i] + r);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = fmin(lo, __shfl_xor(lo, o, 64));
-    hi = fmax(hi, __shfl_xor(hi, o, 64));
-  }
+  lo = wmin(lo);
+  hi = wmax(hi);
   const double tnorm = fmax(fabs(lo), fabs(hi));
   const double pivmin = 1.0e-290 + eps * eps * tnorm * tnorm;
   lo -= 4.0 * eps * tnorm + pivmin;
@@ -62,6 +107,7 @@ static __device__ void tridiag_min_block(int N, const double* d, const double* e
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
+#pragma unroll 4
     for (int i = 1; i < N; ++i) {
       q = (d[i] - x) - (e[i - 1] * e[i - 1]) * recip(q);
       if (fabs(q) < pivmin) q = -pivmin;
@@ -79,9 +125,11 @@ static __device__ void tridiag_min_block(int N, const double* d, const double* e
     const double nhi = (first < nthreads) ? lo + step * (double)(first + 1) : hi;
     lo = nlo;
     hi = nhi;
-    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    if (hi - lo <= fmax(2.0 * eps * fmax(fabs(lo), fabs(hi)), 1.0e-3 * eps * tnorm) + pivmin) break;
   }
   theta = 0.5 * (lo + hi);
+  zlast = 0.0;
+  if (!want_z) return;
   // inverse iteration on wave 0: LDL^T of T - sigma I (pivots piv_i), two
   // solves from b = ones with a normalization between them.  The sequential
   // sweeps run on every lane with the same scalars; lane i keeps entry i.
@@ -104,7 +152,7 @@ static __device__ void tridiag_min_block(int N, const double* d, const double* e
     // backward: z_i = (y_i - e_i z_{i+1}) / piv_i
     double znext = 0.0;
     for (int i = N - 1; i >= 0; --i) {
-      const double ri = __shfl(rpiv, i, 64), yi = __shfl(y, i, 64);
+      const double ri = rdlane(rpiv, i), yi = rdlane(y, i);
       const double ei = (i < N - 1) ? e[i] : 0.0;
       const double zi2 = (yi - ei * znext) * ri;
       if (lane == i) zi = zi2;
@@ -112,36 +160,36 @@ static __device__ void tridiag_min_block(int N, const double* d, const double* e
     }
     // normalize and second solve with b = z
     double s2 = (lane < N) ? zi * zi : 0.0;
-    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o, 64);
+    s2 = wsum(s2);
     const double b = zi / sqrt(s2);
     rprev = 1.0;
     yprev = 0.0;
     for (int i = 0; i < N; ++i) {
       const double ei = (i > 0) ? e[i - 1] : 0.0;
       const double l = (i > 0) ? ei * rprev : 0.0;
-      const double yi = __shfl(b, i, 64) - l * yprev;
+      const double yi = rdlane(b, i) - l * yprev;
       if (lane == i) y = yi;
-      rprev = __shfl(rpiv, i, 64);
+      rprev = rdlane(rpiv, i);
       yprev = yi;
     }
     znext = 0.0;
     for (int i = N - 1; i >= 0; --i) {
-      const double ri = __shfl(rpiv, i, 64), yi = __shfl(y, i, 64);
+      const double ri = rdlane(rpiv, i), yi = rdlane(y, i);
       const double ei = (i < N - 1) ? e[i] : 0.0;
       const double zi2 = (yi - ei * znext) * ri;
       if (lane == i) zi = zi2;
       znext = zi2;
     }
     double nrm2 = (lane < N) ? zi * zi : 0.0;
-    for (int o = 32; o > 0; o >>= 1) nrm2 += __shfl_xor(nrm2, o, 64);
-    const double last = __shfl(zi, N - 1, 64);
+    nrm2 = wsum(nrm2);
+    const double last = rdlane(zi, N - 1);
     zlast = last / sqrt(nrm2);
   }
 }
 
 // Smallest eigenvalue of a symmetric n x n (n <= 64) matrix in LDS (ld 65),
 // cyclic Jacobi; lane k updates row/column k.
-static __device__ double jacobi_min_wave(int n, double* A) {
+static __device__ __forceinline__ double jacobi_min_wave(int n, double* A) {
   const int lane = threadIdx.x;
   for (int sweep = 0; sweep < 100; ++sweep) {
     double off = 0.0, dg = 0.0;
@@ -180,7 +228,7 @@ static __device__ double jacobi_min_wave(int n, double* A) {
   }
   __syncthreads();
   double dv = lane < n ? A[lane + 65 * lane] : 1.0e300;
-  for (int o = 32; o > 0; o >>= 1) dv = fmin(dv, __shfl_xor(dv, o, 64));
+  dv = wmin(dv);
   return dv;
 }
 

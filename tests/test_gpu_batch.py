@@ -25,11 +25,15 @@ B = pytest.importorskip("sdpa_amd.batch")
 
 
 def unbounded():
-    # the user dual asks F_1.Y = 1 and F_2.Y = 2 with F_1 = F_2 = E11
-    recs = [(0, 1, 1, 1, 1.0), (0, 1, 2, 2, 1.0), (0, 1, 3, 3, 1.0), (1, 1, 1, 1, 1.0),
-            (2, 1, 1, 1, 1.0), (3, 1, 2, 2, 1.0), (3, 1, 3, 3, 1.0), (4, 1, 2, 3, 1.0)]
+    # F_1 = -I with c_1 = 1: the user dual asks tr(Y) = -1 (infeasible) and
+    # the primal objective x_1 decreases without bound.  (Dependent
+    # constraints, F_i = F_j, are left out: there the Schur pivot cancels to
+    # rounding level and whether rATL_dpotrfL's adjustment fires -- bounded
+    # step or not -- depends on the last bits of the iterate.)
+    recs = [(0, 1, 1, 1, 1.0), (0, 1, 2, 2, 2.0), (0, 1, 3, 3, 3.0), (1, 1, 1, 1, -1.0),
+            (1, 1, 2, 2, -1.0), (1, 1, 3, 3, -1.0), (2, 1, 1, 2, 1.0), (3, 1, 2, 3, 1.0)]
     K, L, I, J, V = PR._records(recs)
-    return PR._make(4, [3], [1, 2, 1, 0], K, L, I, J, V, "unbounded_n3_m4")
+    return PR._make(3, [3], [1, 0, 0], K, L, I, J, V, "unbounded_n3_m3")
 
 
 def scaled_maxcut():
