@@ -43,6 +43,7 @@ constexpr int LDA = NMAX + 1;   // LDS leading dimension of block matrices
 constexpr int LDB = MMAX + 1;   // of the Schur matrix
 constexpr int kMats = 12;       // n x n work matrices per instance
 constexpr int kLz = 64;         // Lanczos tridiagonal limit (sqrt(64) + 10 steps)
+constexpr int kPadCols = 16;    // scratch columns behind A and B2 (straight-line loops)
 
 enum { noINFO, pFEAS, dFEAS, pdFEAS, pdINF, pFEAS_dINF, pINF_dFEAS, pdOPT, pUNBD, dUNBD };
 
@@ -74,8 +75,8 @@ struct BData {
 
 template <int BT>
 struct Shared {
-  double A[NMAX * LDA];   // GEMM operand / X factor / step-length matrix
-  double B2[NMAX * LDA];  // GEMM operand / Z factor
+  double A[(NMAX + kPadCols) * LDA];   // GEMM operand / X factor / step-length matrix
+  double B2[(NMAX + kPadCols) * LDA];  // GEMM operand / Z factor
   double S[MMAX * LDB];   // Schur matrix and its factor (lower)
   double ev[kEnt];        // constraint entries: value, row, column, constraint
   double y[MMAX], dy[MMAX], g[MMAX], p[MMAX], bv[MMAX], piv[MMAX];
@@ -83,6 +84,7 @@ struct Shared {
   double red[8 * (BT / 64)];
   double trisc[2 * (BT / 64)];
   double sc[4];
+  double pv[2][NMAX];         // pivots of the X and Z factorizations
   unsigned short perm[kEnt];  // entries in position-major order
   unsigned short cp[MMAX + 1];
   unsigned char er[kEnt], ec[kEnt], con[kEnt], head[kEnt];
@@ -352,9 +354,12 @@ __device__ __forceinline__ void gemm(int n, const double* A, int ta, const doubl
 
 // Cholesky of the Schur matrix in sh.S with rATL_dpotrfL's pivot rule
 // (sdpa_dpotrf.cpp:218-265: a pivot below -1e-6 fails, one below 1e-14
-// becomes 1e100) and its operations (L21 = A21 / L11, A22 -= L21 L21^T):
-// right-looking, the column scaled before the trailing update.  On success
-// S holds L (lower) and sh.piv[k] = 1 / L_kk.  Returns 0 or k + 1.
+// becomes 1e100).  Right-looking with one barrier per column: column k stays
+// unscaled until the end and each update forms the scaled values it needs,
+// S_ij -= (S_ik / L_kk)(S_jk / L_kk), the division as a multiplication by
+// 1 / L_kk as the reference BLAS dtrsm does.  Four elements per thread per
+// pass, their loads issued together.  On success S holds L (lower) and
+// sh.piv[k] = 1 / L_kk.  Returns 0 or k + 1.
 template <int BT>
 __device__ __forceinline__ int chol_schur(int m, Shared<BT>& sh) {
   double* S = sh.S;
@@ -362,21 +367,36 @@ __device__ __forceinline__ int chol_schur(int m, Shared<BT>& sh) {
     double d = uni(S[k + LDB * k]);
     if (d < -1.0e-6) return k + 1;
     if (d < 1.0e-14) d = 1.0e100;
-    const double s = sqrt(d);
-    for (int i = k + 1 + threadIdx.x; i < m; i += BT) S[i + LDB * k] /= s;
-    if (threadIdx.x == 0) {
-      S[k + LDB * k] = s;
-      sh.piv[k] = 1.0 / s;
-    }
-    __syncthreads();
-    const int rem = m - k - 1;
-    for (int e = threadIdx.x; e < rem * rem; e += BT) {
-      const int i = k + 1 + e % rem, j = k + 1 + e / rem;
-      // the upper triangle (not read again) takes a zero update: no branch
-      S[i + LDB * j] -= (i >= j) ? S[i + LDB * k] * S[j + LDB * k] : 0.0;
+    const double s = sqrt(d), rs = 1.0 / s;
+    if (threadIdx.x == 0) sh.piv[k] = s;
+    const int rem = m - k - 1, tot = rem * rem;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * BT) {
+      double sij[4], lik[4], ljk[4];
+      int idx[4];
+      bool low[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(e0 + u * BT, tot - 1);
+        const int i = k + 1 + e % rem, j = k + 1 + e / rem;
+        idx[u] = i + LDB * j;
+        low[u] = i >= j;
+        sij[u] = S[idx[u]];
+        lik[u] = S[i + LDB * k];
+        ljk[u] = S[j + LDB * k];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e0 + u * BT < tot && low[u]) S[idx[u]] = sij[u] - (lik[u] * rs) * (ljk[u] * rs);
     }
     __syncthreads();
   }
+  for (int k = threadIdx.x; k < m; k += BT) {
+    const double s = sh.piv[k], rs = 1.0 / s;
+    S[k + LDB * k] = s;
+    for (int i = k + 1; i < m; ++i) S[i + LDB * k] *= rs;
+    sh.piv[k] = rs;
+  }
+  __syncthreads();
   return 0;
 }
 
@@ -411,44 +431,74 @@ __device__ __forceinline__ void schur_solve(int m, double* x, Shared<BT>& sh) {
   __syncthreads();
 }
 
-// One wave: M (LDS, ld LDA, symmetric n x n, lower part read) := L^-1
-// (lower) where M = L L^T, with LAPACK's rule (a pivot that is not > 0
-// fails).  Lane l owns row l; column values of the step come by readlane.
-// The inverse is dtrti2's column sweep from the right: column j of L^-1 is
-// -(L^-1)_{22} L_{21} / L_jj with the trailing inverse already in place.
-// Returns 0 or k + 1.
-__device__ __forceinline__ int chol_inv_wave(int n, double* M) {
+// M (LDS, ld LDA, symmetric n x n, lower part read) := L^-1 (lower) where
+// M = L L^T, with LAPACK's rule (a pivot that is not > 0 fails), by a group
+// of NS waves (ws = this wave's index in the group).  Every wave of the
+// work-group calls at once: the groups for X and Z pass the same barriers.
+// Lane l owns row l; wave ws updates the columns j = ws (mod NS).  Column k
+// stays unscaled during the factorization (the waves read it while others
+// update theirs) and each update forms L_lk L_jk from it, the column values
+// of the step coming by readlane; one barrier per column.  The inverse is
+// dtrti2's column sweep from the right on wave 0 of the group: column j of
+// L^-1 is -(L^-1)_{22} L_{21} / L_jj with the trailing inverse in place.
+// M must have kPadCols writable columns behind its 64.  pv: n doubles of
+// LDS for the pivots.  Returns 0 or k + 1.
+template <int NS>
+__device__ __forceinline__ int chol_inv_group(int n, double* M, double* pv, int ws) {
   const int l = threadIdx.x & 63;
+  int info = 0;
   for (int k = 0; k < n; ++k) {
     const double d = uni(M[k + LDA * k]);
-    if (!(d > 0.0)) return k + 1;
-    const double s = sqrt(d), r = 1.0 / s;
-    const bool below = l > k && l < n;
-    const double li = below ? M[l + LDA * k] * r : 0.0;
-    if (below) M[l + LDA * k] = li;
+    if (!info && !(d > 0.0)) info = k + 1;
+    const double s = info ? 1.0 : sqrt(d), r = 1.0 / s;
+    const double li = (l > k && l < n) ? M[l + LDA * k] * r : 0.0;
+    if (ws == 0 && l == k) pv[k] = s;
+    // owned columns past k, eight at a time, straight-line: the loads are
+    // in flight together, rows l < j take a zero update, and columns past
+    // 63 fall in the pad columns behind the matrix (lane l >= n holds
+    // li = 0, so columns in [n, 64) are rewritten unchanged)
+    const int jf = k + 1 + (((ws - (k + 1)) % NS) + NS) % NS;
+    for (int j0 = jf; j0 < n; j0 += 8 * NS) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = M[l + LDA * (j0 + NS * u)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = j0 + NS * u;
+        const double upd = li * rdlane(li, j & 63);
+        M[l + LDA * j] = t[u] - ((l >= j) ? upd : 0.0);
+      }
+    }
+    if (NS > 1) __syncthreads();
+  }
+  // scale the columns: L_lk = M_lk / L_kk (as the multiplication above)
+  for (int k = ws; k < n; k += NS) {
+    const double s = pv[k], r = 1.0 / s;
+    if (l > k && l < n) M[l + LDA * k] *= r;
     if (l == k) M[k + LDA * k] = s;
-    // rows l < j (and l >= n) subtract zero: no branch, the loads of
-    // successive columns can be in flight together
-#pragma unroll 4
-    for (int j = k + 1; j < n; ++j) {
-      const double lj = rdlane(li, j);
-      M[l + LDA * j] -= (l >= j) ? li * lj : 0.0;
+  }
+  if (NS > 1) __syncthreads();
+  if (ws == 0 && !info) {
+    for (int j = n - 1; j >= 0; --j) {
+      const double ajj = 1.0 / M[j + LDA * j];
+      const double x = (l > j && l < n) ? M[l + LDA * j] : 0.0;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = j + 1; k0 < n; k0 += 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = M[l + LDA * (k0 + u)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int k = k0 + u;
+          const double xk = rdlane(x, k & 63);
+          acc[u & 3] += (l >= k && k < n) ? t[u] * xk : 0.0;
+        }
+      }
+      if (l > j && l < n) M[l + LDA * j] = -ajj * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+      if (l == j) M[j + LDA * j] = ajj;
     }
   }
-  for (int j = n - 1; j >= 0; --j) {
-    const double ajj = 1.0 / M[j + LDA * j];
-    const double x = (l > j && l < n) ? M[l + LDA * j] : 0.0;
-    double acc = 0.0;
-#pragma unroll 4
-    for (int k = j + 1; k < n; ++k) {
-      const double xk = rdlane(x, k);
-      const double t = M[l + LDA * k];
-      acc += (l >= k && l < n) ? t * xk : 0.0;
-    }
-    if (l > j && l < n) M[l + LDA * j] = -ajj * acc;
-    if (l == j) M[j + LDA * j] = ajj;
-  }
-  return 0;
+  return info;
 }
 
 // calF3 with A_j the owner (outer loop), sdpa_newton.cpp:1319-1403, over
@@ -670,7 +720,8 @@ __device__ __forceinline__ double min_eig(int n, const double* L, const double* 
 }
 
 // invChol of X and Z (LXi, LZi, both lower) and Zi = LZi^T LZi: the two
-// factorizations run on waves 0 and 1 at once.  0, or 1 if one failed.
+// factorizations run at once on the two halves of the work-group.  0, or 1
+// if one failed.
 template <int BT>
 __device__ __forceinline__ int inv_chol(int n, const double* X, const double* Z, double* LXi, double* LZi,
                         double* Zi, Shared<BT>& sh) {
@@ -681,9 +732,12 @@ __device__ __forceinline__ int inv_chol(int n, const double* X, const double* Z,
     sh.B2[i + LDA * j] = Z[e];
   }
   __syncthreads();
-  if (w < 2) {
-    const int r = chol_inv_wave(n, w ? sh.B2 : sh.A);
-    if ((tid & 63) == 0) sh.info[w] = r;
+  {
+    // waves [0, NS) factor X, waves [NS, 2 NS) factor Z
+    constexpr int NS = BT / 128;
+    const int grp = w / NS, ws = w % NS;
+    const int r = chol_inv_group<NS>(n, grp ? sh.B2 : sh.A, sh.pv[grp], ws);
+    if ((tid & 63) == 0 && ws == 0) sh.info[grp] = r;
   }
   __syncthreads();
   if (sh.info[0] || sh.info[1]) return 1;
