@@ -17,7 +17,9 @@ HBM).  The JSON line also carries:
     calibration), or null when that summary is absent;
   - "cpu_baseline": the FP64 CPU oracle (OpenMP) on a bounded sample of the
     same workload (its first IPM iterations), scaled to the GPU's iteration
-    count.
+    count;
+  - "batched": 4096 small MaxCut SDPs in one persistent launch (solves/s,
+    phase counts, gap histogram, its own CPU-oracle sample).
 
 Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config lovasz|maxcut|dense|multiblock]
@@ -73,6 +75,60 @@ def cpu_baseline(d, gpu_iterations, sample_iters=2):
             "sample": f"the first {done} of {gpu_iterations} IPM iterations of the same instance "
                       f"({t:.2f} s wall incl. setup, FP64 C oracle with OpenMP), time per iteration "
                       f"x {gpu_iterations}"}
+
+
+def batched(steps, warmup, with_cpu):
+    """The batched workload: 4096 independent seeded MaxCut SDPs (n ~ U{8..64},
+    p = 0.2, a trace row on every fourth), all solved by one persistent launch
+    (sdpa_batch_solve).  Reports solves/s over the kernel time (and over the
+    whole call, which includes the upload of the packed instances and the
+    result copy), the phase counts, the iteration histogram and the gap
+    distribution; the launch shape is the faster of 256 and 128 threads per
+    work-group on the first run (a two-point autotune)."""
+    from sdpa_amd import batch as B
+    from sdpa_amd import problems as PR
+    insts = PR.maxcut_batch(4096)
+    P = B.Packed(insts)
+    tune = {}
+    for thr in (256, 128):
+        tune[thr] = B.solve(insts, packed=P, threads=thr).kernel_ms
+    thr = min(tune, key=tune.get)
+    for _ in range(warmup):
+        B.solve(insts, packed=P, threads=thr)
+    kms, wall = [], []
+    for _ in range(max(1, steps)):
+        t0 = time.perf_counter()
+        r = B.solve(insts, packed=P, threads=thr)
+        wall.append(time.perf_counter() - t0)
+        kms.append(r.kernel_ms)
+    k = float(np.mean(kms)) * 1e-3
+    w = float(np.mean(wall))
+    phases = {}
+    for ph in r.phase:
+        phases[ph] = phases.get(ph, 0) + 1
+    gaps = r.relative_gap
+    out = {"workload": "maxcut_batch_4096_n8-64_p0.2_seed4 (trace row on every fourth)",
+           "metric": "solves/s", "value": len(insts) / k, "wall_solves_per_s": len(insts) / w,
+           "kernel_ms": k * 1e3, "threads": thr, "autotune_kernel_ms": tune,
+           "phases": phases,
+           "iterations": {int(i): int(c) for i, c in enumerate(np.bincount(r.iterations)) if c},
+           "gap_histogram": {"<=1e-8": int((gaps <= 1e-8).sum()),
+                             "(1e-8,1e-7]": int(((gaps > 1e-8) & (gaps <= 1e-7)).sum()),
+                             ">1e-7": int((gaps > 1e-7).sum())},
+           "max_feasibility_error": float(max(r.primal_error.max(), r.dual_error.max()))}
+    if with_cpu:
+        # the FP64 oracle on the first 48 instances, one at a time
+        from oracle import oracle as O
+        sample = insts[:48]
+        t0 = time.perf_counter()
+        for d in sample:
+            O.Problem(d).solve()
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": len(sample) / t, "unit": "solves/s",
+                               "cores": int(O.lib().orc_num_threads()), "kind": "port",
+                               "sample": f"the first {len(sample)} of the 4096 instances solved one "
+                                         f"after another by the FP64 C oracle ({t:.2f} s)"}
+    return out
 
 
 def read_traffic():
@@ -234,6 +290,9 @@ def main():
                       ("factors", "fallbacks", "solves", "refine_steps", "stalls")},
         "phase_ms_per_iteration": {k: v / max(1, rm.iterations) for k, v in mpt.items()},
     }
+    log("bench: batched workload")
+    batch = batched(args.steps, args.warmup, not args.no_cpu_baseline)
+    log(f"bench: batched {batch['value']:.0f} solves/s ({batch['kernel_ms']:.1f} ms kernel)")
     its = rt.iterations
     m = d.m
     # F3 bytes per assembly (lower triangle written): 8*T + 16*sum(nnz) + 16*n^2
@@ -273,6 +332,7 @@ def main():
         "cholesky": {"flops": m ** 3 / 3.0, "ms": chol_ms,
                      "TFLOPs": m ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None},
         "mixed_vs_fp64": mixed,
+        "batched": batch,
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

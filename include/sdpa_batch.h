@@ -17,13 +17,20 @@
  *   b: m[k] doubles at boff[k]; constraints: entry pointers cptr at
  *   pptr[k] .. pptr[k] + m[k] (m[k] + 1 values, global entry indices), and
  *   entries er, ec (0-based, er <= ec), ev.
+ *   At most 1536 stored entries per instance.
+ * lambda: the initial-point scale of each instance (> 0).
  * param: maxIteration, epsilonStar, epsilonDash, betaStar, betaBar,
  *   gammaStar, omegaStar, lowerBound, upperBound (9 doubles).
+ * threads: 256 or 128 work-items per work-group (one work-group per CU).
+ * kernel_ms (may be null): the launch's device time.
  * out: 8 doubles per instance: phase (0 noINFO .. 9 dUNBD, internal
  *   convention), iterations, objP (= C.X), objD (= b.y), normP (max |b -
  *   A(X)|), normD (max |C - sum y A - Z|), mu, failure (0 none, 1 Schur
  *   Cholesky, 2 step length, 3 inverse Cholesky).
- * Returns 0, or < 0 on an argument or device error. */
+ * Returns 0, or < 0: -1 null argument, -2 threads, -3 n or m out of range,
+ * -4 offsets out of range, -5 entry pointers, -6 an entry outside the block
+ * or below the diagonal, -7 too many entries, -8 lambda <= 0, -100/-101
+ * device errors. */
 #ifndef SDPA_BATCH_H
 #define SDPA_BATCH_H
 #ifdef __cplusplus
