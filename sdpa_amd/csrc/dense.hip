@@ -769,11 +769,14 @@ int DenseWork::ensure_side(int npanels) {
     }
     if (!side) SDPA_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   }
-  if (!evjoin) SDPA_HIP_CHECK(hipEventCreateWithFlags(&evjoin, hipEventDisableTiming));
+  // the events only order the two streams of this device: a device-scope
+  // release (no system-scope cache writeback) is enough
+  constexpr unsigned kEvFlags = hipEventDisableTiming | hipEventReleaseToDevice;
+  if (!evjoin) SDPA_HIP_CHECK(hipEventCreateWithFlags(&evjoin, kEvFlags));
   while ((int)evp.size() < npanels) {
     hipEvent_t a, b;
-    SDPA_HIP_CHECK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
-    SDPA_HIP_CHECK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+    SDPA_HIP_CHECK(hipEventCreateWithFlags(&a, kEvFlags));
+    SDPA_HIP_CHECK(hipEventCreateWithFlags(&b, kEvFlags));
     evp.push_back(a);
     evn.push_back(b);
   }
@@ -829,32 +832,37 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
       launch_syrk(s, w->d_mats, d_info, nitems, maxn, K0, PW, K0 + PW, maxn);
     }
   } else {
-    // Look-ahead: panel factorizations on `s`, trailing updates on a side
-    // stream.  The update of panel p+1's columns goes first, so panel p+1 can
-    // be factored while the rest of panel p's trailing update runs; all
-    // trailing updates stay in order on the side stream (no overlapping
-    // read-modify-writes), and panel p+1's columns are disjoint from the rest.
+    // Look-ahead.  Panel p's trailing update is split by columns: panel
+    // p+1's (on `s`, right after the panel: the next panel is factored on
+    // the same stream without a cross-stream wait), panel p+2's and the rest
+    // (both on the CU-masked side stream, beside the next panels).  Every
+    // update of a column block is ordered after the earlier ones that write
+    // it: panel p+1's columns are last written on the side by panel p-1's
+    // second launch (evn[p-1], which the side stream's order puts after all
+    // earlier side launches); the side launches of panel p read the panel
+    // only after it is factored (evp[p]).
     if (int e = w->ensure_side(np)) return e;
     SDPA_HIP_CHECK(hipEventRecord(w->evjoin, s));
     SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evjoin, 0));
     for (int p = 0; p < np; ++p) {
       const int K0 = p * PW;
-      if (p > 0) SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));
       factor_panel(s, w->d_mats, 1, maxn, K0, PW, mode, d_info, dinv, nslot);
       if (K0 + PW >= maxn) break;
       SDPA_HIP_CHECK(hipEventRecord(w->evp[p], s));
+      if (p > 0) SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));
+      launch_syrk(s, w->d_mats, d_info, 1, maxn, K0, PW, K0 + PW, K0 + 2 * PW);
       SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
-      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + PW, K0 + 2 * PW,
+      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, K0 + 3 * PW,
                   w->side_waves);
       SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
-      const int nrest = maxn - (K0 + 2 * PW);
+      const int nrest = maxn - (K0 + 3 * PW);
       // timed: the launches that take the wide-tile kernel (more than 256 columns)
       const bool timed = w->time_trailing && nrest > 256;
       if (timed) {
         if (int e = w->ensure_trailing_events(w->trail_pending + 1)) return e;
         SDPA_HIP_CHECK(hipEventRecord(w->evt0[w->trail_pending], w->side));
       }
-      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, maxn, w->side_waves);
+      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 3 * PW, maxn, w->side_waves);
       if (timed) {
         SDPA_HIP_CHECK(hipEventRecord(w->evt1[w->trail_pending], w->side));
         // algorithmic flops: 2 K per element of the lower triangle updated
@@ -1099,32 +1107,30 @@ int potrf_mixed(hipStream_t s, DenseWork* w, const double* B, int ld, int m, flo
   SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats, pd.data(), sizeof(MatDesc) * np, hipMemcpyHostToDevice, s));
   const int nslot = (m + TB - 1) / TB;
   // Look-ahead as in potrf_batched: the panel chain (widen, factor, narrow)
-  // on `s`, the FP32 trailing updates on the CU-masked side stream, panel
-  // p + 1's columns first.  Panel p + 1 is widened once that update (and,
-  // by the side stream's order, every earlier one) is done; the narrowed
-  // panel p is read by the side stream only after it is written (evp).
+  // and the FP32 update of the next panel's columns on `s`, the rest of each
+  // trailing update on the CU-masked side stream.  The next panel's update
+  // also writes it widened into P64 (free once the narrowing has read it,
+  // same stream); it waits for the rest of the previous panel's update,
+  // which writes the same columns (evn), and the rest of panel p's update
+  // reads the narrowed panel only after it is written (evp).
   if (int e = w->ensure_side(np)) return e;
   SDPA_HIP_CHECK(hipEventRecord(w->evjoin, s));
   SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evjoin, 0));
   for (int p = 0; p < np; ++p) {
     const int K0 = p * PW, pw = std::min(PW, m - K0);
     const dim3 grid((m - K0 + 63) / 64, (pw + 63) / 64);
-    if (p > 0)
-      SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));  // widened by that update
-    else
-      hipLaunchKernelGGL(k_panel_to64, grid, dim3(256), 0, s, A32, m, m, K0, pw, P64, m);
+    if (p == 0) hipLaunchKernelGGL(k_panel_to64, grid, dim3(256), 0, s, A32, m, m, K0, pw, P64, m);
     factor_panel(s, w->d_mats + p, 1, m, K0, PW, 1, d_info, dinv, nslot);
     hipLaunchKernelGGL(k_panel_to32, grid, dim3(256), 0, s, P64, m, m, K0, pw, A32, m,
                        D64 + (size_t)p * PW * PW);
     if (K0 + PW >= m) break;
+    if (p > 0) SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));
+    launch_syrk32(s, A32, m, m, K0, PW, K0 + PW, K0 + 2 * PW, d_info, 0, P64);
     SDPA_HIP_CHECK(hipEventRecord(w->evp[p], s));
     SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
-    // the next panel's columns, also written widened into P64 (free once the
-    // narrowing above has read it: evp)
-    launch_syrk32(w->side, A32, m, m, K0, PW, K0 + PW, K0 + 2 * PW, d_info, w->side_waves, P64);
-    SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
     if (K0 + 2 * PW < m)
       launch_syrk32(w->side, A32, m, m, K0, PW, K0 + 2 * PW, m, d_info, w->side_waves);
+    SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
   }
   SDPA_HIP_CHECK(hipEventRecord(w->evjoin, w->side));
   SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evjoin, 0));
