@@ -87,6 +87,8 @@ struct Ctx {
   double* d_F1C = nullptr;  // F1 dense-pair products
   size_t capF1C = 0;
   double* d_F1P = nullptr;  // their partial sums over slices of the n^2 dimension
+  double* d_hpart = nullptr;  // F3 heavy-pair partial sums (nheavy^2 x slices)
+  size_t capHpart = 0;
   size_t capF1P = 0;
   int* d_info = nullptr;   // factorization info (one per batch item)
   double* d_red = nullptr;  // reduction partials

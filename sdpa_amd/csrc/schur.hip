@@ -80,6 +80,126 @@ __global__ __launch_bounds__(256) void k_f3_light(
   else if (mine) *dst += val;
 }
 
+constexpr int kColsWG = 4;      // columns per work-group pass of k_f3_cols
+constexpr int kColsMaxN = 256;  // block sizes whose staged rows fit LDS
+constexpr int kColsRows = 1024; // rows per work-group of k_f3_cols
+
+// Light F3 pairs by column blocks.  A work-group takes the columns
+// [kc, kc + 4) of the block's constraint list (and, for balance, the mirror
+// block counted from the end) and walks the rows ki >= kc, one thread per
+// row.  For a column with one stored entry (a, b) the rows a and b of X and
+// Z^-1 -- everything its pairs with one-entry rows read -- are staged in LDS
+// first, so those pairs' eight gathers are LDS reads and the stores of B
+// (column-major) are coalesced along ki.  Pairs with a multi-entry
+// constraint go through f3_pair from global memory.  The value of a pair
+// does not depend on which side is the owner (the calF3 terms are the same
+// four products in another order); ownership decides only whether F3 or the
+// F1/F2 path computes it and whether it is light.  Heavy pairs are left to
+// the heavy kernels (stored as 0 when `store`).  grid = (ceil(nblk / 2),
+// row segments).
+__global__ __launch_bounds__(256) void k_f3_cols(
+    const int* __restrict__ con, const int* __restrict__ estart, const int* __restrict__ ecnt,
+    const int* __restrict__ nzeff, const int* __restrict__ form, const int* __restrict__ er,
+    const int* __restrict__ ec, const double* __restrict__ ev, int ncon, int n,
+    const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ B, int m,
+    int store) {
+  __shared__ double sR[kColsWG][4][kColsMaxN];  // X row a, X row b, Z^-1 row a, Z^-1 row b
+  __shared__ double sv[kColsWG];
+  __shared__ int sa[kColsWG], sb[kColsWG], scnt[kColsWG], sst[kColsWG], snz[kColsWG],
+      sform[kColsWG], scon[kColsWG];
+  const int tid = threadIdx.x;
+  const int nblk = (ncon + kColsWG - 1) / kColsWG;
+  // rows of the two blocks, concatenated, cut into segments of kColsRows
+  // (blockIdx.y): more work-groups in flight, each with a short row walk
+  int off = 0;
+  const int s0 = (int)blockIdx.y * kColsRows, s1 = s0 + kColsRows;
+  for (int side = 0; side < 2; ++side) {
+    const int blk = side ? nblk - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    if (side && blk <= (int)blockIdx.x) break;
+    const int kc = blk * kColsWG;
+    const int len = ncon - kc;
+    const int r0 = kc + max(0, s0 - off), r1 = kc + min(len, s1 - off);
+    off += len;
+    if (r0 >= r1) continue;
+    __syncthreads();  // the previous pass is done with the staged rows
+    if (tid < kColsWG) {
+      const int kj = kc + tid;
+      const bool in = kj < ncon;
+      const int st = in ? estart[kj] : 0, ct = in ? ecnt[kj] : 0;
+      scnt[tid] = ct;
+      sst[tid] = st;
+      snz[tid] = in ? nzeff[kj] : 0;
+      sform[tid] = in ? form[kj] : 0;
+      scon[tid] = in ? con[kj] : 0;
+      sa[tid] = ct == 1 ? er[st] : 0;
+      sb[tid] = ct == 1 ? ec[st] : 0;
+      sv[tid] = ct == 1 ? ev[st] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < kColsWG * n; e += 256) {
+      const int c = e / n, g = e - c * n;
+      if (scnt[c] == 1) {
+        const size_t ra = (size_t)n * sa[c], rb = (size_t)n * sb[c];
+        sR[c][0][g] = X[ra + g];
+        sR[c][1][g] = X[rb + g];
+        sR[c][2][g] = Zi[ra + g];
+        sR[c][3][g] = Zi[rb + g];
+      }
+    }
+    __syncthreads();
+    // a row's metadata is loaded one iteration ahead, with the entries of the
+    // current row (every constraint of the list has an entry in the block,
+    // so the loads need no guard; a clamped row index stays in range)
+    int ki = r0 + tid;
+    int kl = min(ki, ncon - 1);
+    int ci = ecnt[kl], si = estart[kl], nzi = nzeff[kl], fi = form[kl], cr = con[kl];
+    for (; ki < r1; ki += 256) {
+      const int g = er[si], d = ec[si];
+      const double vi = ev[si];
+      kl = min(ki + 256, ncon - 1);
+      const int cn = ecnt[kl], sn = estart[kl], nzn = nzeff[kl], fn = form[kl], crn = con[kl];
+      const size_t rowB = (size_t)cr;
+#pragma unroll 2
+      for (int c = 0; c < kColsWG; ++c) {
+        const int kj = kc + c;
+        if (kj >= ncon || ki < kj) continue;
+        const int nzj = snz[c], cj = scnt[c];
+        const bool jo = !(nzi > nzj);  // owner: tie -> the smaller index kj
+        const int co = jo ? cj : ci, cq = jo ? ci : cj, fo = jo ? sform[c] : fi;
+        const bool mine = fo == 3 && co <= kLight && cq <= kLight;
+        double val = 0.0;
+        if (mine) {
+          if (ci == 1 && cj == 1) {
+            const double* Xa = sR[c][0];
+            const double* Xb = sR[c][1];
+            const double* Za = sR[c][2];
+            const double* Zb = sR[c][3];
+            double plu;
+            if (sa[c] != sb[c]) {
+              plu = (g != d) ? Zb[d] * Xa[g] + Zb[g] * Xa[d] + Za[d] * Xb[g] + Za[g] * Xb[d]
+                             : Zb[d] * Xa[g] + Za[d] * Xb[g];
+            } else {
+              plu = (g != d) ? Zb[d] * Xa[g] + Zb[g] * Xa[d] : Zb[d] * Xa[g];
+            }
+            val = sv[c] * (vi * plu);
+          } else {
+            val = jo ? f3_pair(er, ec, ev, sst[c], cj, si, ci, n, X, Zi, 0, 1)
+                     : f3_pair(er, ec, ev, si, ci, sst[c], cj, n, X, Zi, 0, 1);
+          }
+        }
+        double* dst = B + rowB + (size_t)m * scon[c];
+        if (store) *dst = val;
+        else if (mine) *dst += val;
+      }
+      ci = cn;
+      si = sn;
+      nzi = nzn;
+      fi = fn;
+      cr = crn;
+    }
+  }
+}
+
 // One wave per pair that involves a heavy list.  grid = (ceil(ncon/4), nheavy).
 __global__ __launch_bounds__(256) void k_f3_heavy(
     const int* __restrict__ con, const int* __restrict__ estart, const int* __restrict__ ecnt,
@@ -102,13 +222,18 @@ __global__ __launch_bounds__(256) void k_f3_heavy(
   if (lane == 0) B[(size_t)con[ki] + (size_t)m * con[kj]] += v;
 }
 
-// Pairs of two heavy lists: one work-group per pair, the owner x partner
-// term grid flattened over 256 threads, block reduction.  grid = (nheavy, nheavy).
+// Pairs of two heavy lists: kHeavySlices work-groups per pair, each taking
+// every kHeavySlices-th stripe of 256 of the owner x partner term grid, four
+// products per thread per round with their loads issued together; the
+// slices' sums go to `part` and k_f3_heavy_sum adds them to B in slice
+// order (deterministic).  grid = (nheavy, nheavy, kHeavySlices).
+constexpr int kHeavySlices = 16;
+
 __global__ __launch_bounds__(256) void k_f3_heavy_pair(
-    const int* __restrict__ con, const int* __restrict__ estart, const int* __restrict__ ecnt,
-    const int* __restrict__ nzeff, const int* __restrict__ form, const int* __restrict__ heavy,
+    const int* __restrict__ estart, const int* __restrict__ ecnt, const int* __restrict__ nzeff,
+    const int* __restrict__ form, const int* __restrict__ heavy, int nheavy,
     const int* __restrict__ er, const int* __restrict__ ec, const double* __restrict__ ev, int n,
-    const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ B, int m) {
+    const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ part) {
   const int h1 = heavy[blockIdx.x], h2 = heavy[blockIdx.y];
   if (h1 < h2) return;
   const int ki = h1, kj = h2;
@@ -117,31 +242,75 @@ __global__ __launch_bounds__(256) void k_f3_heavy_pair(
   if (form[o] != 3) return;
   const int so = estart[o], co = ecnt[o], sp = estart[p], cp = ecnt[p];
   const long total = (long)co * cp;
+  constexpr int kStep = 256 * kHeavySlices;
+  // (e1, e2) of t = e1 cp + e2, advanced by kStep without a 64-bit division
+  const int qs = kStep / cp, rs = kStep % cp;
+  const int t00 = 256 * (int)blockIdx.z + (int)threadIdx.x;
+  int e1c = t00 / cp, e2c = t00 % cp;
   double acc = 0.0;
-  for (long t = threadIdx.x; t < total; t += 256) {
-    const int e1 = (int)(t / cp), e2 = (int)(t % cp);
-    const int a = er[so + e1], b = ec[so + e1];
-    const int g = er[sp + e2], d = ec[sp + e2];
-    const double* Xa = X + (size_t)n * a;
-    const double* Xb = X + (size_t)n * b;
-    const double* Za = Zi + (size_t)n * a;
-    const double* Zb = Zi + (size_t)n * b;
-    double plu;
-    if (a != b) {
-      if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d] + Za[d] * Xb[g] + Za[g] * Xb[d];
-      else plu = Zb[d] * Xa[g] + Za[d] * Xb[g];
-    } else {
-      if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d];
-      else plu = Zb[d] * Xa[g];
+  for (long t0 = t00; t0 < total; t0 += 4L * kStep) {
+    int a[4], b[4], g[4], d[4];
+    double w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = t0 + (long)kStep * u < total;
+      const int e1 = in ? e1c : 0, e2 = in ? e2c : 0;
+      e1c += qs;
+      e2c += rs;
+      if (e2c >= cp) {
+        e2c -= cp;
+        ++e1c;
+      }
+      a[u] = er[so + e1];
+      b[u] = ec[so + e1];
+      g[u] = er[sp + e2];
+      d[u] = ec[sp + e2];
+      w[u] = in ? ev[so + e1] * ev[sp + e2] : 0.0;
     }
-    acc += ev[so + e1] * (ev[sp + e2] * plu);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double* Xa = X + (size_t)n * a[u];
+      const double* Xb = X + (size_t)n * b[u];
+      const double* Za = Zi + (size_t)n * a[u];
+      const double* Zb = Zi + (size_t)n * b[u];
+      const int gg = g[u], dd = d[u];
+      double plu;
+      if (a[u] != b[u]) {
+        if (gg != dd) plu = Zb[dd] * Xa[gg] + Zb[gg] * Xa[dd] + Za[dd] * Xb[gg] + Za[gg] * Xb[dd];
+        else plu = Zb[dd] * Xa[gg] + Za[dd] * Xb[gg];
+      } else {
+        if (gg != dd) plu = Zb[dd] * Xa[gg] + Zb[gg] * Xa[dd];
+        else plu = Zb[dd] * Xa[gg];
+      }
+      acc += w[u] * plu;
+    }
   }
   __shared__ double red[4];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0)
-    B[(size_t)con[ki] + (size_t)m * con[kj]] += (red[0] + red[1]) + (red[2] + red[3]);
+    part[((size_t)blockIdx.x * nheavy + blockIdx.y) * kHeavySlices + blockIdx.z] =
+        (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// B[i, j] += the slices of each heavy pair, in slice order.  One thread per
+// (x, y) of the nheavy x nheavy grid.
+__global__ __launch_bounds__(256) void k_f3_heavy_sum(
+    const int* __restrict__ con, const int* __restrict__ nzeff, const int* __restrict__ form,
+    const int* __restrict__ heavy, int nheavy, const double* __restrict__ part,
+    double* __restrict__ B, int m) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nheavy * nheavy) return;
+  const int x = t / nheavy, y = t % nheavy;
+  const int h1 = heavy[x], h2 = heavy[y];
+  if (h1 < h2) return;
+  const int o = (nzeff[h1] > nzeff[h2]) ? h1 : h2;
+  if (form[o] != 3) return;
+  const double* ps = part + (size_t)t * kHeavySlices;
+  double sum = 0.0;
+  for (int z = 0; z < kHeavySlices; ++z) sum += ps[z];
+  B[(size_t)con[h1] + (size_t)m * con[h2]] += sum;
 }
 
 // ---- F1/F2 ----------------------------------------------------------------
@@ -380,18 +549,35 @@ int make_bmat(Ctx* c) {
     if (T.nF1 > 0)
       if (int e = f1_block(c, T, X, Zi)) return e;
     if (T.nF3 > 0) {
-      dim3 g((T.ncon + 63) / 64, (T.ncon + 3) / 4);
-      hipLaunchKernelGGL(k_f3_light, g, dim3(256), 0, s, T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff,
-                         T.d_form, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n, X, Zi, c->d_B, m,
-                         c->bmatStoreFirst ? 1 : 0);
+      if (T.n <= kColsMaxN) {
+        const int nblk = (T.ncon + kColsWG - 1) / kColsWG;
+        const dim3 gc((nblk + 1) / 2, (T.ncon + kColsWG + kColsRows - 1) / kColsRows);
+        hipLaunchKernelGGL(k_f3_cols, gc, dim3(256), 0, s, T.d_con, T.d_estart,
+                           T.d_ecnt, T.d_nzeff, T.d_form, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n,
+                           X, Zi, c->d_B, m, c->bmatStoreFirst ? 1 : 0);
+      } else {
+        dim3 g((T.ncon + 63) / 64, (T.ncon + 3) / 4);
+        hipLaunchKernelGGL(k_f3_light, g, dim3(256), 0, s, T.d_con, T.d_estart, T.d_ecnt,
+                           T.d_nzeff, T.d_form, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n, X, Zi,
+                           c->d_B, m, c->bmatStoreFirst ? 1 : 0);
+      }
       if (T.nheavy > 0) {
         dim3 gh((T.ncon + 3) / 4, T.nheavy);
         hipLaunchKernelGGL(k_f3_heavy, gh, dim3(256), 0, s, T.d_con, T.d_estart, T.d_ecnt,
                            T.d_nzeff, T.d_form, T.d_heavy, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n,
                            X, Zi, c->d_B, m);
-        hipLaunchKernelGGL(k_f3_heavy_pair, dim3(T.nheavy, T.nheavy), dim3(256), 0, s, T.d_con,
-                           T.d_estart, T.d_ecnt, T.d_nzeff, T.d_form, T.d_heavy, c->d_er, c->d_ec,
-                           c->d_ev, T.n, X, Zi, c->d_B, m);
+        const size_t np = (size_t)T.nheavy * T.nheavy * kHeavySlices;
+        if (np > c->capHpart) {
+          if (c->d_hpart) SDPA_HIP_CHECK(hipFree(c->d_hpart));
+          c->capHpart = np;
+          SDPA_HIP_CHECK(hipMalloc(&c->d_hpart, np * sizeof(double)));
+        }
+        hipLaunchKernelGGL(k_f3_heavy_pair, dim3(T.nheavy, T.nheavy, kHeavySlices), dim3(256), 0,
+                           s, T.d_estart, T.d_ecnt, T.d_nzeff, T.d_form, T.d_heavy, T.nheavy,
+                           c->d_er, c->d_ec, c->d_ev, T.n, X, Zi, c->d_hpart);
+        hipLaunchKernelGGL(k_f3_heavy_sum, dim3((T.nheavy * T.nheavy + 255) / 256), dim3(256), 0,
+                           s, T.d_con, T.d_nzeff, T.d_form, T.d_heavy, T.nheavy, c->d_hpart,
+                           c->d_B, m);
       }
     }
   }

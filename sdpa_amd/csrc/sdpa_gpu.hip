@@ -60,14 +60,14 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_Dx, (void**)&c->d_Dz, (void**)&c->d_dy, (void**)&c->d_rzinv,
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
-                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P,
+                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P, (void**)&c->d_hpart,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
   for (void** p : ptrs) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
-  c->capG = c->capF1C = c->capF1P = 0;
+  c->capG = c->capF1C = c->capF1P = c->capHpart = 0;
   free_mixed(c);
   c->loaded = false;
 }
