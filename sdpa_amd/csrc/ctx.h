@@ -15,6 +15,7 @@ struct BlockTable {
   int nF1 = 0, nF3 = 0;
   int nheavy = 0;
   int ndense = 0;     // constraints whose block is dense (changeToDense)
+  int f3cols = 0;     // F3 by column blocks (nearly all constraints one entry, n <= 256)
   // device arrays, length ncon (or as noted)
   int* d_con = nullptr;    // global constraint index (increasing)
   int* d_estart = nullptr; // first entry in the entry pool

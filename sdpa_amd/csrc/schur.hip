@@ -549,7 +549,7 @@ int make_bmat(Ctx* c) {
     if (T.nF1 > 0)
       if (int e = f1_block(c, T, X, Zi)) return e;
     if (T.nF3 > 0) {
-      if (T.n <= kColsMaxN) {
+      if (T.f3cols && T.n <= kColsMaxN) {
         const int nblk = (T.ncon + kColsWG - 1) / kColsWG;
         const dim3 gc((nblk + 1) / 2, (T.ncon + kColsWG + kColsRows - 1) / kColsRows);
         hipLaunchKernelGGL(k_f3_cols, gc, dim3(256), 0, s, T.d_con, T.d_estart,

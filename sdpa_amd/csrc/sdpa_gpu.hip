@@ -271,6 +271,15 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       if (!isdense[k] && ecnt[k] > kLightHost) heavy.push_back(k);
     }
     T.nheavy = (int)heavy.size();
+    {
+      // the column-block F3 kernel stages the two rows of X and Z^-1 a
+      // one-entry column reads; it pays when nearly every constraint of the
+      // block has one entry (constraints spread over many small blocks with
+      // a few entries each take the pair kernel)
+      long one = 0;
+      for (int k = 0; k < nc; ++k) one += ecnt[k] == 1;
+      T.f3cols = (n <= 256 && nc >= 64 && one * 10 >= (long)nc * 9) ? 1 : 0;
+    }
     if (int e = upload(c, &T.d_con, con)) return e;
     if (int e = upload(c, &T.d_estart, estart)) return e;
     if (int e = upload(c, &T.d_ecnt, ecnt)) return e;

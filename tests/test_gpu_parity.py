@@ -67,6 +67,10 @@ CASES = {
     "maxcut60_trace": lambda: PR.maxcut(60, 0.2, 4, trace_row=True),
     "sparse_mixed": sparse_mixed,
     "lovasz40": lambda: PR.lovasz(40, 0.3, 5),
+    # one-entry constraints with a heavy trace row: the column-block F3 kernel
+    # (with the heavy-pair slices) on a 541- and an 81-constraint block
+    "lovasz60": lambda: PR.lovasz(60, 0.3, 7),
+    "maxcut80_trace": lambda: PR.maxcut(80, 0.2, 8, trace_row=True),
     "dense_n20_m30": lambda: PR.dense_random(n=20, m=30, seed=2),
     "dense_n70_m40": lambda: PR.dense_random(n=70, m=40, seed=6),
     "multiblock_lp": lambda: PR.multi_block(n_sdp_blocks=6, nmin=3, nmax=30, lp=12, m=40,
