@@ -80,12 +80,12 @@ __global__ __launch_bounds__(256) void k_f3_light(
   else if (mine) *dst += val;
 }
 
-constexpr int kColsWG = 4;      // columns per work-group pass of k_f3_cols
+constexpr int kColsWG = 2;      // columns per work-group pass of k_f3_cols
 constexpr int kColsMaxN = 256;  // block sizes whose staged rows fit LDS
 constexpr int kColsRows = 1024; // rows per work-group of k_f3_cols
 
 // Light F3 pairs by column blocks.  A work-group takes the columns
-// [kc, kc + 4) of the block's constraint list (and, for balance, the mirror
+// [kc, kc + 2) of the block's constraint list (and, for balance, the mirror
 // block counted from the end) and walks the rows ki >= kc, one thread per
 // row.  For a column with one stored entry (a, b) the rows a and b of X and
 // Z^-1 -- everything its pairs with one-entry rows read -- are staged in LDS
@@ -150,6 +150,20 @@ __global__ __launch_bounds__(256) void k_f3_cols(
     // a row's metadata is loaded one iteration ahead, with the entries of the
     // current row (every constraint of the list has an entry in the block,
     // so the loads need no guard; a clamped row index stays in range)
+    // the columns' metadata in registers for the whole walk
+    int cnz[kColsWG], ccnt[kColsWG], cform[kColsWG];
+    size_t ccol[kColsWG];
+    bool cdiag[kColsWG];
+    double cv[kColsWG];
+#pragma unroll
+    for (int c = 0; c < kColsWG; ++c) {
+      cnz[c] = snz[c];
+      ccnt[c] = scnt[c];
+      cform[c] = sform[c];
+      ccol[c] = (size_t)m * scon[c];
+      cdiag[c] = sa[c] == sb[c];
+      cv[c] = sv[c];
+    }
     int ki = r0 + tid;
     int kl = min(ki, ncon - 1);
     int ci = ecnt[kl], si = estart[kl], nzi = nzeff[kl], fi = form[kl], cr = con[kl];
@@ -159,13 +173,13 @@ __global__ __launch_bounds__(256) void k_f3_cols(
       kl = min(ki + 256, ncon - 1);
       const int cn = ecnt[kl], sn = estart[kl], nzn = nzeff[kl], fn = form[kl], crn = con[kl];
       const size_t rowB = (size_t)cr;
-#pragma unroll 2
+#pragma unroll
       for (int c = 0; c < kColsWG; ++c) {
         const int kj = kc + c;
         if (kj >= ncon || ki < kj) continue;
-        const int nzj = snz[c], cj = scnt[c];
-        const bool jo = !(nzi > nzj);  // owner: tie -> the smaller index kj
-        const int co = jo ? cj : ci, cq = jo ? ci : cj, fo = jo ? sform[c] : fi;
+        const int cj = ccnt[c];
+        const bool jo = !(nzi > cnz[c]);  // owner: tie -> the smaller index kj
+        const int co = jo ? cj : ci, cq = jo ? ci : cj, fo = jo ? cform[c] : fi;
         const bool mine = fo == 3 && co <= kLight && cq <= kLight;
         double val = 0.0;
         if (mine) {
@@ -175,19 +189,19 @@ __global__ __launch_bounds__(256) void k_f3_cols(
             const double* Za = sR[c][2];
             const double* Zb = sR[c][3];
             double plu;
-            if (sa[c] != sb[c]) {
+            if (!cdiag[c]) {
               plu = (g != d) ? Zb[d] * Xa[g] + Zb[g] * Xa[d] + Za[d] * Xb[g] + Za[g] * Xb[d]
                              : Zb[d] * Xa[g] + Za[d] * Xb[g];
             } else {
               plu = (g != d) ? Zb[d] * Xa[g] + Zb[g] * Xa[d] : Zb[d] * Xa[g];
             }
-            val = sv[c] * (vi * plu);
+            val = cv[c] * (vi * plu);
           } else {
             val = jo ? f3_pair(er, ec, ev, sst[c], cj, si, ci, n, X, Zi, 0, 1)
                      : f3_pair(er, ec, ev, si, ci, sst[c], cj, n, X, Zi, 0, 1);
           }
         }
-        double* dst = B + rowB + (size_t)m * scon[c];
+        double* dst = B + rowB + ccol[c];
         if (store) *dst = val;
         else if (mine) *dst += val;
       }
