@@ -70,13 +70,20 @@ def block_features(d: PR.SdpData) -> np.ndarray:
 
 def scaled_problem(seed: int) -> PR.SdpData:
     """Training and evaluation family: small MaxCut and Lovasz-theta
-    instances whose cost matrix and right-hand side are scaled by
+    instances and small multi-block problems (the generator of the
+    200-block configuration at 3-20 blocks of 5-50, an LP block of 0-200,
+    m = 50-400), whose cost matrix and right-hand side are scaled by
     independent factors 10^U(-3, 4), so the best lambda* varies over
     decades (and the default 100 can end in a wrong infeasibility or
     unboundedness verdict)."""
     rng = np.random.default_rng(seed)
-    kind = int(rng.integers(3))
-    if kind == 0:
+    kind = int(rng.integers(4))
+    if kind == 3:
+        d = PR.multi_block(n_sdp_blocks=int(rng.integers(3, 21)), nmin=5, nmax=50,
+                           lp=int(rng.integers(0, 201)), m=int(rng.integers(50, 401)),
+                           density=float(10.0 ** rng.uniform(-3, -1.5)),
+                           seed=int(rng.integers(1 << 30)))
+    elif kind == 0:
         d = PR.maxcut(int(rng.integers(10, 41)), float(rng.uniform(0.1, 0.5)), int(rng.integers(1 << 30)))
     elif kind == 1:
         d = PR.lovasz(int(rng.integers(8, 21)), float(rng.uniform(0.3, 0.5)), int(rng.integers(1 << 30)))

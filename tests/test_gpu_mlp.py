@@ -58,7 +58,7 @@ def test_training_fits_a_smooth_target():
 def test_trained_model_beats_default_lambda():
     model = LM.LambdaModel.load()
     di, mi, ok0, ok1 = [], [], 0, 0
-    for sd in range(70000, 70016):  # held out: training used 1000.., evaluation 50000..
+    for sd in range(70000, 70040):  # held out: training used 1000.., evaluation 50000..
         d = LM.scaled_problem(sd)
         s = sa.Solver(0).load(d).prepare()
         r0, _ = LM.solve(d, None, s)
@@ -68,10 +68,12 @@ def test_trained_model_beats_default_lambda():
         ok1 += r1.phase == "pdOPT"
         if r0.phase == "pdOPT":
             # wherever the default start converges, the predicted one does
-            # (directly or through the fallback) to the same optimum
+            # (directly or through the fallback) to the same optimum: both
+            # stop at gap <= 1e-7 max(1, |objective|), so objectives below 1
+            # in magnitude agree to about 2e-7 absolute
             assert r1.phase == "pdOPT", (sd, r1.phase)
-            assert r1.primal_obj == pytest.approx(r0.primal_obj, rel=1e-6, abs=1e-8)
-            assert r1.dual_obj == pytest.approx(r0.dual_obj, rel=1e-6, abs=1e-8)
+            assert r1.primal_obj == pytest.approx(r0.primal_obj, rel=1e-6, abs=2e-7)
+            assert r1.dual_obj == pytest.approx(r0.dual_obj, rel=1e-6, abs=2e-7)
             di.append(r0.iterations)
             mi.append(r1.iterations)
     print("pdOPT default", ok0, "model", ok1, "iterations default", np.mean(di), di,
