@@ -14,7 +14,8 @@ The prediction only chooses the starting point: the stopping rules are the
 solver's own, and `solve` falls back to the default lambda* (no initial
 point) whenever the predicted start does not end in pdOPT.
 
-    python -m sdpa_amd.lambda_mlp train --out sdpa_amd/data/lambda_mlp.npz
+    python -m sdpa_amd.lambda_mlp train --out sdpa_amd/data/lambda_mlp.npz \
+        --report sdpa_amd/data/lambda_mlp_report.json
 """
 from __future__ import annotations
 
@@ -94,6 +95,25 @@ def scaled_problem(seed: int) -> PR.SdpData:
     v = d.v.copy()
     v[d.k == 0] *= sc
     return dataclasses.replace(d, v=v, c=d.c * sb, name=f"{d.name}_sc{sc:.3g}_sb{sb:.3g}")
+
+
+def large_problem(seed: int) -> PR.SdpData:
+    """Training family at the size of the 200-block configuration: 50-200
+    SDP blocks of 5-50, an LP block of 200-1000, m = 1000-5000 at density
+    1e-3; half of them with C and b scaled by 10^U(-2, 2)."""
+    rng = np.random.default_rng(seed)
+    d = PR.multi_block(n_sdp_blocks=int(rng.integers(50, 201)), nmin=5, nmax=50,
+                       lp=int(rng.integers(200, 1001)), m=int(rng.integers(1000, 5001)),
+                       density=1e-3, seed=int(rng.integers(1 << 30)))
+    if rng.random() < 0.5:
+        return d
+    sc, sb = 10.0 ** rng.uniform(-2, 2), 10.0 ** rng.uniform(-2, 2)
+    v = d.v.copy()
+    v[d.k == 0] *= sc
+    return dataclasses.replace(d, v=v, c=d.c * sb, name=f"{d.name}_sc{sc:.3g}_sb{sb:.3g}")
+
+
+FAMILIES = {"small": scaled_problem, "large": large_problem}
 
 
 def init_point(d: PR.SdpData, lam) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -237,12 +257,13 @@ def best_lambda(d: PR.SdpData, grid=LAMBDA_GRID):
     return label_from_runs(runs), runs
 
 
-def generate(seeds):
+def generate(seeds, family=scaled_problem):
     """Features and labels (log10 lambda*) over the given problem seeds."""
     F, t, meta = [], [], []
     for sd in seeds:
-        d = scaled_problem(int(sd))
+        d = family(int(sd))
         lam, runs = best_lambda(d)
+        print(f"lambda_mlp: {d.name}: label {lam}", flush=True)
         if lam is None:
             continue
         Fb = block_features(d)
@@ -278,27 +299,38 @@ def main():
     ap.add_argument("cmd", choices=["train"])
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--train", type=int, default=105,
-                    help="training problems (19 solves each: 1995 solves)")
+                    help="training problems of the small family (19 solves each)")
     ap.add_argument("--eval", type=int, default=40, help="held-out problems")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--report", default="")
+    ap.add_argument("--large", type=int, default=30,
+                    help="training problems of the large family (seeds 2000..)")
+    ap.add_argument("--eval-large", type=int, default=8, help="held-out large problems")
     a = ap.parse_args()
     t0 = time.time()
     F, t, meta = generate(range(1000, 1000 + a.train))
+    if a.large:
+        F2, t2, meta2 = generate(range(2000, 2000 + a.large), large_problem)
+        F, t, meta = np.concatenate([F, F2]), np.concatenate([t, t2]), meta + meta2
     print(f"lambda_mlp: {len(meta)} training problems, {len(t)} blocks, "
-          f"{a.train * len(LAMBDA_GRID)} solves, {time.time() - t0:.1f}s", flush=True)
+          f"{(a.train + a.large) * len(LAMBDA_GRID)} solves, {time.time() - t0:.1f}s", flush=True)
     model = LambdaModel(0, seed=7)
     losses = model.fit(F, t, steps=a.steps)
     print(f"lambda_mlp: training loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
     model.save(a.out)
     ev = []
-    for sd in range(50000, 50000 + a.eval):
-        d = scaled_problem(sd)
+    evs = [(sd, scaled_problem) for sd in range(50000, 50000 + a.eval)]
+    evs += [(sd, large_problem) for sd in range(60000, 60000 + a.eval_large)]
+    for sd, fam in evs:
+        d = fam(sd)
         s = Solver(0).load(d).prepare()
         r0, _ = solve(d, None, s)
         r1, lam = solve(d, model, s)
         s.close()
-        ev.append({"seed": sd, "default_iters": r0.iterations, "default_phase": r0.phase,
+        print(f"lambda_mlp: eval {d.name}: default {r0.phase} {r0.iterations}, model {r1.phase} "
+              f"{r1.iterations}{' (fallback)' if lam is None else ''}", flush=True)
+        ev.append({"seed": sd, "family": fam.__name__, "default_iters": r0.iterations,
+                   "default_phase": r0.phase,
                    "model_iters": r1.iterations, "model_phase": r1.phase,
                    "fallback": lam is None,
                    "obj_rel_diff": abs(r1.primal_obj - r0.primal_obj) / max(1e-12, abs(r0.primal_obj))})
@@ -306,7 +338,7 @@ def main():
     di = float(np.mean([e["default_iters"] for e in both])) if both else None
     mi = float(np.mean([e["model_iters"] for e in both])) if both else None
     rep = {"train_problems": len(meta), "train_blocks": int(len(t)),
-           "train_solves": a.train * len(LAMBDA_GRID), "loss_first": losses[0],
+           "train_solves": (a.train + a.large) * len(LAMBDA_GRID), "loss_first": losses[0],
            "loss_last": losses[-1], "eval_problems": len(ev),
            "pdOPT_default": sum(e["default_phase"] == "pdOPT" for e in ev),
            "pdOPT_model": sum(e["model_phase"] == "pdOPT" for e in ev),

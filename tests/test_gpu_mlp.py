@@ -80,3 +80,25 @@ def test_trained_model_beats_default_lambda():
           "model", np.mean(mi), mi)
     assert ok1 >= ok0
     assert np.mean(mi) <= np.mean(di) - 1.0
+
+
+@pytest.mark.skipif(not os.path.exists(LM.DEFAULT_WEIGHTS), reason="no trained weights")
+def test_trained_model_on_200_block_configuration():
+    # 200 SDP blocks of 5-50, an LP block of 1000, m = 5000 (seed 5): the
+    # predicted start reaches pdOPT by itself (no fallback), in fewer
+    # iterations than lambda* = 100, at the same optimum (both stop at gap
+    # <= 1e-7, objectives about 1.1e4)
+    from sdpa_amd import problems as PR
+    d = PR.multi_block()
+    model = LM.LambdaModel.load()
+    s = sa.Solver(0).load(d).prepare()
+    r0, _ = LM.solve(d, None, s)
+    r1, lam = LM.solve(d, model, s)
+    s.close()
+    print("default", r0.phase, r0.iterations, r0.primal_obj, "model", r1.phase, r1.iterations,
+          r1.primal_obj, "lambda range", None if lam is None else (lam.min(), lam.max()))
+    assert r0.phase == r1.phase == "pdOPT"
+    assert lam is not None
+    assert r1.iterations < r0.iterations
+    assert r1.primal_obj == pytest.approx(r0.primal_obj, rel=1e-6)
+    assert r1.dual_obj == pytest.approx(r0.dual_obj, rel=1e-6)
