@@ -27,6 +27,11 @@ struct BlockTable {
   int* d_f1own = nullptr;  // local indices of F1/F2 owners (nF1)
   int* d_dcon = nullptr;   // global constraint of each dense pool column (ndense)
   double* d_dense = nullptr;  // n*n x ndense dense pool (column per dense A_k)
+  // f3cols blocks: one 16-byte record per constraint (con, estart, nzeff,
+  // min(ecnt, 9) | form << 4 | a << 8 | b << 16 with (a, b) its entry when it
+  // has one) and that entry's value (0 otherwise)
+  int* d_rmeta = nullptr;
+  double* d_rval = nullptr;
   std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart, h_f1own;
 };
 

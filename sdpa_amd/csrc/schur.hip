@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_f3_light(
 
 constexpr int kColsWG = 2;      // columns per work-group pass of k_f3_cols
 constexpr int kColsMaxN = 256;  // block sizes whose staged rows fit LDS
-constexpr int kColsRows = 1024; // rows per work-group of k_f3_cols
+constexpr int kColsRows = 4096; // rows per work-group of k_f3_cols
 
 // Light F3 pairs by column blocks.  A work-group takes the columns
 // [kc, kc + 2) of the block's constraint list (and, for balance, the mirror
@@ -95,18 +95,20 @@ constexpr int kColsRows = 1024; // rows per work-group of k_f3_cols
 // does not depend on which side is the owner (the calF3 terms are the same
 // four products in another order); ownership decides only whether F3 or the
 // F1/F2 path computes it and whether it is light.  Heavy pairs are left to
-// the heavy kernels (stored as 0 when `store`).  grid = (ceil(nblk / 2),
+// the heavy kernels (stored as 0 when `store`).  A constraint's metadata is
+// one 16-byte record (rmeta: con, estart, nzeff, packed count/form/entry
+// position) and its entry's value (rval), two coalesced loads per row.
+// (Interleaving the staged rows as 16-byte pairs was measured slower: more
+// LDS bank conflicts on the random-row reads.)  grid = (ceil(nblk / 2),
 // row segments).
 __global__ __launch_bounds__(256) void k_f3_cols(
-    const int* __restrict__ con, const int* __restrict__ estart, const int* __restrict__ ecnt,
-    const int* __restrict__ nzeff, const int* __restrict__ form, const int* __restrict__ er,
+    const int4* __restrict__ rmeta, const double* __restrict__ rval, const int* __restrict__ er,
     const int* __restrict__ ec, const double* __restrict__ ev, int ncon, int n,
     const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ B, int m,
     int store) {
   __shared__ double sR[kColsWG][4][kColsMaxN];  // X row a, X row b, Z^-1 row a, Z^-1 row b
+  __shared__ int4 smeta[kColsWG];
   __shared__ double sv[kColsWG];
-  __shared__ int sa[kColsWG], sb[kColsWG], scnt[kColsWG], sst[kColsWG], snz[kColsWG],
-      sform[kColsWG], scon[kColsWG];
   const int tid = threadIdx.x;
   const int nblk = (ncon + kColsWG - 1) / kColsWG;
   // rows of the two blocks, concatenated, cut into segments of kColsRows
@@ -124,22 +126,15 @@ __global__ __launch_bounds__(256) void k_f3_cols(
     __syncthreads();  // the previous pass is done with the staged rows
     if (tid < kColsWG) {
       const int kj = kc + tid;
-      const bool in = kj < ncon;
-      const int st = in ? estart[kj] : 0, ct = in ? ecnt[kj] : 0;
-      scnt[tid] = ct;
-      sst[tid] = st;
-      snz[tid] = in ? nzeff[kj] : 0;
-      sform[tid] = in ? form[kj] : 0;
-      scon[tid] = in ? con[kj] : 0;
-      sa[tid] = ct == 1 ? er[st] : 0;
-      sb[tid] = ct == 1 ? ec[st] : 0;
-      sv[tid] = ct == 1 ? ev[st] : 0.0;
+      smeta[tid] = kj < ncon ? rmeta[kj] : make_int4(0, 0, 0, 0);
+      sv[tid] = kj < ncon ? rval[kj] : 0.0;
     }
     __syncthreads();
     for (int e = tid; e < kColsWG * n; e += 256) {
       const int c = e / n, g = e - c * n;
-      if (scnt[c] == 1) {
-        const size_t ra = (size_t)n * sa[c], rb = (size_t)n * sb[c];
+      const int w = smeta[c].w;
+      if ((w & 15) == 1) {
+        const size_t ra = (size_t)n * ((w >> 8) & 255), rb = (size_t)n * ((w >> 16) & 255);
         sR[c][0][g] = X[ra + g];
         sR[c][1][g] = X[rb + g];
         sR[c][2][g] = Zi[ra + g];
@@ -147,38 +142,41 @@ __global__ __launch_bounds__(256) void k_f3_cols(
       }
     }
     __syncthreads();
-    // a row's metadata is loaded one iteration ahead, with the entries of the
-    // current row (every constraint of the list has an entry in the block,
-    // so the loads need no guard; a clamped row index stays in range)
     // the columns' metadata in registers for the whole walk
-    int cnz[kColsWG], ccnt[kColsWG], cform[kColsWG];
+    int cnz[kColsWG], ccnt[kColsWG], cform[kColsWG], cst[kColsWG];
     size_t ccol[kColsWG];
     bool cdiag[kColsWG];
     double cv[kColsWG];
 #pragma unroll
     for (int c = 0; c < kColsWG; ++c) {
-      cnz[c] = snz[c];
-      ccnt[c] = scnt[c];
-      cform[c] = sform[c];
-      ccol[c] = (size_t)m * scon[c];
-      cdiag[c] = sa[c] == sb[c];
+      const int4 q = smeta[c];
+      cnz[c] = q.z;
+      ccnt[c] = q.w & 15;
+      cform[c] = (q.w >> 4) & 3;
+      cst[c] = q.y;
+      ccol[c] = (size_t)m * q.x;
+      cdiag[c] = ((q.w >> 8) & 255) == ((q.w >> 16) & 255);
       cv[c] = sv[c];
     }
+    // a row's record is loaded one iteration ahead (a clamped row index stays
+    // in range)
     int ki = r0 + tid;
     int kl = min(ki, ncon - 1);
-    int ci = ecnt[kl], si = estart[kl], nzi = nzeff[kl], fi = form[kl], cr = con[kl];
+    int4 qi = rmeta[kl];
+    double vi = rval[kl];
     for (; ki < r1; ki += 256) {
-      const int g = er[si], d = ec[si];
-      const double vi = ev[si];
       kl = min(ki + 256, ncon - 1);
-      const int cn = ecnt[kl], sn = estart[kl], nzn = nzeff[kl], fn = form[kl], crn = con[kl];
-      const size_t rowB = (size_t)cr;
+      const int4 qn = rmeta[kl];
+      const double vn = rval[kl];
+      const int ci = qi.w & 15, fi = (qi.w >> 4) & 3, g = (qi.w >> 8) & 255,
+                d = (qi.w >> 16) & 255;
+      const size_t rowB = (size_t)qi.x;
 #pragma unroll
       for (int c = 0; c < kColsWG; ++c) {
         const int kj = kc + c;
         if (kj >= ncon || ki < kj) continue;
         const int cj = ccnt[c];
-        const bool jo = !(nzi > cnz[c]);  // owner: tie -> the smaller index kj
+        const bool jo = !(qi.z > cnz[c]);  // owner: tie -> the smaller index kj
         const int co = jo ? cj : ci, cq = jo ? ci : cj, fo = jo ? cform[c] : fi;
         const bool mine = fo == 3 && co <= kLight && cq <= kLight;
         double val = 0.0;
@@ -197,19 +195,16 @@ __global__ __launch_bounds__(256) void k_f3_cols(
             }
             val = cv[c] * (vi * plu);
           } else {
-            val = jo ? f3_pair(er, ec, ev, sst[c], cj, si, ci, n, X, Zi, 0, 1)
-                     : f3_pair(er, ec, ev, si, ci, sst[c], cj, n, X, Zi, 0, 1);
+            val = jo ? f3_pair(er, ec, ev, cst[c], cj, qi.y, ci, n, X, Zi, 0, 1)
+                     : f3_pair(er, ec, ev, qi.y, ci, cst[c], cj, n, X, Zi, 0, 1);
           }
         }
         double* dst = B + rowB + ccol[c];
         if (store) *dst = val;
         else if (mine) *dst += val;
       }
-      ci = cn;
-      si = sn;
-      nzi = nzn;
-      fi = fn;
-      cr = crn;
+      qi = qn;
+      vi = vn;
     }
   }
 }
@@ -566,9 +561,9 @@ int make_bmat(Ctx* c) {
       if (T.f3cols && T.n <= kColsMaxN) {
         const int nblk = (T.ncon + kColsWG - 1) / kColsWG;
         const dim3 gc((nblk + 1) / 2, (T.ncon + kColsWG + kColsRows - 1) / kColsRows);
-        hipLaunchKernelGGL(k_f3_cols, gc, dim3(256), 0, s, T.d_con, T.d_estart,
-                           T.d_ecnt, T.d_nzeff, T.d_form, c->d_er, c->d_ec, c->d_ev, T.ncon, T.n,
-                           X, Zi, c->d_B, m, c->bmatStoreFirst ? 1 : 0);
+        hipLaunchKernelGGL(k_f3_cols, gc, dim3(256), 0, s, (const int4*)T.d_rmeta, T.d_rval,
+                           c->d_er, c->d_ec, c->d_ev, T.ncon, T.n, X, Zi, c->d_B, m,
+                           c->bmatStoreFirst ? 1 : 0);
       } else {
         dim3 g((T.ncon + 63) / 64, (T.ncon + 3) / 4);
         hipLaunchKernelGGL(k_f3_light, g, dim3(256), 0, s, T.d_con, T.d_estart, T.d_ecnt,

@@ -47,7 +47,7 @@ void free_problem(Ctx* c) {
   for (auto& T : c->blk) {
     for (void* p : {(void*)T.d_con, (void*)T.d_estart, (void*)T.d_ecnt, (void*)T.d_nzeff,
                     (void*)T.d_form, (void*)T.d_dpos, (void*)T.d_heavy, (void*)T.d_f1own,
-                    (void*)T.d_dcon, (void*)T.d_dense})
+                    (void*)T.d_dcon, (void*)T.d_dense, (void*)T.d_rmeta, (void*)T.d_rval})
       if (p) (void)hipFree(p);
   }
   c->blk.clear();
@@ -289,6 +289,22 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     if (int e = upload(c, &T.d_heavy, heavy)) return e;
     if (int e = upload(c, &T.d_f1own, f1own)) return e;
     if (int e = upload(c, &T.d_dcon, dcon)) return e;
+    if (T.f3cols) {
+      std::vector<int> rm(4 * (size_t)nc);
+      std::vector<double> rv(nc, 0.0);
+      for (int k = 0; k < nc; ++k) {
+        const int st = estart[k];
+        const bool one = ecnt[k] == 1;
+        rm[4 * k] = con[k];
+        rm[4 * k + 1] = st;
+        rm[4 * k + 2] = nzeff[k];
+        rm[4 * k + 3] = std::min(ecnt[k], kLightHost + 1) | (form[k] << 4) |
+                        (one ? (h_er[st] << 8) | (h_ec[st] << 16) : 0);
+        if (one) rv[k] = h_ev[st];
+      }
+      if (int e = upload(c, &T.d_rmeta, rm)) return e;
+      if (int e = upload(c, &T.d_rval, rv)) return e;
+    }
     T.h_con = con;
     T.h_form = form;
     T.h_dpos = dpos;
