@@ -3,15 +3,17 @@
 // sdpa_jordan.cpp:58-109).
 //
 // For each SDP block the scaled direction M = L D L^T (L = inverse Cholesky
-// factor, D = Dx or Dz) is formed by two batched MFMA GEMMs.  One wave per
-// (block, X/Z) then runs
-//   n > 20 : the truncated Lanczos of Lal::getMinEigen (sdpa_linear.cpp:29-143):
-//            start vector of ones, at most min(n, sqrt(n)+10) steps, the
-//            tridiagonal eigenproblem re-solved on the schedule of line 84,
-//            the stop tests of lines 52-56, result min - |err*beta|;
-//   n <= 20: all eigenvalues of M by cyclic Jacobi (the dsyev path, 145-182).
-// The tridiagonal eigenproblem is solved by implicit QL with Wilkinson shifts
-// (Bowdler-Martin-Reinsch-Wilkinson 1968), eigenvectors kept one row per lane.
+// factor, D = Dx or Dz) is formed by two batched MFMA GEMMs (or library
+// GEMMs for one large block).  Then, per (block, X/Z):
+//   n > 20 : one 256-thread work-group runs the truncated Lanczos of
+//            Lal::getMinEigen (sdpa_linear.cpp:29-143): start vector of ones,
+//            at most min(n, sqrt(n)+10) steps, the tridiagonal eigenproblem
+//            re-solved on the schedule of line 84, the stop tests of lines
+//            52-56, result min - |err*beta|.  The tridiagonal problem is
+//            solved by Sturm multisection with inverse iteration for the last
+//            eigenvector component (tridiag_min_block, eig_dev.h);
+//   n <= 20: one wave finds all eigenvalues of M by cyclic Jacobi (the dsyev
+//            path, sdpa_linear.cpp:145-182).
 #include "ctx.h"
 #include "eig_dev.h"
 
@@ -46,8 +48,12 @@ __global__ __launch_bounds__(64) void k_min_eig_small(const double* __restrict__
   if (lane == 0) out[which * nblk + q] = result;
 }
 
-constexpr int kLzThreads = 256;
+constexpr int kLzThreads = 256;  // global-memory matrix-vector product (n > kLrMaxN)
+constexpr int kLrThreads = 512;  // matrix held in registers (20 < n <= kLrMaxN)
+constexpr int kLrCols = 104;     // columns per thread in the register-resident product
+constexpr int kLrMaxN = 2 * kLrCols;
 
+template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   v = wsum(v);
   const int w = threadIdx.x >> 6;
@@ -55,30 +61,57 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   if ((threadIdx.x & 63) == 0) red[w] = v;
   __syncthreads();
   double s = 0.0;
-  for (int i = 0; i < kLzThreads / 64; ++i) s += red[i];
+  for (int i = 0; i < NT / 64; ++i) s += red[i];
   return s;
 }
 
-// n > 20: the truncated Lanczos of Lal::getMinEigen on M, 256 threads per
-// (block, which); in the matrix-vector product each thread owns rows, the
-// tridiagonal eigenproblem runs on wave 0.
-__global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict__ Ms, long stride,
-                                                        const long* __restrict__ off,
-                                                        const int* __restrict__ nn, int nblk,
-                                                        double* __restrict__ out) {
+// n > 20: the truncated Lanczos of Lal::getMinEigen on M, one work-group of
+// NT threads per (block, which).  The matrix-vector product w = M q:
+//   NC == 0: one thread per row, streaming M from global memory every step
+//            (lanes over consecutive rows: each column read is coalesced);
+//   NC > 0 : M stays on chip for the whole run.  Wave w holds rows
+//            64 (w % 4) + lane and the column half w / 4 (NC columns, n <= 2
+//            NC) in registers, loaded once; a step is NC register FMAs per
+//            thread against LDS broadcasts of q, then the two halves are
+//            added.  (Per step the streaming form moves all n^2 doubles
+//            through one CU's L1: about 8 us at n = 200, against about 1 us.)
+// The tridiagonal eigenproblem (tridiag_min_block) takes all NT threads.
+template <int NT, int NC>
+__global__ __launch_bounds__(NT) void k_lanczos(const double* __restrict__ Ms, long stride,
+                                                const long* __restrict__ off,
+                                                const int* __restrict__ nn, int nblk,
+                                                double* __restrict__ out) {
+  constexpr int VN = NC > 0 ? 2 * NC : kMaxN;  // LDS vector length
   const int q = blockIdx.x, which = blockIdx.y;
   const int n = nn[q];
-  if (n <= 20) return;
+  if (n <= 20 || (NC > 0 && n > 2 * NC) || (NC == 0 && n <= kLrMaxN)) return;
   const double* M = Ms + which * stride + off[q];
   const int tid = threadIdx.x;
   __shared__ double dg[kMaxLanczos], od[kMaxLanczos];  // tridiagonal of the recurrence
-  __shared__ double qv[kMaxN], qold[kMaxN], rv[kMaxN], wv[kMaxN];
-  __shared__ double red[kLzThreads / 64];
+  __shared__ double qv[VN], qold[VN], rv[VN], wv[VN];
+  __shared__ double part[NC > 0 ? 2 * 256 : 1];
+  __shared__ double red[NT / 64];
   __shared__ double sh[4];  // theta, last eigenvector component
-  __shared__ double trisc[2 * (kLzThreads / 64)];
+  __shared__ double trisc[2 * (NT / 64)];
+  // register-resident M (NC > 0): row ri, columns [cj0, cj0 + NC)
+  const int wave = tid >> 6, ri = 64 * (wave & 3) + (tid & 63), half = (n + 1) / 2;
+  const int cj0 = (wave >> 2) * half, cjn = min(n, cj0 + half) - cj0;
+  double mreg[NC > 0 ? NC : 1];
+  if (NC > 0) {
+    // straight-line loads from clamped indices (cjn >= 10 here), all in
+    // flight at once, then zeros outside the block (by a 0/1 factor: a select
+    // there becomes a branch around each load)
+    const double* Mr = M + min(ri, n - 1) + (size_t)n * cj0;
+    const double rowf = ri < n ? 1.0 : 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) mreg[c] = __builtin_nontemporal_load(Mr + (size_t)n * min(c, cjn - 1));
+#pragma unroll
+    for (int c = 0; c < NC; ++c) mreg[c] *= (c < cjn) ? rowf : 0.0;
+  }
   double mn = 1.0e+51, mn_old = 1.0e+52, error = 1.0e+10;
   int k = 0, kk = 0;
-  for (int i = tid; i < n; i += kLzThreads) { qv[i] = 0.0; rv[i] = 1.0; }
+  // entries past n stay 0: the register product reads q up to index 2 NC
+  for (int i = tid; i < VN; i += NT) { qv[i] = 0.0; rv[i] = i < n ? 1.0 : 0.0; }
   double beta = sqrt((double)n);
   const double sqn = sqrt((double)n);
   while (k < n && k < sqn + 10 && k < kMaxLanczos && beta > 1.0e-16 &&
@@ -86,38 +119,46 @@ __global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict
           fabs(error * beta) > (1.0e-2) * fabs(mn) + (1.0e-4))) {
     const double inv = 1.0 / beta;
     __syncthreads();
-    for (int i = tid; i < n; i += kLzThreads) { qold[i] = qv[i]; qv[i] = rv[i] * inv; }
+    for (int i = tid; i < n; i += NT) { qold[i] = qv[i]; qv[i] = rv[i] * inv; }
     __syncthreads();
-    // w = M q: one thread per row i, lanes over consecutive rows, so each
-    // step j reads 64 consecutive entries of column j (coalesced); q_j is an
-    // LDS broadcast.  32 columns of loads in flight per thread, no cross-lane
-    // reduction.
-    for (int i = tid; i < n; i += kLzThreads) {
-      const double* Mi = M + i;
+    if (NC > 0) {
       double a[4] = {0.0, 0.0, 0.0, 0.0};
-      int j = 0;
-      for (; j + 32 <= n; j += 32) {
-        double mv[32];
 #pragma unroll
-        for (int u = 0; u < 32; ++u) mv[u] = Mi[(size_t)n * (j + u)];
+      for (int c = 0; c < NC; ++c) a[c & 3] += mreg[c] * qv[cj0 + c];
+      part[(wave >> 2) * 256 + ri] = (a[0] + a[1]) + (a[2] + a[3]);
+      __syncthreads();
+      for (int i = tid; i < n; i += NT) wv[i] = part[i] + part[256 + i];
+    } else {
+      // one thread per row i, lanes over consecutive rows, so each step j
+      // reads 64 consecutive entries of column j (coalesced); q_j is an LDS
+      // broadcast.  32 columns of loads in flight per thread.
+      for (int i = tid; i < n; i += NT) {
+        const double* Mi = M + i;
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        int j = 0;
+        for (; j + 32 <= n; j += 32) {
+          double mv[32];
 #pragma unroll
-        for (int u = 0; u < 32; ++u) a[u & 3] += mv[u] * qv[j + u];
+          for (int u = 0; u < 32; ++u) mv[u] = Mi[(size_t)n * (j + u)];
+#pragma unroll
+          for (int u = 0; u < 32; ++u) a[u & 3] += mv[u] * qv[j + u];
+        }
+        for (; j < n; ++j) a[j & 3] += Mi[(size_t)n * j] * qv[j];
+        wv[i] = (a[0] + a[1]) + (a[2] + a[3]);
       }
-      for (; j < n; ++j) a[j & 3] += Mi[(size_t)n * j] * qv[j];
-      wv[i] = (a[0] + a[1]) + (a[2] + a[3]);
     }
     __syncthreads();
     double a = 0.0;
-    for (int i = tid; i < n; i += kLzThreads) a += qv[i] * wv[i];
-    const double alpha = block_sum(a, red);
+    for (int i = tid; i < n; i += NT) a += qv[i] * wv[i];
+    const double alpha = block_sum<NT>(a, red);
     if (tid == 0) dg[k] = alpha;
-    for (int i = tid; i < n; i += kLzThreads) rv[i] = (wv[i] - qv[i] * alpha) - qold[i] * beta;
+    for (int i = tid; i < n; i += NT) rv[i] = (wv[i] - qv[i] * alpha) - qold[i] * beta;
     if (kk >= sqrt((double)k) || k == n - 1 || k > sqrt((double)n + 9)) {
       kk = 0;
       __syncthreads();
       {
         double th = 0.0, zl = 0.0;
-        tridiag_min_block(k + 1, dg, od, th, zl, kLzThreads, trisc);
+        tridiag_min_block(k + 1, dg, od, th, zl, NT, trisc);
         if (tid == 0) { sh[0] = th; sh[1] = zl; }
       }
       __syncthreads();
@@ -126,8 +167,8 @@ __global__ __launch_bounds__(kLzThreads) void k_lanczos(const double* __restrict
       error = sh[1];
     }
     double v2 = 0.0;
-    for (int i = tid; i < n; i += kLzThreads) v2 += rv[i] * rv[i];
-    beta = sqrt(block_sum(v2, red));
+    for (int i = tid; i < n; i += NT) v2 += rv[i] * rv[i];
+    beta = sqrt(block_sum<NT>(v2, red));
     if (tid == 0) od[k] = beta;
     ++k;
     ++kk;
@@ -176,8 +217,17 @@ int min_eig(Ctx* c, double* lamX, double* lamZ) {
     }
     hipLaunchKernelGGL(k_min_eig_small, dim3(nb, 2), dim3(64), 0, c->stream, c->d_w1, c->lpOff,
                        c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
-    hipLaunchKernelGGL(k_lanczos, dim3(nb, 2), dim3(kLzThreads), 0, c->stream, c->d_w1, c->lpOff,
-                       c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
+    int nlr = 0, nlz = 0;
+    for (int q = 0; q < nb; ++q) {
+      nlr += c->sdpN[q] > 20 && c->sdpN[q] <= kLrMaxN;
+      nlz += c->sdpN[q] > kLrMaxN;
+    }
+    if (nlr)
+      hipLaunchKernelGGL((k_lanczos<kLrThreads, kLrCols>), dim3(nb, 2), dim3(kLrThreads), 0,
+                         c->stream, c->d_w1, c->lpOff, c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
+    if (nlz)
+      hipLaunchKernelGGL((k_lanczos<kLzThreads, 0>), dim3(nb, 2), dim3(kLzThreads), 0, c->stream,
+                         c->d_w1, c->lpOff, c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
     SDPA_HIP_CHECK(hipGetLastError());
     std::vector<double> h(2 * nb);
     SDPA_HIP_CHECK(hipMemcpyAsync(h.data(), c->d_eig, sizeof(double) * 2 * nb,

@@ -194,6 +194,27 @@ def test_min_eig_matches_oracle():
     assert lz == pytest.approx(oz, rel=1e-7, abs=1e-10)
 
 
+@pytest.mark.parametrize("n", [21, 100, 200, 208, 209, 300])
+def test_min_eig_per_block_size(n):
+    # one SDP block: 21..208 take the Lanczos kernel with the matrix held in
+    # registers, larger blocks the one that streams it from memory
+    d = PR.multi_block(n_sdp_blocks=1, nmin=n, nmax=n, lp=0, m=4, density=0.05, seed=n)
+    rng = np.random.default_rng(n)
+    X, Z, _ = packed_random(d, rng, cond=50.0)
+    g = ctx_for(d)
+    g.set("X", X); g.set("Z", Z)
+    assert g.inv_chol() == 0
+    D = []
+    for _ in range(2):
+        b = rng.standard_normal((n, n))
+        D.append(((b + b.T) / 2).reshape(-1, order="F"))
+    g.set("Dx", D[0]); g.set("Dz", D[1])
+    lx, lz = g.min_eig()
+    P = O.Problem(d)
+    assert lx == pytest.approx(P.min_eig(g.get("invCholX"), D[0]), rel=1e-7, abs=1e-10)
+    assert lz == pytest.approx(P.min_eig(g.get("invCholZ"), D[1]), rel=1e-7, abs=1e-10)
+
+
 @pytest.mark.parametrize("name", ["example1", "example2"])
 def test_solver_reproduces_manual(golden_dir, name):
     import json, os
