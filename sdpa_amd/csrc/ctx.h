@@ -27,12 +27,25 @@ struct BlockTable {
   int* d_f1own = nullptr;  // local indices of F1/F2 owners (nF1)
   int* d_dcon = nullptr;   // global constraint of each dense pool column (ndense)
   double* d_dense = nullptr;  // n*n x ndense dense pool (column per dense A_k)
-  // f3cols blocks: one 16-byte record per constraint (con, estart, nzeff,
+  // blocks with n <= 256: one 16-byte record per constraint (con, estart, nzeff,
   // min(ecnt, 9) | form << 4 | a << 8 | b << 16 with (a, b) its entry when it
   // has one) and that entry's value (0 otherwise)
   int* d_rmeta = nullptr;
   double* d_rval = nullptr;
   std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart, h_f1own;
+};
+
+// one SDP block's F3 lists, for the kernel that takes all blocks at once
+struct F3Blk {
+  const int* con;
+  const int* estart;
+  const int* ecnt;
+  const int* nzeff;
+  const int* form;
+  const int4* rmeta;   // BlockTable::d_rmeta (n <= 256), else null
+  const double* rval;
+  long off;  // packed offset of the block in X, Z^-1
+  int n, ncon;
 };
 
 struct Ctx {
@@ -51,6 +64,13 @@ struct Ctx {
   int nF[3] = {0, 0, 0};
   int schurDense = 1;
   bool bmatStoreFirst = false;  // single block covering all constraints, all F3
+  // F3 of all SDP blocks by one launch (several blocks, m <= kF3MultiMaxM):
+  // per constraint j the (block, position in the block's list) pairs, in
+  // block order
+  int f3multi = 0;
+  F3Blk* d_f3blk = nullptr;
+  int* d_jptr = nullptr;  // m + 1
+  int* d_jent = nullptr;  // 2 ints per (block, position)
   std::vector<BlockTable> blk;
   // SDP sparse entry pool (local row/col, value)
   int* d_er = nullptr;
@@ -126,7 +146,9 @@ struct Ctx {
 };
 
 // schur.hip
+constexpr int kF3MultiMaxM = 8192;  // the all-blocks F3 kernel keeps a Schur column in LDS
 int make_bmat(Ctx* c);
+int f3multi_prepare(int m);  // LDS size of that kernel for m constraints
 // ops.hip
 enum DotCode {
   DOT_CX = 0, DOT_BY, DOT_CDX, DOT_BDY, DOT_XDZ, DOT_DXZ, DOT_DXDZ, DOT_XZ,

@@ -56,6 +56,145 @@ __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int*
   return ret;
 }
 
+constexpr int kMultiN = 64;  // blocks up to this size: rows of A_j's entries staged in LDS
+
+// One term of calF3 (f3_pair's plu) for the owner entry (a, b) and the
+// partner entry (g, d), given the eight values it reads: xag = X[a][g],
+// xad = X[a][d], xbg = X[b][g], xbd = X[b][d], and the same of Z^-1.
+__device__ __forceinline__ double f3_plu(bool ab, bool gd, double xag, double xad, double xbg,
+                                         double xbd, double zag, double zad, double zbg,
+                                         double zbd) {
+  if (ab) return gd ? zbd * xag + zbg * xad + zad * xbg + zag * xbd : zbd * xag + zad * xbg;
+  return gd ? zbd * xag + zbg * xad : zbd * xag;
+}
+
+// F3 of several SDP blocks in one launch: one work-group per Schur column j
+// (constraint j), the column kept in LDS (acc).  For each block that holds
+// data of A_j, in block order: the rows a, b of X and Z^-1 of each entry
+// (a, b) of A_j in the block are staged in LDS (blocks of n <= kMultiN), then
+// the threads walk the block's constraint list from j's position on (rows
+// i >= j: the lists are in increasing constraint order), one 16-byte record
+// per row, and add each light pair's value to acc[i].  The value follows
+// f3_pair: the same owner rule, loops and order of terms; every X and Z^-1
+// entry it reads comes from a staged row -- directly when j owns the pair,
+// through the symmetry X[u][v] = X[v][u] when i does.  Then the column's
+// lower part is stored once.  Against one launch per block this writes each
+// entry of B once, needs no zero-fill, and reads X and Z^-1 from LDS.  F1/F2
+// and heavy-pair terms are added afterwards.
+// Dynamic LDS: m doubles (acc) + kLight * 4 * kMultiN doubles (rows).
+__global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks,
+                                                  const int* __restrict__ jptr,
+                                                  const int* __restrict__ jent,
+                                                  const int* __restrict__ er,
+                                                  const int* __restrict__ ec,
+                                                  const double* __restrict__ ev,
+                                                  const double* __restrict__ X,
+                                                  const double* __restrict__ Zi,
+                                                  double* __restrict__ B, int m) {
+  extern __shared__ double acc[];
+  double* sR = acc + m;  // [e][4][kMultiN]: X row a_e, X row b_e, Z row a_e, Z row b_e
+  __shared__ int sja[kLight], sjb[kLight];
+  __shared__ double sjv[kLight];
+  const int j = blockIdx.x, tid = threadIdx.x;
+  for (int r = j + tid; r < m; r += 256) acc[r] = 0.0;
+  const int t1 = jptr[j + 1];
+  for (int t = jptr[j]; t < t1; ++t) {
+    const int q = jent[2 * t], pj = jent[2 * t + 1];
+    const F3Blk Bq = blks[q];
+    const int nzj = Bq.nzeff[pj], cj = Bq.ecnt[pj], sj = Bq.estart[pj], fj = Bq.form[pj];
+    if (cj > kLight) continue;  // every pair of this block is heavy (k_f3_heavy)
+    const int n = Bq.n;
+    const double* Xq = X + Bq.off;
+    const double* Zq = Zi + Bq.off;
+    const bool staged = n <= kMultiN && Bq.rmeta != nullptr;
+    __syncthreads();  // the previous pass is done with acc and the staged rows
+    if (staged) {
+      if (tid < cj) {
+        sja[tid] = er[sj + tid];
+        sjb[tid] = ec[sj + tid];
+        sjv[tid] = ev[sj + tid];
+      }
+      for (int x = tid; x < cj * 2 * n; x += 256) {
+        const int e = x / (2 * n), h = (x / n) & 1, g = x % n;  // h: row a or row b
+        const int row = h ? ec[sj + e] : er[sj + e];
+        sR[(e * 4 + h) * kMultiN + g] = Xq[(size_t)n * row + g];
+        sR[(e * 4 + 2 + h) * kMultiN + g] = Zq[(size_t)n * row + g];
+      }
+    }
+    __syncthreads();
+    for (int ki = pj + tid; ki < Bq.ncon; ki += 256) {
+      int ci, fi, nzi, si, crow;
+      int4 rec;
+      double vi = 0.0;
+      if (staged) {
+        rec = Bq.rmeta[ki];
+        vi = Bq.rval[ki];
+        crow = rec.x, si = rec.y, nzi = rec.z, ci = rec.w & 15, fi = (rec.w >> 4) & 3;
+      } else {
+        crow = Bq.con[ki], si = Bq.estart[ki], nzi = Bq.nzeff[ki], ci = Bq.ecnt[ki],
+        fi = Bq.form[ki];
+        rec = make_int4(0, 0, 0, 0);
+      }
+      const bool jo = !(nzi > nzj);  // owner: tie -> the smaller index (j)
+      const int co = jo ? cj : ci, cp = jo ? ci : cj, fo = jo ? fj : fi;
+      if (!(fo == 3 && co <= kLight && cp <= kLight)) continue;
+      double val;
+      if (!staged) {
+        val = jo ? f3_pair(er, ec, ev, sj, cj, si, ci, n, Xq, Zq, 0, 1)
+                 : f3_pair(er, ec, ev, si, ci, sj, cj, n, Xq, Zq, 0, 1);
+      } else if (jo) {
+        // owner j: outer over A_j's entries (staged rows a, b), inner over A_i's
+        val = 0.0;
+        for (int e1 = 0; e1 < cj; ++e1) {
+          const double* Xa = sR + (e1 * 4 + 0) * kMultiN;
+          const double* Xb = sR + (e1 * 4 + 1) * kMultiN;
+          const double* Za = sR + (e1 * 4 + 2) * kMultiN;
+          const double* Zb = sR + (e1 * 4 + 3) * kMultiN;
+          const bool ab = sja[e1] != sjb[e1];
+          double sum = 0.0;
+          for (int e2 = 0; e2 < ci; ++e2) {
+            const int g = ci == 1 ? (rec.w >> 8) & 255 : er[si + e2];
+            const int d = ci == 1 ? (rec.w >> 16) & 255 : ec[si + e2];
+            const double v2 = ci == 1 ? vi : ev[si + e2];
+            sum += v2 * f3_plu(ab, g != d, Xa[g], Xa[d], Xb[g], Xb[d], Za[g], Za[d], Zb[g], Zb[d]);
+          }
+          val += sjv[e1] * sum;
+        }
+      } else {
+        // owner i: outer over A_i's entries (a, b), inner over A_j's (g, d),
+        // X[a][g] read as X[g][a] from the staged row g, and so on
+        val = 0.0;
+        for (int e1 = 0; e1 < ci; ++e1) {
+          const int a = ci == 1 ? (rec.w >> 8) & 255 : er[si + e1];
+          const int b = ci == 1 ? (rec.w >> 16) & 255 : ec[si + e1];
+          const double v1 = ci == 1 ? vi : ev[si + e1];
+          double sum = 0.0;
+          for (int e2 = 0; e2 < cj; ++e2) {
+            const double* Xg = sR + (e2 * 4 + 0) * kMultiN;
+            const double* Xd = sR + (e2 * 4 + 1) * kMultiN;
+            const double* Zg = sR + (e2 * 4 + 2) * kMultiN;
+            const double* Zd = sR + (e2 * 4 + 3) * kMultiN;
+            sum += sjv[e2] * f3_plu(a != b, sja[e2] != sjb[e2], Xg[a], Xd[a], Xg[b], Xd[b], Zg[a],
+                                    Zd[a], Zg[b], Zd[b]);
+          }
+          val += v1 * sum;
+        }
+      }
+      acc[crow] += val;  // one thread per row in this pass
+    }
+  }
+  __syncthreads();
+  for (int r = j + tid; r < m; r += 256) B[(size_t)r + (size_t)m * j] = acc[r];
+}
+
+int f3multi_prepare(int m) {
+  if (m > kF3MultiMaxM) return -1;
+  SDPA_HIP_CHECK(hipFuncSetAttribute(
+      (const void*)k_f3_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)(sizeof(double) * ((size_t)kF3MultiMaxM + (size_t)kLight * 4 * kMultiN))));
+  return 0;
+}
+
 // One thread per pair (ki >= kj) of the block's constraint list.
 // grid = (ceil(ncon/64), ceil(ncon/4)), block = 256 (64 rows x 4 columns).
 __global__ __launch_bounds__(256) void k_f3_light(
@@ -548,8 +687,15 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
 int make_bmat(Ctx* c) {
   hipStream_t s = c->stream;
   const int m = c->m;
-  if (!c->bmatStoreFirst)
+  if (c->f3multi) {
+    // stores every entry of the lower triangle: the first writer of B
+    hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(256),
+                       sizeof(double) * ((size_t)m + (size_t)kLight * 4 * kMultiN), s,
+                       c->d_f3blk, c->d_jptr, c->d_jent, c->d_er, c->d_ec, c->d_ev, c->d_X,
+                       c->d_Zinv, c->d_B, m);
+  } else if (!c->bmatStoreFirst) {
     SDPA_HIP_CHECK(hipMemsetAsync(c->d_B, 0, sizeof(double) * (size_t)m * m, s));
+  }
   for (int q = 0; q < c->nSDP; ++q) {
     BlockTable& T = c->blk[q];
     if (T.ncon == 0) continue;
@@ -558,7 +704,9 @@ int make_bmat(Ctx* c) {
     if (T.nF1 > 0)
       if (int e = f1_block(c, T, X, Zi)) return e;
     if (T.nF3 > 0) {
-      if (T.f3cols && T.n <= kColsMaxN) {
+      if (c->f3multi) {
+        // light pairs done by k_f3_multi
+      } else if (T.f3cols && T.n <= kColsMaxN) {
         const int nblk = (T.ncon + kColsWG - 1) / kColsWG;
         const dim3 gc((nblk + 1) / 2, (T.ncon + kColsWG + kColsRows - 1) / kColsRows);
         hipLaunchKernelGGL(k_f3_cols, gc, dim3(256), 0, s, (const int4*)T.d_rmeta, T.d_rval,

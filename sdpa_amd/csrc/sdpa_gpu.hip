@@ -60,6 +60,7 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_Dx, (void**)&c->d_Dz, (void**)&c->d_dy, (void**)&c->d_rzinv,
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
+                   (void**)&c->d_f3blk, (void**)&c->d_jptr, (void**)&c->d_jent,
                    (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P, (void**)&c->d_hpart,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
@@ -289,7 +290,7 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     if (int e = upload(c, &T.d_heavy, heavy)) return e;
     if (int e = upload(c, &T.d_f1own, f1own)) return e;
     if (int e = upload(c, &T.d_dcon, dcon)) return e;
-    if (T.f3cols) {
+    if (n <= 256) {  // entry positions fit the record's 8-bit fields
       std::vector<int> rm(4 * (size_t)nc);
       std::vector<double> rv(nc, 0.0);
       for (int k = 0; k < nc; ++k) {
@@ -438,6 +439,34 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     }
   }
   c->bmatStoreFirst = (nS == 1 && c->nLP == 0 && c->blk[0].ncon == m && c->blk[0].nF1 == 0);
+  c->f3multi = 0;
+  {
+    long nf3 = 0;
+    for (int q = 0; q < nS; ++q) nf3 += c->blk[q].nF3;
+    if (nS > 1 && nf3 > 0 && m <= kF3MultiMaxM) {
+      std::vector<int> cnt(m + 1, 0);
+      for (int q = 0; q < nS; ++q)
+        for (int k : c->blk[q].h_con) ++cnt[k + 1];
+      for (int k = 0; k < m; ++k) cnt[k + 1] += cnt[k];
+      std::vector<int> jent(2 * (size_t)cnt[m]), fill(cnt.begin(), cnt.end() - 1);
+      std::vector<F3Blk> fb(nS);
+      for (int q = 0; q < nS; ++q) {
+        const BlockTable& T = c->blk[q];
+        for (int pos = 0; pos < T.ncon; ++pos) {
+          const int at = fill[T.h_con[pos]]++;
+          jent[2 * (size_t)at] = q;
+          jent[2 * (size_t)at + 1] = pos;
+        }
+        fb[q] = F3Blk{T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff, T.d_form,
+                      (const int4*)T.d_rmeta, T.d_rval, T.off, T.n, T.ncon};
+      }
+      if (int e = upload(c, &c->d_jptr, cnt)) return e;
+      if (int e = upload(c, &c->d_jent, jent)) return e;
+      if (int e = upload(c, &c->d_f3blk, fb)) return e;
+      if (int e = f3multi_prepare(m)) return e;
+      c->f3multi = 1;
+    }
+  }
 
   std::vector<double> hb(cvec, cvec + m);
   if (int e = upload(c, &c->d_b, hb)) return e;

@@ -75,6 +75,10 @@ CASES = {
     "dense_n70_m40": lambda: PR.dense_random(n=70, m=40, seed=6),
     "multiblock_lp": lambda: PR.multi_block(n_sdp_blocks=6, nmin=3, nmax=30, lp=12, m=40,
                                             density=0.05, seed=9),
+    # constraints spread over 12 blocks, with heavy lists (> 8 entries in a
+    # block) and dense F1 owners (small blocks) added after the all-blocks F3
+    "multiblock_wide": lambda: PR.multi_block(n_sdp_blocks=12, nmin=3, nmax=40, lp=30, m=300,
+                                              density=0.02, seed=11),
 }
 
 
