@@ -32,7 +32,7 @@ struct BlockTable {
   // has one) and that entry's value (0 otherwise)
   int* d_rmeta = nullptr;
   double* d_rval = nullptr;
-  std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart, h_f1own;
+  std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart, h_f1own, h_nzeff;
 };
 
 // one SDP block's F3 lists, for the kernel that takes all blocks at once
@@ -70,7 +70,9 @@ struct Ctx {
   int f3multi = 0;
   F3Blk* d_f3blk = nullptr;
   int* d_jptr = nullptr;  // m + 1
-  int* d_jent = nullptr;  // 2 ints per (block, position)
+  // per (block q, constraint j) one record: q | min(ecnt, 15) << 20 | form << 24,
+  // j's position in the block's list, estart, nzeff
+  int* d_jent = nullptr;
   std::vector<BlockTable> blk;
   // SDP sparse entry pool (local row/col, value)
   int* d_er = nullptr;

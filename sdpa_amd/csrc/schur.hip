@@ -57,6 +57,7 @@ __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int*
 }
 
 constexpr int kMultiN = 64;  // blocks up to this size: rows of A_j's entries staged in LDS
+constexpr int kMultiE = 4;   // ... when A_j has at most this many entries in the block
 
 // One term of calF3 (f3_pair's plu) for the owner entry (a, b) and the
 // partner entry (g, d), given the eight values it reads: xag = X[a][g],
@@ -81,7 +82,8 @@ __device__ __forceinline__ double f3_plu(bool ab, bool gd, double xag, double xa
 // lower part is stored once.  Against one launch per block this writes each
 // entry of B once, needs no zero-fill, and reads X and Z^-1 from LDS.  F1/F2
 // and heavy-pair terms are added afterwards.
-// Dynamic LDS: m doubles (acc) + kLight * 4 * kMultiN doubles (rows).
+// The next block's record and descriptor are loaded one pass ahead.
+// Dynamic LDS: m doubles (acc) + kMultiE * 4 * kMultiN doubles (rows).
 __global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks,
                                                   const int* __restrict__ jptr,
                                                   const int* __restrict__ jent,
@@ -93,20 +95,25 @@ __global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks
                                                   double* __restrict__ B, int m) {
   extern __shared__ double acc[];
   double* sR = acc + m;  // [e][4][kMultiN]: X row a_e, X row b_e, Z row a_e, Z row b_e
-  __shared__ int sja[kLight], sjb[kLight];
-  __shared__ double sjv[kLight];
+  __shared__ int sja[kMultiE], sjb[kMultiE];
+  __shared__ double sjv[kMultiE];
   const int j = blockIdx.x, tid = threadIdx.x;
   for (int r = j + tid; r < m; r += 256) acc[r] = 0.0;
-  const int t1 = jptr[j + 1];
-  for (int t = jptr[j]; t < t1; ++t) {
-    const int q = jent[2 * t], pj = jent[2 * t + 1];
-    const F3Blk Bq = blks[q];
-    const int nzj = Bq.nzeff[pj], cj = Bq.ecnt[pj], sj = Bq.estart[pj], fj = Bq.form[pj];
+  const int t0 = jptr[j], t1 = jptr[j + 1];
+  int4 jr = t0 < t1 ? reinterpret_cast<const int4*>(jent)[t0] : make_int4(0, 0, 0, 0);
+  F3Blk Bq = blks[jr.x & 0xfffff];
+  for (int t = t0; t < t1; ++t) {
+    const int pj = jr.y, sj = jr.z, nzj = jr.w, cj = (jr.x >> 20) & 15, fj = (jr.x >> 24) & 3;
+    const F3Blk Bc = Bq;
+    if (t + 1 < t1) {
+      jr = reinterpret_cast<const int4*>(jent)[t + 1];
+      Bq = blks[jr.x & 0xfffff];
+    }
     if (cj > kLight) continue;  // every pair of this block is heavy (k_f3_heavy)
-    const int n = Bq.n;
-    const double* Xq = X + Bq.off;
-    const double* Zq = Zi + Bq.off;
-    const bool staged = n <= kMultiN && Bq.rmeta != nullptr;
+    const int n = Bc.n;
+    const double* Xq = X + Bc.off;
+    const double* Zq = Zi + Bc.off;
+    const bool staged = n <= kMultiN && cj <= kMultiE && Bc.rmeta != nullptr;
     __syncthreads();  // the previous pass is done with acc and the staged rows
     if (staged) {
       if (tid < cj) {
@@ -122,22 +129,11 @@ __global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks
       }
     }
     __syncthreads();
-    for (int ki = pj + tid; ki < Bq.ncon; ki += 256) {
-      int ci, fi, nzi, si, crow;
-      int4 rec;
-      double vi = 0.0;
-      if (staged) {
-        rec = Bq.rmeta[ki];
-        vi = Bq.rval[ki];
-        crow = rec.x, si = rec.y, nzi = rec.z, ci = rec.w & 15, fi = (rec.w >> 4) & 3;
-      } else {
-        crow = Bq.con[ki], si = Bq.estart[ki], nzi = Bq.nzeff[ki], ci = Bq.ecnt[ki],
-        fi = Bq.form[ki];
-        rec = make_int4(0, 0, 0, 0);
-      }
+    // one row of the block's list: its pair with j, added to acc
+    auto add_row = [&](int crow, int si, int nzi, int ci, int fi, int4 rec, double vi) {
       const bool jo = !(nzi > nzj);  // owner: tie -> the smaller index (j)
       const int co = jo ? cj : ci, cp = jo ? ci : cj, fo = jo ? fj : fi;
-      if (!(fo == 3 && co <= kLight && cp <= kLight)) continue;
+      if (!(fo == 3 && co <= kLight && cp <= kLight)) return;
       double val;
       if (!staged) {
         val = jo ? f3_pair(er, ec, ev, sj, cj, si, ci, n, Xq, Zq, 0, 1)
@@ -181,6 +177,29 @@ __global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks
         }
       }
       acc[crow] += val;  // one thread per row in this pass
+    };
+    const int ncon = Bc.ncon;
+    if (staged) {
+      // four rows per thread per round, their records loaded together
+      // (clamped indices stay in the list)
+      for (int k0 = pj + tid; k0 < ncon; k0 += 4 * 256) {
+        int4 rq[4];
+        double vq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kl = min(k0 + 256 * u, ncon - 1);
+          rq[u] = Bc.rmeta[kl];
+          vq[u] = Bc.rval[kl];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (k0 + 256 * u < ncon)
+            add_row(rq[u].x, rq[u].y, rq[u].z, rq[u].w & 15, (rq[u].w >> 4) & 3, rq[u], vq[u]);
+      }
+    } else {
+      for (int ki = pj + tid; ki < ncon; ki += 256)
+        add_row(Bc.con[ki], Bc.estart[ki], Bc.nzeff[ki], Bc.ecnt[ki], Bc.form[ki],
+                make_int4(0, 0, 0, 0), 0.0);
     }
   }
   __syncthreads();
@@ -191,7 +210,7 @@ int f3multi_prepare(int m) {
   if (m > kF3MultiMaxM) return -1;
   SDPA_HIP_CHECK(hipFuncSetAttribute(
       (const void*)k_f3_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
-      (int)(sizeof(double) * ((size_t)kF3MultiMaxM + (size_t)kLight * 4 * kMultiN))));
+      (int)(sizeof(double) * ((size_t)kF3MultiMaxM + (size_t)kMultiE * 4 * kMultiN))));
   return 0;
 }
 
@@ -690,7 +709,7 @@ int make_bmat(Ctx* c) {
   if (c->f3multi) {
     // stores every entry of the lower triangle: the first writer of B
     hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(256),
-                       sizeof(double) * ((size_t)m + (size_t)kLight * 4 * kMultiN), s,
+                       sizeof(double) * ((size_t)m + (size_t)kMultiE * 4 * kMultiN), s,
                        c->d_f3blk, c->d_jptr, c->d_jent, c->d_er, c->d_ec, c->d_ev, c->d_X,
                        c->d_Zinv, c->d_B, m);
   } else if (!c->bmatStoreFirst) {

@@ -312,6 +312,7 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     T.h_f1own = f1own;
     T.h_ecnt = ecnt;
     T.h_estart = estart;
+    T.h_nzeff = nzeff;
     if (T.ndense > 0) {
       const size_t nn2 = (size_t)n * n;
       std::vector<double> pool(nn2 * T.ndense, 0.0);
@@ -443,19 +444,21 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   {
     long nf3 = 0;
     for (int q = 0; q < nS; ++q) nf3 += c->blk[q].nF3;
-    if (nS > 1 && nf3 > 0 && m <= kF3MultiMaxM) {
+    if (nS > 1 && nS < (1 << 20) && nf3 > 0 && m <= kF3MultiMaxM) {
       std::vector<int> cnt(m + 1, 0);
       for (int q = 0; q < nS; ++q)
         for (int k : c->blk[q].h_con) ++cnt[k + 1];
       for (int k = 0; k < m; ++k) cnt[k + 1] += cnt[k];
-      std::vector<int> jent(2 * (size_t)cnt[m]), fill(cnt.begin(), cnt.end() - 1);
+      std::vector<int> jent(4 * (size_t)cnt[m]), fill(cnt.begin(), cnt.end() - 1);
       std::vector<F3Blk> fb(nS);
       for (int q = 0; q < nS; ++q) {
         const BlockTable& T = c->blk[q];
         for (int pos = 0; pos < T.ncon; ++pos) {
           const int at = fill[T.h_con[pos]]++;
-          jent[2 * (size_t)at] = q;
-          jent[2 * (size_t)at + 1] = pos;
+          jent[4 * (size_t)at] = q | (std::min(T.h_ecnt[pos], 15) << 20) | (T.h_form[pos] << 24);
+          jent[4 * (size_t)at + 1] = pos;
+          jent[4 * (size_t)at + 2] = T.h_estart[pos];
+          jent[4 * (size_t)at + 3] = T.h_nzeff[pos];
         }
         fb[q] = F3Blk{T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff, T.d_form,
                       (const int4*)T.d_rmeta, T.d_rval, T.off, T.n, T.ncon};
