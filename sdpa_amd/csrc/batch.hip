@@ -592,51 +592,7 @@ __device__ __forceinline__ void make_bmat(int n, int m, const double* X, const d
 // multisection on the whole work-group.
 template <int BT>
 __device__ __forceinline__ double small_min_eig(int n, Shared<BT>& sh) {
-  double* A = sh.A;
-  if (threadIdx.x < 64) {
-    const int l = threadIdx.x;
-    for (int j = 0; j < n; ++j)
-      if (l < j) A[l + LDA * j] = A[j + LDA * l];
-    for (int k = 0; k + 2 < n; ++k) {
-      const bool act = l > k && l < n;
-      const double xl = act ? A[l + LDA * k] : 0.0;
-      const double x0 = rdlane(xl, k + 1);
-      const double tail = wsum((l > k + 1 && l < n) ? xl * xl : 0.0);
-      if (tail == 0.0) {  // already tridiagonal in this column
-        if (l == 0) {
-          sh.dg[k] = A[k + LDA * k];
-          sh.od[k] = x0;
-        }
-        continue;
-      }
-      const double alpha = -copysign(sqrt(x0 * x0 + tail), x0);
-      const double vl = (l == k + 1) ? x0 - alpha : xl;
-      const double tau = 2.0 / (tail + (x0 - alpha) * (x0 - alpha));
-      double pl = 0.0;
-      for (int j = k + 1; j < n; ++j) {
-        const double vj = rdlane(vl, j);
-        if (act) pl += A[l + LDA * j] * vj;
-      }
-      pl *= tau;
-      const double K = 0.5 * tau * wsum(pl * vl);
-      const double wl = pl - K * vl;
-      for (int j = k + 1; j < n; ++j) {
-        const double vj = rdlane(vl, j), wj = rdlane(wl, j);
-        if (act) A[l + LDA * j] -= vl * wj + wl * vj;
-      }
-      if (l == 0) {
-        sh.dg[k] = A[k + LDA * k];
-        sh.od[k] = alpha;
-      }
-    }
-    if (l == 0) {
-      if (n >= 2) {
-        sh.dg[n - 2] = A[(n - 2) + LDA * (n - 2)];
-        sh.od[n - 2] = A[(n - 1) + LDA * (n - 2)];
-      }
-      sh.dg[n - 1] = A[(n - 1) + LDA * (n - 1)];
-    }
-  }
+  if (threadIdx.x < 64) householder_tridiag_wave(n, sh.A, LDA, sh.dg, sh.od);
   __syncthreads();
   double th = 0.0, zl = 0.0;
   tridiag_min_block(n, sh.dg, sh.od, th, zl, BT, sh.trisc, false);

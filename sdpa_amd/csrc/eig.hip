@@ -12,8 +12,9 @@
 //            52-56, result min - |err*beta|.  The tridiagonal problem is
 //            solved by Sturm multisection with inverse iteration for the last
 //            eigenvector component (tridiag_min_block, eig_dev.h);
-//   n <= 20: one wave finds all eigenvalues of M by cyclic Jacobi (the dsyev
-//            path, sdpa_linear.cpp:145-182).
+//   n <= 20: one wave reduces M to tridiagonal form (Householder) and takes
+//            its smallest eigenvalue by Sturm multisection (the dsyev path,
+//            sdpa_linear.cpp:145-182, which needs only the smallest).
 #include "ctx.h"
 #include "eig_dev.h"
 
@@ -24,7 +25,8 @@ namespace sdpa_amd {
 constexpr int kMaxLanczos = 64;   // tridiagonal size limit (sqrt(n)+10 <= 64 for n <= 2916)
 constexpr int kMaxN = 2048;       // largest block the single-wave kernel holds in LDS vectors
 
-// n <= 20: one wave per (block, which), cyclic Jacobi on M = L D L^T.
+// n <= 20: one wave per (block, which): Householder reduction of M = L D L^T
+// to tridiagonal form and its smallest eigenvalue by Sturm multisection.
 // Ms: packed scaled directions for X (which 0) and Z (which 1), at offsets 0
 // and `stride`.
 __global__ __launch_bounds__(64) void k_min_eig_small(const double* __restrict__ Ms, long stride,
@@ -37,15 +39,17 @@ __global__ __launch_bounds__(64) void k_min_eig_small(const double* __restrict__
   const double* M = Ms + which * stride + off[q];
   const int lane = threadIdx.x;
   __shared__ double Zs[64 * 65];
+  __shared__ double dg[64], od[64], trisc[2];
   for (int j = 0; j < n; ++j)
     if (lane < n) Zs[lane + 65 * j] = M[lane + (size_t)n * j];
-  // dsyev('L') reads the lower triangle: mirror it
   __syncthreads();
-  for (int j = 0; j < n; ++j)
-    if (lane < n && lane < j) Zs[lane + 65 * j] = Zs[j + 65 * lane];
+  // dsyev('L') reads the lower triangle (mirrored inside); its smallest
+  // eigenvalue through the tridiagonal form
+  householder_tridiag_wave(n, Zs, 65, dg, od);
   __syncthreads();
-  const double result = jacobi_min_wave(n, Zs);
-  if (lane == 0) out[which * nblk + q] = result;
+  double th = 0.0, zl = 0.0;
+  tridiag_min_block(n, dg, od, th, zl, 64, trisc, false);
+  if (lane == 0) out[which * nblk + q] = th;
 }
 
 constexpr int kLzThreads = 256;  // global-memory matrix-vector product (n > kLrMaxN)

@@ -2,8 +2,8 @@
 // and the batched small-SDP solver (batch.hip): wave sums, an FP64
 // reciprocal, the smallest eigenvalue of a symmetric tridiagonal matrix with
 // the last component of its eigenvector (what the reference reads from
-// dsteqr, sdpa_linear.cpp:98-127), and cyclic Jacobi for small blocks (the
-// dsyev path, sdpa_linear.cpp:145-182).
+// dsteqr, sdpa_linear.cpp:98-127), and the Householder reduction to
+// tridiagonal form for small blocks (the dsyev path, sdpa_linear.cpp:145-182).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -187,49 +187,56 @@ static __device__ __forceinline__ void tridiag_min_block(int N, const double* d,
   }
 }
 
-// Smallest eigenvalue of a symmetric n x n (n <= 64) matrix in LDS (ld 65),
-// cyclic Jacobi; lane k updates row/column k.
-static __device__ __forceinline__ double jacobi_min_wave(int n, double* A) {
-  const int lane = threadIdx.x;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, dg = 0.0;
-    for (int j = 0; j < n; ++j) {
-      double v = (lane < n && lane > j) ? A[lane + 65 * j] : 0.0;
-      off += v * v;
-    }
-    off = wsum(off);
-    double dv = lane < n ? A[lane + 65 * lane] : 0.0;
-    dg = wsum(dv * dv);
-    // converged when the off-diagonal norm is below eps times the diagonal's
-    // (a tighter test is never met in FP64 and ran every call to the sweep cap)
-    if (off == 0.0 || off <= 1e-31 * dg) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        __syncthreads();
-        double apq = A[p + 65 * q];
-        if (apq == 0.0) continue;
-        double app = A[p + 65 * p], aqq = A[q + 65 * q];
-        double theta = (aqq - app) / (2.0 * apq);
-        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        __syncthreads();
-        if (lane < n) {
-          double akp = A[lane + 65 * p], akq = A[lane + 65 * q];
-          A[lane + 65 * p] = c * akp - s * akq;
-          A[lane + 65 * q] = s * akp + c * akq;
-        }
-        __syncthreads();
-        if (lane < n) {
-          double apk = A[p + 65 * lane], aqk = A[q + 65 * lane];
-          A[p + 65 * lane] = c * apk - s * aqk;
-          A[q + 65 * lane] = s * apk + c * aqk;
-        }
+// Householder reduction of the symmetric n x n matrix A (LDS, leading
+// dimension lda; the lower triangle is read and mirrored first) to
+// tridiagonal form: diagonal dg[0..n), off-diagonal od[0..n-1).  One wave,
+// lane l holds row l (n <= 64); A is overwritten.  Columns already
+// tridiagonal are skipped.
+static __device__ __forceinline__ void householder_tridiag_wave(int n, double* A, int lda,
+                                                                double* dg, double* od) {
+  const int l = threadIdx.x & 63;
+  for (int j = 0; j < n; ++j)
+    if (l < j) A[l + lda * j] = A[j + lda * l];
+  for (int k = 0; k + 2 < n; ++k) {
+    const bool act = l > k && l < n;
+    const double xl = act ? A[l + lda * k] : 0.0;
+    const double x0 = rdlane(xl, k + 1);
+    const double tail = wsum((l > k + 1 && l < n) ? xl * xl : 0.0);
+    if (tail == 0.0) {  // already tridiagonal in this column
+      if (l == 0) {
+        dg[k] = A[k + lda * k];
+        od[k] = x0;
       }
+      continue;
+    }
+    const double alpha = -copysign(sqrt(x0 * x0 + tail), x0);
+    const double vl = (l == k + 1) ? x0 - alpha : xl;
+    const double tau = 2.0 / (tail + (x0 - alpha) * (x0 - alpha));
+    double pl = 0.0;
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = rdlane(vl, j);
+      if (act) pl += A[l + lda * j] * vj;
+    }
+    pl *= tau;
+    const double K = 0.5 * tau * wsum(pl * vl);
+    const double wl = pl - K * vl;
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = rdlane(vl, j), wj = rdlane(wl, j);
+      if (act) A[l + lda * j] -= vl * wj + wl * vj;
+    }
+    if (l == 0) {
+      dg[k] = A[k + lda * k];
+      od[k] = alpha;
+    }
   }
-  __syncthreads();
-  double dv = lane < n ? A[lane + 65 * lane] : 1.0e300;
-  dv = wmin(dv);
-  return dv;
+  if (l == 0) {
+    if (n >= 2) {
+      dg[n - 2] = A[(n - 2) + lda * (n - 2)];
+      od[n - 2] = A[(n - 1) + lda * (n - 2)];
+    }
+    dg[n - 1] = A[(n - 1) + lda * (n - 1)];
+  }
 }
+
 
 }  // namespace sdpa_amd

@@ -198,10 +198,11 @@ def test_min_eig_matches_oracle():
     assert lz == pytest.approx(oz, rel=1e-7, abs=1e-10)
 
 
-@pytest.mark.parametrize("n", [21, 100, 200, 208, 209, 300])
+@pytest.mark.parametrize("n", [2, 7, 20, 21, 100, 200, 208, 209, 300])
 def test_min_eig_per_block_size(n):
-    # one SDP block: 21..208 take the Lanczos kernel with the matrix held in
-    # registers, larger blocks the one that streams it from memory
+    # one SDP block: up to 20 the tridiagonal reduction (the dsyev branch),
+    # 21..208 the Lanczos kernel with the matrix held in registers, larger
+    # blocks the one that streams it from memory
     d = PR.multi_block(n_sdp_blocks=1, nmin=n, nmax=n, lp=0, m=4, density=0.05, seed=n)
     rng = np.random.default_rng(n)
     X, Z, _ = packed_random(d, rng, cond=50.0)
