@@ -56,9 +56,11 @@ def block_features(d: PR.SdpData) -> np.ndarray:
     a2 = np.zeros(nb)
     np.add.at(c2, d.l[d.k == 0] - 1, sq[d.k == 0])
     np.add.at(a2, d.l[d.k > 0] - 1, sq[d.k > 0])
-    ncon = np.zeros(nb)
-    for l in range(nb):
-        ncon[l] = len(np.unique(d.k[(d.l == l + 1) & (d.k > 0)]))
+    # distinct constraints with data in each block: one pass over the (block,
+    # constraint) keys
+    sel = d.k > 0
+    keys = np.unique((d.l[sel].astype(np.int64) - 1) * (d.m + 1) + d.k[sel])
+    ncon = np.bincount(keys // (d.m + 1), minlength=nb).astype(float)
     b2 = float(np.linalg.norm(d.c))
     binf = float(np.max(np.abs(d.c))) if d.m else 0.0
     F = np.zeros((nb, NIN))
