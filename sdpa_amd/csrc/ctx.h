@@ -64,9 +64,10 @@ struct Ctx {
   int nF[3] = {0, 0, 0};
   int schurDense = 1;
   bool bmatStoreFirst = false;  // single block covering all constraints, all F3
-  // F3 of all SDP blocks by one launch (several blocks, m <= kF3MultiMaxM):
-  // per constraint j the (block, position in the block's list) pairs, in
-  // block order
+  // F3 of the SDP blocks of n <= kMultiN by one launch (several blocks,
+  // m <= kF3MultiMaxM; larger blocks keep their per-block kernels): per
+  // constraint j the (block, position in the block's list) pairs, in block
+  // order
   int f3multi = 0;
   F3Blk* d_f3blk = nullptr;
   int* d_jptr = nullptr;  // m + 1
@@ -149,6 +150,7 @@ struct Ctx {
 
 // schur.hip
 constexpr int kF3MultiMaxM = 8192;  // the all-blocks F3 kernel keeps a Schur column in LDS
+constexpr int kMultiN = 64;         // ... and takes the blocks up to this size
 int make_bmat(Ctx* c);
 int f3multi_prepare(int m);  // LDS size of that kernel for m constraints
 // ops.hip

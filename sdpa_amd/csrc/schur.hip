@@ -56,8 +56,9 @@ __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int*
   return ret;
 }
 
-constexpr int kMultiN = 64;  // blocks up to this size: rows of A_j's entries staged in LDS
-constexpr int kMultiE = 4;   // ... when A_j has at most this many entries in the block
+// k_f3_multi stages the rows of A_j's entries in LDS (blocks of n <= kMultiN)
+// when A_j has at most kMultiE entries in the block
+constexpr int kMultiE = 4;
 
 // One term of calF3 (f3_pair's plu) for the owner entry (a, b) and the
 // partner entry (g, d), given the eight values it reads: xag = X[a][g],
@@ -723,7 +724,7 @@ int make_bmat(Ctx* c) {
     if (T.nF1 > 0)
       if (int e = f1_block(c, T, X, Zi)) return e;
     if (T.nF3 > 0) {
-      if (c->f3multi) {
+      if (c->f3multi && T.n <= kMultiN) {
         // light pairs done by k_f3_multi
       } else if (T.f3cols && T.n <= kColsMaxN) {
         const int nblk = (T.ncon + kColsWG - 1) / kColsWG;

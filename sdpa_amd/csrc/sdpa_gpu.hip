@@ -442,17 +442,22 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   c->bmatStoreFirst = (nS == 1 && c->nLP == 0 && c->blk[0].ncon == m && c->blk[0].nF1 == 0);
   c->f3multi = 0;
   {
+    // the blocks it takes: n <= kMultiN (the others keep their per-block
+    // kernels, which then add to B)
     long nf3 = 0;
-    for (int q = 0; q < nS; ++q) nf3 += c->blk[q].nF3;
+    for (int q = 0; q < nS; ++q)
+      if (c->blk[q].n <= kMultiN) nf3 += c->blk[q].nF3;
     if (nS > 1 && nS < (1 << 20) && nf3 > 0 && m <= kF3MultiMaxM) {
       std::vector<int> cnt(m + 1, 0);
       for (int q = 0; q < nS; ++q)
-        for (int k : c->blk[q].h_con) ++cnt[k + 1];
+        if (c->blk[q].n <= kMultiN)
+          for (int k : c->blk[q].h_con) ++cnt[k + 1];
       for (int k = 0; k < m; ++k) cnt[k + 1] += cnt[k];
       std::vector<int> jent(4 * (size_t)cnt[m]), fill(cnt.begin(), cnt.end() - 1);
       std::vector<F3Blk> fb(nS);
       for (int q = 0; q < nS; ++q) {
         const BlockTable& T = c->blk[q];
+        if (T.n > kMultiN) continue;
         for (int pos = 0; pos < T.ncon; ++pos) {
           const int at = fill[T.h_con[pos]]++;
           jent[4 * (size_t)at] = q | (std::min(T.h_ecnt[pos], 15) << 20) | (T.h_form[pos] << 24);

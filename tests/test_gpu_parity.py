@@ -79,6 +79,9 @@ CASES = {
     # block) and dense F1 owners (small blocks) added after the all-blocks F3
     "multiblock_wide": lambda: PR.multi_block(n_sdp_blocks=12, nmin=3, nmax=40, lp=30, m=300,
                                               density=0.02, seed=11),
+    # blocks above 64 (per-block kernels adding to B) beside small ones
+    "multiblock_sizes": lambda: PR.multi_block(n_sdp_blocks=6, nmin=10, nmax=120, lp=10, m=150,
+                                               density=0.01, seed=13),
 }
 
 
