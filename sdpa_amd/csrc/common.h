@@ -84,6 +84,7 @@ int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, cons
 // pipelined kernels do not apply to this size (m <= 512 included).
 int chol_solve_f32(hipStream_t s, DenseWork* w, const float* L, int ld, int m, const double* Winv,
                    double* x);
+bool chol_solve_f32_supported(int m, int ld);
 
 struct DenseWork {
   GemmItem* d_items = nullptr;  // device array of descriptors

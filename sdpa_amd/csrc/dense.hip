@@ -2129,6 +2129,18 @@ int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, cons
   return 0;
 }
 
+// Whether chol_solve_f32 applies to an m x m float factor with leading
+// dimension ld (the pipelined kernels' size range, cooperative launch).
+bool chol_solve_f32_supported(int m, int ld) {
+  const int nb = (m + SB - 1) / SB;
+  if (m <= 2 * SB || nb > 48 || (size_t)ld * m * sizeof(float) >= (size_t)1 << 31) return false;
+  int dev = 0, coop = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess)
+    return false;
+  return coop != 0;
+}
+
 // x := L^-T L^-1 x with a float factor (the mixed-precision path); the
 // pipelined cooperative kernels only.  Returns 1 when they do not apply.
 int chol_solve_f32(hipStream_t s, DenseWork* w, const float* L, int ld, int m, const double* Winv,

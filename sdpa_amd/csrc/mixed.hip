@@ -199,7 +199,9 @@ __global__ __launch_bounds__(1024) void k_mx_anorm(const double* __restrict__ y,
 
 int factor_mixed(Ctx* c) {
   const int m = c->m;
-  if (m <= 512) return 1;  // below the pipelined solves' range
+  // outside the float solves' range (m <= 512, or more than 48 blocks of 256
+  // rows): the FP64 factorization
+  if (m <= 512 || !chol_solve_f32_supported(m, m)) return 1;
   if (int e = ensure_mixed(c)) return e;
   hipStream_t s = c->stream;
   if (int e = potrf_mixed(s, &c->dw, c->d_B, m, m, c->d_B32, c->d_scale, c->d_P64, c->d_D64,

@@ -112,3 +112,19 @@ def test_mixed_full_solve_matches_oracle(n):
     assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)
     assert r.relative_gap <= 1e-7
     assert r.primal_error <= 1e-7 and r.dual_error <= 1e-7
+
+
+def test_mixed_beyond_float_solve_range_uses_fp64():
+    # m = 13464 (Lovasz n = 300): more than 48 blocks of 256 rows, beyond the
+    # pipelined float solves; the mixed factorization falls back to FP64 (and
+    # the run stays there), so the run matches the FP64 run (two iterations)
+    d = PR.lovasz(300, 0.3, 2)
+    s = sa.Solver(0).load(d).prepare()
+    r0 = s.set_param(precision=0, maxIteration=2).run()
+    r1 = s.set_param(precision=1, maxIteration=2).run()
+    st = s.mixed_stats()
+    print(d.m, st, r0.primal_obj, r1.primal_obj)
+    assert d.m == 13464
+    assert r1.iterations == r0.iterations == 2
+    assert st["factors"] == 0 and st["fallbacks"] >= 1
+    assert r1.primal_obj == r0.primal_obj and r1.dual_obj == r0.dual_obj
