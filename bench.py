@@ -19,10 +19,14 @@ HBM).  The JSON line also carries:
     same workload (its first IPM iterations), scaled to the GPU's iteration
     count;
   - "batched": 4096 small MaxCut SDPs in one persistent launch (solves/s,
-    phase counts, gap histogram, its own CPU-oracle sample).
+    phase counts, gap histogram, its own CPU-oracle sample);
+  - "lambda_star": the 200-block multi-block SDP (m = 5000) solved from the
+    default lambda* = 100 and from the learned per-block lambda* (iterations,
+    seconds per solve including feature extraction and prediction).
 
 Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config lovasz|maxcut|dense|multiblock]
+  python bench.py --no-lambda               # without the learned-lambda* comparison
   python bench.py --copy-calibration        # one 256 MiB device copy (for PMC scaling)
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR   # print the traffic summary
 """
@@ -75,6 +79,38 @@ def cpu_baseline(d, gpu_iterations, sample_iters=2):
             "sample": f"the first {done} of {gpu_iterations} IPM iterations of the same instance "
                       f"({t:.2f} s wall incl. setup, FP64 C oracle with OpenMP), time per iteration "
                       f"x {gpu_iterations}"}
+
+
+def lambda_star(steps):
+    """Default lambda* against the learned per-block lambda* on the 200-block
+    multi-block SDP (200 blocks of 5-50, LP 1000, m = 5000, seed 5): best of
+    `steps` timed solves each after one untimed, through lambda_mlp.solve (the
+    learned start falls back to the default if it does not reach pdOPT)."""
+    import sdpa_amd
+    from sdpa_amd import lambda_mlp as LM
+    from sdpa_amd import problems as PR
+    if not os.path.exists(LM.DEFAULT_WEIGHTS):
+        return {"value": None, "error": "no trained weights"}
+    d = PR.multi_block()
+    model = LM.LambdaModel.load()
+    s = sdpa_amd.Solver(0).load(d).prepare()
+    out = {"workload": "multiblock_200x[5,50]_lp1000_m5000_seed5"}
+    for name, mdl in (("default", None), ("learned", model)):
+        LM.solve(d, mdl, s)
+        best, r, lam = None, None, None
+        for _ in range(max(1, steps)):
+            t0 = time.perf_counter()
+            r, lam = LM.solve(d, mdl, s)
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+        out[name] = {"seconds": best, "iterations": int(r.iterations), "phase": r.phase,
+                     "relative_gap": r.relative_gap, "primal_obj": r.primal_obj,
+                     "fallback": bool(mdl is not None and lam is None)}
+        if mdl is not None and lam is not None:
+            out[name]["lambda_range"] = [float(lam.min()), float(lam.max())]
+    s.close()
+    out["speedup"] = out["default"]["seconds"] / out["learned"]["seconds"]
+    return out
 
 
 def batched(steps, warmup, with_cpu):
@@ -219,6 +255,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="lovasz", choices=list(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-lambda", action="store_true",
+                    help="skip the learned-lambda* comparison (kernel profiles of the headline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, one 256 MiB device copy (PMC scaling)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
@@ -293,6 +331,14 @@ def main():
     log("bench: batched workload")
     batch = batched(args.steps, args.warmup, not args.no_cpu_baseline)
     log(f"bench: batched {batch['value']:.0f} solves/s ({batch['kernel_ms']:.1f} ms kernel)")
+    lam = None
+    if not args.no_lambda:
+        log("bench: learned lambda* on the 200-block configuration")
+        try:
+            lam = lambda_star(args.steps)
+        except Exception as e:  # reported beside the headline, never part of it
+            lam = {"value": None, "error": str(e)}
+        log(f"bench: lambda* {json.dumps(lam)[:200]}")
     its = rt.iterations
     m = d.m
     # F3 bytes per assembly (lower triangle written): 8*T + 16*sum(nnz) + 16*n^2
@@ -333,6 +379,7 @@ def main():
                      "TFLOPs": m ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None},
         "mixed_vs_fp64": mixed,
         "batched": batch,
+        "lambda_star": lam,
         "roofline": roof,
     }
     if not args.no_cpu_baseline:
