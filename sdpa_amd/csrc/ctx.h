@@ -27,6 +27,7 @@ struct BlockTable {
   int* d_f1own = nullptr;  // local indices of F1/F2 owners (nF1)
   int* d_dcon = nullptr;   // global constraint of each dense pool column (ndense)
   double* d_dense = nullptr;  // n*n x ndense dense pool (column per dense A_k)
+  int* d_dlist = nullptr;     // local index of each dense pool column's constraint (ndense)
   // blocks with n <= 256: one 16-byte record per constraint (con, estart, nzeff,
   // min(ecnt, 9) | form << 4 | a << 8 | b << 16 with (a, b) its entry when it
   // has one) and that entry's value (0 otherwise)

@@ -621,17 +621,9 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
       c->capF1C = needP;
     }
   }
-  // dense-member list (local indices of dense constraints, in pool order)
-  static thread_local std::vector<int> dl;
-  int* d_dlist = nullptr;
-  if (T.ndense > 0) {
-    dl.assign(T.ndense, 0);
-    for (int k = 0; k < T.ncon; ++k)
-      if (T.h_dpos[k] >= 0) dl[T.h_dpos[k]] = k;
-    SDPA_HIP_CHECK(hipMallocAsync((void**)&d_dlist, sizeof(int) * T.ndense, c->stream));
-    SDPA_HIP_CHECK(hipMemcpyAsync(d_dlist, dl.data(), sizeof(int) * T.ndense,
-                                  hipMemcpyHostToDevice, c->stream));
-  }
+  // dense-member list (local indices of dense constraints, in pool order),
+  // built at load time
+  const int* d_dlist = T.d_dlist;
   for (int first = 0; first < T.nF1; first += chunk) {
     const int cnt = std::min(chunk, T.nF1 - first);
     // the chunk's A_t: straight from the dense pool when its owners are
@@ -699,7 +691,6 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
                          c->d_er, c->d_ec, c->d_ev, T.ncon, n, G, c->d_B, c->m);
     // the next chunk rewrites the descriptor slots and scratch: stream order keeps it safe
   }
-  if (d_dlist) SDPA_HIP_CHECK(hipFreeAsync(d_dlist, c->stream));
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
 }

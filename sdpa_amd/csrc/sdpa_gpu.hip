@@ -47,7 +47,8 @@ void free_problem(Ctx* c) {
   for (auto& T : c->blk) {
     for (void* p : {(void*)T.d_con, (void*)T.d_estart, (void*)T.d_ecnt, (void*)T.d_nzeff,
                     (void*)T.d_form, (void*)T.d_dpos, (void*)T.d_heavy, (void*)T.d_f1own,
-                    (void*)T.d_dcon, (void*)T.d_dense, (void*)T.d_rmeta, (void*)T.d_rval})
+                    (void*)T.d_dcon, (void*)T.d_dense, (void*)T.d_rmeta, (void*)T.d_rval,
+                    (void*)T.d_dlist})
       if (p) (void)hipFree(p);
   }
   c->blk.clear();
@@ -314,6 +315,10 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     T.h_estart = estart;
     T.h_nzeff = nzeff;
     if (T.ndense > 0) {
+      std::vector<int> dl(T.ndense, 0);
+      for (int k = 0; k < nc; ++k)
+        if (dpos[k] >= 0) dl[dpos[k]] = k;
+      if (int e = upload(c, &T.d_dlist, dl)) return e;
       const size_t nn2 = (size_t)n * n;
       std::vector<double> pool(nn2 * T.ndense, 0.0);
       for (int k = 0; k < nc; ++k) {
