@@ -59,6 +59,7 @@ __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int*
 // k_f3_multi stages the rows of A_j's entries in LDS (blocks of n <= kMultiN)
 // when A_j has at most kMultiE entries in the block
 constexpr int kMultiE = 4;
+constexpr int kMultiT = 512;  // threads per work-group (one Schur column)
 
 // One term of calF3 (f3_pair's plu) for the owner entry (a, b) and the
 // partner entry (g, d), given the eight values it reads: xag = X[a][g],
@@ -85,7 +86,7 @@ __device__ __forceinline__ double f3_plu(bool ab, bool gd, double xag, double xa
 // and heavy-pair terms are added afterwards.
 // The next block's record and descriptor are loaded one pass ahead.
 // Dynamic LDS: m doubles (acc) + kMultiE * 4 * kMultiN doubles (rows).
-__global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks,
+__global__ __launch_bounds__(kMultiT) void k_f3_multi(const F3Blk* __restrict__ blks,
                                                   const int* __restrict__ jptr,
                                                   const int* __restrict__ jent,
                                                   const int* __restrict__ er,
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks
   __shared__ int sja[kMultiE], sjb[kMultiE];
   __shared__ double sjv[kMultiE];
   const int j = blockIdx.x, tid = threadIdx.x;
-  for (int r = j + tid; r < m; r += 256) acc[r] = 0.0;
+  for (int r = j + tid; r < m; r += kMultiT) acc[r] = 0.0;
   const int t0 = jptr[j], t1 = jptr[j + 1];
   int4 jr = t0 < t1 ? reinterpret_cast<const int4*>(jent)[t0] : make_int4(0, 0, 0, 0);
   F3Blk Bq = blks[jr.x & 0xfffff];
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks
         sjb[tid] = ec[sj + tid];
         sjv[tid] = ev[sj + tid];
       }
-      for (int x = tid; x < cj * 2 * n; x += 256) {
+      for (int x = tid; x < cj * 2 * n; x += kMultiT) {
         const int e = x / (2 * n), h = (x / n) & 1, g = x % n;  // h: row a or row b
         const int row = h ? ec[sj + e] : er[sj + e];
         sR[(e * 4 + h) * kMultiN + g] = Xq[(size_t)n * row + g];
@@ -183,28 +184,28 @@ __global__ __launch_bounds__(256) void k_f3_multi(const F3Blk* __restrict__ blks
     if (staged) {
       // four rows per thread per round, their records loaded together
       // (clamped indices stay in the list)
-      for (int k0 = pj + tid; k0 < ncon; k0 += 4 * 256) {
+      for (int k0 = pj + tid; k0 < ncon; k0 += 4 * kMultiT) {
         int4 rq[4];
         double vq[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int kl = min(k0 + 256 * u, ncon - 1);
+          const int kl = min(k0 + kMultiT * u, ncon - 1);
           rq[u] = Bc.rmeta[kl];
           vq[u] = Bc.rval[kl];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-          if (k0 + 256 * u < ncon)
+          if (k0 + kMultiT * u < ncon)
             add_row(rq[u].x, rq[u].y, rq[u].z, rq[u].w & 15, (rq[u].w >> 4) & 3, rq[u], vq[u]);
       }
     } else {
-      for (int ki = pj + tid; ki < ncon; ki += 256)
+      for (int ki = pj + tid; ki < ncon; ki += kMultiT)
         add_row(Bc.con[ki], Bc.estart[ki], Bc.nzeff[ki], Bc.ecnt[ki], Bc.form[ki],
                 make_int4(0, 0, 0, 0), 0.0);
     }
   }
   __syncthreads();
-  for (int r = j + tid; r < m; r += 256) B[(size_t)r + (size_t)m * j] = acc[r];
+  for (int r = j + tid; r < m; r += kMultiT) B[(size_t)r + (size_t)m * j] = acc[r];
 }
 
 int f3multi_prepare(int m) {
@@ -700,7 +701,7 @@ int make_bmat(Ctx* c) {
   const int m = c->m;
   if (c->f3multi) {
     // stores every entry of the lower triangle: the first writer of B
-    hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(256),
+    hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(kMultiT),
                        sizeof(double) * ((size_t)m + (size_t)kMultiE * 4 * kMultiN), s,
                        c->d_f3blk, c->d_jptr, c->d_jent, c->d_er, c->d_ec, c->d_ev, c->d_X,
                        c->d_Zinv, c->d_B, m);
