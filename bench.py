@@ -3,21 +3,26 @@
 G(200, 0.3) graph (seed 2: m = 6011 constraints, one 200x200 block, dense
 6011 x 6011 Schur matrix), FP64 GPU path, one MI355X.
 
-A "step" is one complete interior-point solve (initializeSolve + solve with
-the default parameters, epsilonStar = epsilonDash = 1e-7) of the problem,
-which is uploaded and analysed before the timed region (inputs resident in
-HBM).  The JSON line also carries:
+A "step" is one complete run of SDPA::initializeSolve + SDPA::solve with the
+default parameters (epsilonStar = epsilonDash = 1e-7) on the problem held in
+host memory by the solver (as after SDPA::inputElement; file reading is not
+timed): the upload to HBM, the structural analysis (changeToDense, formula
+choice, Schur decision) and the interior-point loop to pdOPT.  `value` is the
+median over the timed steps.  The JSON line also carries:
+  - "solve_only": the same loop on the already-uploaded problem;
   - per-phase device times of one extra solve run with HIP-event timing;
-  - the F3 bMat assembly bandwidth;
-  - "roofline": the dominant kernel, the Schur Cholesky's wide trailing
-    update (k_syrk_update, MFMA-bound): algorithmic flops per launch over its
-    average launch duration, both from HIP events recorded on the side stream
-    it runs on; "traffic" = HBM bytes per launch from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a device-copy
-    calibration), or null when that summary is absent;
-  - "cpu_baseline": the FP64 CPU oracle (OpenMP) on a bounded sample of the
-    same workload (its first IPM iterations), scaled to the GPU's iteration
-    count;
+  - the F3 bMat assembly bandwidth and the Schur Cholesky's rate;
+  - "roofline": the dominant kernel of the Schur factorization (MFMA-bound):
+    algorithmic flops per launch over its average launch duration, both
+    from HIP events recorded on the stream it runs on; "traffic" = HBM bytes
+    per launch from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by a device-copy calibration), or null when that
+    summary is absent;
+  - "cpu_baseline": one complete solve of the same instance by the FP64 CPU
+    oracle (OpenMP over the host threads), same stopping rule;
+  - "mixed_vs_fp64": the same workload with the mixed-precision Schur path;
+  - "configs": time-to-1e-7 of the other single-GPU workloads (MaxCut
+    n = 100, dense-data n = 500 / m = 2000, the 200-block problem);
   - "batched": 4096 small MaxCut SDPs in one persistent launch (solves/s,
     phase counts, gap histogram, its own CPU-oracle sample);
   - "lambda_star": the 200-block multi-block SDP (m = 5000) solved from the
@@ -26,7 +31,7 @@ HBM).  The JSON line also carries:
 
 Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config lovasz|maxcut|dense|multiblock]
-  python bench.py --no-lambda               # without the learned-lambda* comparison
+  python bench.py --no-lambda --no-configs  # the headline only (kernel profiles)
   python bench.py --copy-calibration        # one 256 MiB device copy (for PMC scaling)
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR   # print the traffic summary
 """
@@ -63,22 +68,68 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(d, gpu_iterations, sample_iters=2):
-    """FP64 oracle (OpenMP, all host threads it is given) on the same
-    instance: `sample_iters` IPM iterations timed, scaled to the GPU's
-    iteration count."""
+def cpu_baseline(d, name):
+    """One complete solve of the same instance by the FP64 oracle (OpenMP,
+    all host threads it is given), initializeSolve included."""
     from oracle import oracle as O
-    P = O.Problem(d)
     t0 = time.perf_counter()
-    out = P.solve(max_iter=sample_iters)
+    P = O.Problem(d)
+    out = P.solve()
     t = time.perf_counter() - t0
-    done = max(1, out.iterations)
-    per_it = t / done
-    return {"value": per_it * gpu_iterations, "unit": "s", "cores": int(O.lib().orc_num_threads()),
-            "kind": "port",
-            "sample": f"the first {done} of {gpu_iterations} IPM iterations of the same instance "
-                      f"({t:.2f} s wall incl. setup, FP64 C oracle with OpenMP), time per iteration "
-                      f"x {gpu_iterations}"}
+    return {"value": t, "unit": "s", "cores": int(O.lib().orc_num_threads()), "kind": "port",
+            "iterations": int(out.iterations), "phase": out.phase,
+            "primal_obj": out.primal_obj, "dual_obj": out.dual_obj,
+            "sample": f"one complete solve of {name} by the FP64 C oracle with OpenMP "
+                      f"({out.iterations} iterations to {out.phase}), problem set-up included"}
+
+
+def timed_solves(s, steps, warmup, full, label):
+    """Median wall time of `steps` solves after `warmup` untimed ones.  full:
+    initializeSolve + solve (upload and analysis included), else solve only."""
+    if not full:
+        s.prepare()
+    r = None
+    for w in range(warmup):
+        r = s.solve() if full else s.run()
+        log(f"bench: {label} warmup {w}: phase={r.phase} it={r.iterations}")
+    times = []
+    for k in range(max(1, steps)):
+        t0 = time.perf_counter()
+        r = s.solve() if full else s.run()  # returns after the final device->host copy
+        times.append(time.perf_counter() - t0)
+        log(f"bench: {label} step {k}: phase={r.phase} it={r.iterations} t={times[-1]:.4f}s "
+            f"gap={r.relative_gap:.2e}")
+    return float(np.median(times)), r
+
+
+def other_config(key, steps):
+    """time-to-1e-7 (initializeSolve + solve, median of `steps`) of another
+    single-GPU workload, with its per-phase device times."""
+    import sdpa_amd
+    from sdpa_amd import problems as PR
+    name, gen = CONFIGS[key]
+    d = gen(PR)
+    s = sdpa_amd.Solver(0).load(d)
+    val, r = timed_solves(s, steps, 1, True, name)
+    s.enable_timing(True)
+    rt = s.run()
+    pt = s.phase_times()
+    s.enable_timing(False)
+    s.close()
+    its = max(1, rt.iterations)
+    out = {"workload": name, "m": d.m, "time_to_1e7_s": val, "phase": r.phase,
+           "iterations": r.iterations, "relative_gap": r.relative_gap,
+           "primal_error": r.primal_error, "dual_error": r.dual_error,
+           "primal_obj": r.primal_obj, "dual_obj": r.dual_obj,
+           "phase_ms_per_iteration": {k: v / its for k, v in pt.items()}}
+    if key == "dense":
+        n, m = 500, d.m
+        fl = m * 4.0 * n ** 3 + m * (m + 1.0) * n ** 2  # the reference F1's useful flops
+        ms = pt["bmat"] / its
+        out["bmat_f1"] = {"flops": fl, "ms": ms, "TFLOPs": fl / (ms * 1e-3) / 1e12,
+                          "frac_fp64_peak": fl / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    del d
+    return out
 
 
 def lambda_star(steps):
@@ -257,6 +308,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lambda", action="store_true",
                     help="skip the learned-lambda* comparison (kernel profiles of the headline)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other single-GPU workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, one 256 MiB device copy (PMC scaling)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
@@ -277,46 +330,29 @@ def main():
     d = gen(PR)
     n_sdp = int(d.block_struct[0])
     log(f"bench: config {name}: m={d.m} blocks={list(d.block_struct)[:4]} nnz={d.nnz}")
-    s = sdpa_amd.Solver(0).load(d).prepare()
-    for w in range(args.warmup):
-        r = s.run()
-        log(f"bench: warmup {w}: phase={r.phase} it={r.iterations} t={r.solve_seconds:.3f}s")
-    times = []
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        r = s.run()  # returns after the final device->host copy (stream synchronized)
-        times.append(time.perf_counter() - t0)
-        log(f"bench: step {k}: phase={r.phase} it={r.iterations} t={times[-1]:.4f}s "
-            f"gap={r.relative_gap:.2e}")
-    value = float(np.mean(times))
+    s = sdpa_amd.Solver(0).load(d)
+    value, r = timed_solves(s, args.steps, args.warmup, True, "fp64")
+    solve_only, _ = timed_solves(s, args.steps, 1, False, "fp64 solve-only")
     # one extra solve with per-phase HIP-event timing (synchronizing, not in `value`)
     s.enable_timing(True)
     rt = s.run()
     pt = s.phase_times()
     tr_ms, tr_n, tr_flops = s.trailing_update_timing()
     s.enable_timing(False)
-    log(f"bench: timed solve: phase={rt.phase} it={rt.iterations}; trailing update {tr_n} launches")
+    log(f"bench: timed solve: phase={rt.phase} it={rt.iterations}; dominant kernel {tr_n} launches")
     # the same workload with the mixed-precision Schur factorization (FP32
     # factor + FP64 refinement, FP64 after the first stall), same step count
     s.set_param(precision=1)
-    for w in range(args.warmup):
-        s.run()
     st0 = s.mixed_stats()
-    mtimes = []
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        rm = s.run()
-        mtimes.append(time.perf_counter() - t0)
-        log(f"bench: mixed step {k}: phase={rm.phase} it={rm.iterations} t={mtimes[-1]:.4f}s "
-            f"gap={rm.relative_gap:.2e}")
+    mval, rm = timed_solves(s, args.steps, args.warmup, True, "mixed")
     st1 = s.mixed_stats()
     s.enable_timing(True)
     s.run()
     mpt = s.phase_times()
     s.enable_timing(False)
     s.set_param(precision=0)
-    mval = float(np.mean(mtimes))
-    nrun = max(1, args.steps)
+    s.close()
+    nrun = max(1, args.steps) + args.warmup
     mixed = {
         "value": mval, "unit": "s",
         "fp64_over_mixed": value / mval,
@@ -328,14 +364,25 @@ def main():
                       ("factors", "fallbacks", "solves", "refine_steps", "stalls")},
         "phase_ms_per_iteration": {k: v / max(1, rm.iterations) for k, v in mpt.items()},
     }
+    configs = None
+    if not args.no_configs:
+        configs = {}
+        for key in ("maxcut", "dense", "multiblock"):
+            if key == args.config:
+                continue
+            log(f"bench: config {key}")
+            try:
+                configs[key] = other_config(key, min(3, max(1, args.steps)))
+            except Exception as e:  # reported beside the headline, never part of it
+                configs[key] = {"error": str(e)}
     log("bench: batched workload")
-    batch = batched(args.steps, args.warmup, not args.no_cpu_baseline)
+    batch = batched(min(args.steps, 5), min(args.warmup, 2), not args.no_cpu_baseline)
     log(f"bench: batched {batch['value']:.0f} solves/s ({batch['kernel_ms']:.1f} ms kernel)")
     lam = None
     if not args.no_lambda:
         log("bench: learned lambda* on the 200-block configuration")
         try:
-            lam = lambda_star(args.steps)
+            lam = lambda_star(min(args.steps, 3))
         except Exception as e:  # reported beside the headline, never part of it
             lam = {"value": None, "error": str(e)}
         log(f"bench: lambda* {json.dumps(lam)[:200]}")
@@ -377,14 +424,24 @@ def main():
                     "frac_hbm": bmat_gbs(bmat_bytes, bmat_ms) / HBM_PEAK_GBS},
         "cholesky": {"flops": m ** 3 / 3.0, "ms": chol_ms,
                      "TFLOPs": m ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None},
+        "solve_only": {"value": solve_only, "unit": "s",
+                       "note": "SDPA::solve on the uploaded problem (initializeSolve not timed)"},
         "mixed_vs_fp64": mixed,
+        "configs": configs,
         "batched": batch,
         "lambda_star": lam,
         "roofline": roof,
     }
     if not args.no_cpu_baseline:
         try:
-            line["cpu_baseline"] = cpu_baseline(d, its)
+            cb = cpu_baseline(d, name)
+            line["cpu_baseline"] = cb
+            # the GPU results' objective errors against the FP64 oracle's
+            for key, rr in (("fp64", rt), ("mixed", rm)):
+                line.setdefault("error_vs_oracle", {})[key] = {
+                    "primal_obj_rel": abs(rr.primal_obj - cb["primal_obj"]) / abs(cb["primal_obj"]),
+                    "dual_obj_rel": abs(rr.dual_obj - cb["dual_obj"]) / abs(cb["dual_obj"]),
+                    "iterations": [int(rr.iterations), cb["iterations"]]}
         except Exception as e:  # the baseline is reported, never the target
             line["cpu_baseline"] = {"value": None, "error": str(e)}
     if args.copy_calibration:

@@ -82,7 +82,10 @@ int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* ctx, int precision);
  * factor left by sdpa_gpu_factor_bmat.  After a mixed factorization the
  * solution is refined in FP64 until the residual gVec - bMat dy is at FP64
  * accuracy; returns 2 if the refinement stalls (dy is then not set: factor
- * again with precision 0 and solve again). */
+ * again with precision 0 and solve again).  The FP64 solve is asynchronous:
+ * if its cooperative kernels give up waiting (work-groups not co-resident),
+ * the next sdpa_gpu_dots call returns -60 and clears the condition (so does
+ * sdpa_gpu_load_problem). */
 int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* ctx, int refine);
 /* Mixed-precision counters (no counterpart in the reference): out[0]
  * mixed factorizations, [1] fallbacks to FP64 at factorization, [2] refined

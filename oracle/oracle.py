@@ -81,6 +81,8 @@ def lib():
         L.orc_min_eig_packed.restype = C.c_double
         L.orc_min_eig_packed.argtypes = [_P, _dp, _dp]
         L.orc_make_bmat.argtypes = [_P, _dp, _dp, _dp]
+        L.orc_direction.argtypes = [_P, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, _dp]
         L.orc_default_param.argtypes = [C.POINTER(Param)]
         L.orc_solve.argtypes = [_P, C.POINTER(Param), C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.POINTER(Trace), C.POINTER(Result), C.c_void_p, C.c_void_p,
@@ -159,6 +161,25 @@ class Problem:
     def min_eig(self, Lp, Dp) -> float:
         return lib().orc_min_eig_packed(self.h, np.ascontiguousarray(Lp, np.float64),
                                         np.ascontiguousarray(Dp, np.float64))
+
+    def direction(self, X, y, Z, infeasible: bool, mu: float, beta_pred: float,
+                  beta_corr: float):
+        """Predictor and corrector Newton directions from (X, y, Z) (internal
+        convention, packed): two dicts of g, dy, Dx, Dz."""
+        m, T = self.m, self.total
+        out = np.zeros(2 * (2 * m + 2 * T))
+        rc = lib().orc_direction(self.h, np.ascontiguousarray(X, np.float64),
+                                 np.ascontiguousarray(y, np.float64),
+                                 np.ascontiguousarray(Z, np.float64), int(bool(infeasible)),
+                                 float(mu), float(beta_pred), float(beta_corr), out)
+        if rc != 0:
+            raise RuntimeError(f"orc_direction failed ({rc})")
+        res = []
+        for p in range(2):
+            o = out[p * (2 * m + 2 * T):(p + 1) * (2 * m + 2 * T)]
+            res.append({"g": o[:m].copy(), "dy": o[m:2 * m].copy(),
+                        "Dx": o[2 * m:2 * m + T].copy(), "Dz": o[2 * m + T:].copy()})
+        return res[0], res[1]
 
     def solve(self, param: Param | None = None, X0=None, Z0=None, y0=None, max_iter: int = 0):
         L = lib()

@@ -1468,6 +1468,61 @@ int orc_solve(const oprob *P, const oparam *prm, const double *X0, const double 
   return it;
 }
 
+/*
+ * One Mehrotra Newton step from a given iterate, without the step lengths:
+ * Newton::Mehrotra(PREDICTOR) then Newton::Mehrotra(CORRECTOR) with the
+ * predictor's factor reused (sdpa_newton.cpp:2755-2809; Make_gVec 793-883,
+ * compute_DyVec 2672-2716, compute_DzMat/DxMat 2718-2752).  The iterate
+ * (X, y, Z internal, packed) is taken as is; computeInverse runs first
+ * (sdpa_dataset.cpp:117-142).  infeasible != 0 takes the branch of phases
+ * noINFO/pFEAS (the X D Z^-1 terms and D added to Dz).  The predictor's
+ * target is betaPred * mu, the corrector's betaCorr * mu.
+ * out (length 2 * (2 m + 2 T)): predictor g, dy, Dx, Dz, then the
+ * corrector's.  Returns 0, the failing Schur pivot (> 0), or -1 if X or Z
+ * is not positive definite.
+ */
+int orc_direction(const oprob *P, const double *X, const double *y, const double *Z,
+                  int infeasible, double mu, double betaPred, double betaCorr, double *out) {
+  ostate s; memset(&s, 0, sizeof(s));
+  ostate *S = &s;
+  S->P = P; S->T = P->total; S->m = P->m; S->nDim = P->nDim;
+  long T = S->T; int m = S->m;
+  double **bufs[] = {&S->X, &S->Z, &S->invCholX, &S->invCholZ, &S->Zinv, &S->Dx, &S->Dz,
+                     &S->rzinv, &S->xrdzinv, &S->work1, &S->work2, &S->dualMat};
+  for (unsigned b = 0; b < sizeof(bufs) / sizeof(bufs[0]); b++) *bufs[b] = xcalloc(T, sizeof(double));
+  S->y = xcalloc(m, sizeof(double)); S->dy = xcalloc(m, sizeof(double));
+  S->g = xcalloc(m, sizeof(double)); S->primalVec = xcalloc(m, sizeof(double));
+  S->B = xcalloc((size_t)m * m, sizeof(double));
+  memcpy(S->X, X, sizeof(double) * T);
+  memcpy(S->Z, Z, sizeof(double) * T);
+  memcpy(S->y, y, sizeof(double) * m);
+  int rc = 0;
+  if (!compute_inverse(P, S->X, S->Z, S->invCholX, S->invCholZ, S->Zinv)) { rc = -1; goto done; }
+  residuals(S);
+  S->phase = infeasible ? noINFO : pdFEAS;
+  S->muCur = mu;
+  S->betaVal = betaPred;
+  make_gvec(S, 0);
+  orc_make_bmat(P, S->X, S->Zinv, S->B);
+  rc = orc_chol_adjust(m, S->B, m);
+  if (rc) goto done;
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) { S->betaVal = betaCorr; make_gvec(S, 1); }
+    memcpy(S->dy, S->g, sizeof(double) * m);
+    orc_chol_solve(m, S->B, m, S->dy);
+    make_dz_dx(S);
+    double *o = out + (size_t)pass * (2 * m + 2 * T);
+    memcpy(o, S->g, sizeof(double) * m);
+    memcpy(o + m, S->dy, sizeof(double) * m);
+    memcpy(o + 2 * m, S->Dx, sizeof(double) * T);
+    memcpy(o + 2 * m + T, S->Dz, sizeof(double) * T);
+  }
+done:
+  for (unsigned b = 0; b < sizeof(bufs) / sizeof(bufs[0]); b++) free(*bufs[b]);
+  free(S->y); free(S->dy); free(S->g); free(S->primalVec); free(S->B);
+  return rc;
+}
+
 /* Size of the structs for the ctypes mirror in tests. */
 int orc_sizeof_param(void) { return (int)sizeof(oparam); }
 int orc_sizeof_trace(void) { return (int)sizeof(otrace); }

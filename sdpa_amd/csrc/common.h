@@ -111,6 +111,9 @@ struct DenseWork {
   int ll_max_nb = -1, ll_max_nb32 = -1;
   unsigned ll_epoch = 0;  // last epoch used in d_ll (reset only with a fresh, zeroed d_ll)
   int ensure_ll(size_t nwords);
+  // clear the give-up flag (device word, stream-ordered, and its pinned copy)
+  // once a failure has been reported, so later solves start clean
+  int clear_err(hipStream_t s);
   // look-ahead Cholesky: side stream (default priority) and per-panel events
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> evp, evn;

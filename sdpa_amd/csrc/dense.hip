@@ -194,6 +194,13 @@ int DenseWork::ensure_ll(size_t nwords) {
   ll_epoch = 0;
   return 0;
 }
+int DenseWork::clear_err(hipStream_t s) {
+  if (!d_err) return 0;
+  SDPA_HIP_CHECK(hipMemsetAsync(d_err, 0, sizeof(int), s));
+  SDPA_HIP_CHECK(hipStreamSynchronize(s));
+  *h_err = 0;
+  return 0;
+}
 int DenseWork::ensure_vec(size_t n) {
   if (n <= cap_vec) return 0;
   if (d_vec) SDPA_HIP_CHECK(hipFree(d_vec));

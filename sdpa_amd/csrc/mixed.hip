@@ -263,7 +263,10 @@ int solve_mixed(Ctx* c) {
     SDPA_HIP_CHECK(hipMemcpyAsync(c->h_mnorm, c->d_mnorm, sizeof(double) * 3,
                                   hipMemcpyDeviceToHost, s));
     SDPA_HIP_CHECK(hipStreamSynchronize(s));
-    if (c->dw.h_err && *c->dw.h_err) return mixed_fail(c, -60, "pipelined solve gave up waiting");
+    if (c->dw.h_err && *c->dw.h_err) {
+      if (int e = c->dw.clear_err(s)) return e;
+      return mixed_fail(c, -60, "pipelined solve gave up waiting");
+    }
     ++c->mx_refine;
     const double nr = c->h_mnorm[0], nx = c->h_mnorm[1];
     // FP64 accuracy on the equilibrated system, the test of LAPACK's mixed

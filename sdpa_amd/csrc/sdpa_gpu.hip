@@ -127,6 +127,7 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   SDPA_HIP_CHECK(hipSetDevice(c->device));
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
   free_problem(c);
+  if (int e = c->dw.clear_err(c->stream)) return e;
   if (m <= 0 || nBlock <= 0) return fail(c, -10, "m and nBlock must be positive");
   c->m = m;
   c->nBlock = nBlock;
@@ -738,8 +739,10 @@ int sdpa_gpu_dots(sdpa_gpu_ctx* h, int nq, const int* codes, double* out) {
   const int r = compute_dots(&h->c, nq, codes, out);
   // compute_dots synchronizes the stream: the failure flag of the last
   // cooperative Schur solve (copied to pinned memory behind it) is current
-  if (r == 0 && h->c.dw.h_err && *h->c.dw.h_err)
+  if (r == 0 && h->c.dw.h_err && *h->c.dw.h_err) {
+    if (int e = h->c.dw.clear_err(h->c.stream)) return e;
     return fail(&h->c, -60, "a cooperative Schur solve gave up waiting (work-groups not co-resident)");
+  }
   return r;
 }
 

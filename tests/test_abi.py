@@ -11,10 +11,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("sdpa_gpu.h", "sdpa_amd.h"):
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         txt = open(os.path.join(ROOT, "include", h)).read()
         # C declarations: return type, name, '('
-        for m in re.finditer(r"^\s*(?:int|void|const char\*|sdpa_solver\*|sdpa_gpu_ctx\*)\s+(sdpa_\w+)\s*\(",
+        for m in re.finditer(r"^\s*(?:int|void|long|double|const char\*|sdpa_\w+\*)\s+(sdpa_\w+)\s*\(",
                              txt, re.M):
             names.add(m.group(1))
     return sorted(names)
@@ -23,7 +25,8 @@ def declared_functions():
 def test_headers_declare_the_boundary():
     names = declared_functions()
     for must in ("sdpa_gpu_make_bmat", "sdpa_gpu_factor_bmat", "sdpa_gpu_solve_bmat",
-                 "sdpa_gpu_inv_chol", "sdpa_gpu_min_eig", "sdpa_solver_solve"):
+                 "sdpa_gpu_inv_chol", "sdpa_gpu_min_eig", "sdpa_solver_solve",
+                 "sdpa_batch_solve", "sdpa_mlp_train", "sdpa_mlp_predict"):
         assert must in names
 
 

@@ -123,3 +123,33 @@ def test_batch_parameters_and_repeat():
         assert rm.iterations[t] == r1.iterations[t % 6]
         assert rm.primal_obj[t] == r1.primal_obj[t % 6]
         assert rm.dual_obj[t] == r1.dual_obj[t % 6]
+
+
+def test_batch4096_matches_oracle_fixture():
+    """All 4096 instances of the batched workload (problems.maxcut_batch(4096))
+    against the oracle's solves committed in tests/golden/oracle_batch4096.json
+    (made by make_oracle_fixtures.py).  Where both end in the same phase:
+    iterations within one, objectives within 1e-6 relative (1e-8 absolute).
+    The phase may differ only between pdOPT and pdFEAS (an instance whose
+    Schur matrix turns numerically singular near the end, where the pivot
+    rule's 1e-14 threshold decides on the last bits), on at most 4 of 4096."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "oracle_batch4096.json")))
+    insts = PR.maxcut_batch(4096)
+    r = B.solve(insts)
+    differ = []
+    for t in range(len(insts)):
+        if r.phase[t] != gold["phase"][t]:
+            differ.append((t, r.phase[t], gold["phase"][t]))
+            continue
+        assert abs(int(r.iterations[t]) - gold["iterations"][t]) <= 1, t
+        assert r.primal_obj[t] == pytest.approx(gold["primal_obj"][t], rel=1e-6, abs=1e-8), t
+        assert r.dual_obj[t] == pytest.approx(gold["dual_obj"][t], rel=1e-6, abs=1e-8), t
+        if r.phase[t] == "pdOPT":
+            assert r.relative_gap[t] <= 1e-7, t
+    print(f"batch4096: {len(insts) - len(differ)} of {len(insts)} in the oracle's phase; "
+          f"differ: {differ}")
+    assert len(differ) <= 4
+    assert all({a, b} == {"pdOPT", "pdFEAS"} for _, a, b in differ)

@@ -1,0 +1,253 @@
+"""The HIP path at the benchmark sizes, against the oracle and against what
+the mathematics fixes.
+
+* Schur factorization and solves at m = 1300, 2100 and 6011 on B = M M^T
+  with M lower triangular (positive diagonal): the Cholesky factor of B is M
+  itself (LAPACK's dpotrf returns M to 1e-15 on these matrices), so the
+  whole look-ahead schedule -- panel chain, side-stream updates, the
+  wide-tile trailing update, the pipelined solves over 24 blocks -- is
+  compared with a closed form.  Tolerance: 1e-12 absolute on entries of
+  size <= 2; solves 1e-10 relative to the known solution.
+* The Lovasz theta workload (G(200, 0.3), seed 2, m = 6011) at a late
+  iterate of the GPU solver (iteration 8): bMat against the oracle's F3
+  (rtol 1e-11), the factor against the oracle's rATL_dpotrfL restatement
+  (max difference <= 1e-9 of the largest entry), the solve's backward
+  error (<= 1e-12), and one predictor-corrector Newton direction (g, dy,
+  Dx, Dz) against the oracle's orc_direction from the same iterate: g to
+  1e-12, the solved quantities to 1e-7 in relative norm (the Schur matrix
+  at that iterate has a condition number near 1e8).
+* Whole solves of the Lovasz (FP64 and mixed precision) and 200-block
+  workloads against the oracle's solves committed in
+  tests/golden/oracle_solves.json (made by make_oracle_fixtures.py): same
+  phase, iteration count within one, objectives within 1e-6 relative,
+  relative gap and both feasibility errors at most 1e-7, and the
+  per-iteration mu within 1e-4 relative while both runs are at the same
+  iteration.
+* The dense-data workload (n = 500, m = 2000, F1): 200 bMat entries against
+  the brute-force trace Tr(A_i X A_j Z^-1) (rtol 1e-10), and the solve's
+  final point checked on the host: b - A(X) and C - sum y_i A_i - Z at most
+  1e-7, X and Z positive semidefinite, the relative gap at most 1e-7.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sdpa_amd import problems as PR
+
+pytestmark = pytest.mark.gpu
+
+sa = pytest.importorskip("sdpa_amd")
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_solves.json")
+
+
+def gold(name):
+    with open(GOLDEN) as f:
+        return json.load(f)[name]
+
+
+def triangular_case(m, seed):
+    rng = np.random.default_rng(seed)
+    M = np.tril(rng.standard_normal((m, m))) / np.sqrt(m)
+    M[np.diag_indices(m)] = 1.0 + rng.uniform(0.0, 1.0, m)
+    return M, M @ M.T
+
+
+@pytest.mark.parametrize("m", [1300, 2100, 6011])
+def test_factor_and_solve_closed_form(m):
+    M, B = triangular_case(m, m)
+    d = PR.maxcut(m, 2.0 / m, seed=1)  # any problem with this m; only bMat is used
+    g = sa.GpuContext(0).load(d)
+    g.set("B", B.reshape(-1, order="F"))
+    assert g.factor_bmat() == 0
+    L = g.bmat()
+    tril = np.tril_indices(m)
+    err = np.abs(L[tril] - M[tril]).max()
+    print(f"m={m}: max |L - M| = {err:.3e}")
+    assert err <= 1e-12
+    rng = np.random.default_rng(m + 1)
+    x = rng.standard_normal(m)
+    g.set("g", B @ x)
+    g.solve_bmat()
+    dy = g.get("dy")
+    rel = np.linalg.norm(dy - x) / np.linalg.norm(x)
+    print(f"m={m}: solve relative error {rel:.3e}")
+    assert rel <= 1e-10
+    g.close()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_lovasz_m6011_late_iterate():
+    d = PR.lovasz(200, 0.3, 2)
+    s = sa.Solver(0).load(d).set_param(maxIteration=8)
+    s.solve()
+    X, Z, y = s.iterate(d.packed_size(), d.m)
+    s.close()
+    g = sa.GpuContext(0).load(d)
+    g.set("X", X); g.set("Z", Z); g.set("y", y)
+    assert g.inv_chol() == 0
+    Zi = g.get("Zinv")
+    P = O.Problem(d)
+    # bMat (F3 with two row segments and the trace row)
+    g.make_bmat()
+    B = g.bmat()
+    Bo = P.make_bmat(X, Zi)
+    tril = np.tril_indices(d.m)
+    np.testing.assert_allclose(B[tril], Bo[tril], rtol=1e-11, atol=1e-13 * np.abs(Bo).max())
+    # factor against rATL_dpotrfL's rule
+    assert g.factor_bmat() == 0
+    L = g.bmat()
+    info, Lo = O.chol_adjust(Bo)
+    assert info == 0
+    ferr = np.abs(L[tril] - Lo[tril]).max() / np.abs(Lo).max()
+    print(f"lovasz m=6011 iteration 8: factor max difference / max entry {ferr:.3e}")
+    assert ferr <= 1e-9
+    # backward error of the solve
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal(d.m)
+    g.set("g", rhs)
+    g.solve_bmat()
+    x = g.get("dy")
+    Bs = np.tril(Bo) + np.tril(Bo, -1).T
+    berr = np.linalg.norm(Bs @ x - rhs) / (np.linalg.norm(Bs, 1) * np.linalg.norm(x) + np.linalg.norm(rhs))
+    print(f"lovasz m=6011 iteration 8: solve backward error {berr:.3e}")
+    assert berr <= 1e-12
+    # one Newton step (predictor then corrector) from the same iterate
+    mu = float(np.dot(X, Z)) / 200.0
+    bp, bc = 0.2, 0.1
+    pred, corr = P.direction(X, y, Z, True, mu, bp, bc)
+    g.set("X", X); g.set("Z", Z); g.set("y", y)
+    assert g.inv_chol() == 0
+    g.residuals()
+    g.make_gvec(0, bp * mu, 1)
+    g.make_bmat()
+    assert g.factor_bmat() == 0
+    got = []
+    for step, beta in ((0, bp), (1, bc)):
+        if step == 1:
+            g.make_gvec(1, bc * mu, 1)
+        g.solve_bmat()
+        g.make_direction(1)
+        got.append({k: g.get(k) for k in ("g", "dy", "Dx", "Dz")})
+    for name, want, have in (("predictor", pred, got[0]), ("corrector", corr, got[1])):
+        errs = {k: rel(have[k], want[k]) for k in want}
+        print(f"lovasz m=6011 iteration 8 {name} direction relative errors: " +
+              ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+        assert errs["g"] <= 1e-12
+        for k in ("dy", "Dx", "Dz"):
+            assert errs[k] <= 1e-7, (name, k)
+    g.close()
+
+
+def check_against_gold(r, o, mu_trace=True):
+    assert r.phase == o["phase"]
+    assert abs(r.iterations - o["iterations"]) <= 1
+    assert r.primal_obj == pytest.approx(o["primal_obj"], rel=1e-6, abs=1e-8)
+    assert r.dual_obj == pytest.approx(o["dual_obj"], rel=1e-6, abs=1e-8)
+    if o["phase"] == "pdOPT":
+        assert r.relative_gap <= 1e-7
+        assert r.primal_error <= 1e-7 and r.dual_error <= 1e-7
+    if mu_trace:
+        # the trajectories agree closely while the Schur matrix is well
+        # conditioned; its condition number grows like mu^-2, and so does the
+        # difference of two FP64 runs that sum in different orders: while mu
+        # is above 1e-6 of its start 1e-6 relative, below that 5e-2
+        n = min(r.iterations, o["iterations"])
+        mu0 = o["trace"][0]["mu"]
+        diffs = []
+        for t in range(n):
+            a, b = r.trace[t][0], o["trace"][t]["mu"]
+            diffs.append(abs(a - b) / abs(b))
+            assert diffs[-1] <= (1e-6 if b >= 1e-6 * mu0 else 5e-2), (t, a, b)
+        print("relative mu difference per iteration: " + " ".join(f"{v:.1e}" for v in diffs))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lovasz_m6011_solve_matches_oracle(precision):
+    d = PR.lovasz(200, 0.3, 2)
+    s = sa.Solver(0).load(d).set_param(precision=precision)
+    r = s.solve()
+    o = gold("lovasz_theta_G200_p0.3_seed2")
+    print(f"precision {precision}: {r.phase} {r.iterations} it, objectives {r.primal_obj!r} "
+          f"{r.dual_obj!r}; oracle {o['iterations']} it {o['primal_obj']!r} {o['dual_obj']!r}")
+    check_against_gold(r, o)
+    s.close()
+
+
+def test_multiblock_m5000_solve_matches_oracle():
+    d = PR.multi_block()
+    s = sa.Solver(0).load(d)
+    r = s.solve()
+    o = gold("multiblock_200x[5,50]_lp1000_m5000_seed5")
+    print(f"multiblock: {r.phase} {r.iterations} it, objectives {r.primal_obj!r} "
+          f"{r.dual_obj!r}; oracle {o['iterations']} it {o['primal_obj']!r}")
+    check_against_gold(r, o)
+    s.close()
+
+
+def test_maxcut_n100_solve_matches_oracle():
+    r = sa.Solver(0).load(PR.maxcut(100, 0.1, 1)).solve()
+    check_against_gold(r, gold("maxcut_G100_p0.1_seed1"))
+
+
+def dense_block(row, n, iu, ju):
+    A = np.zeros((n, n))
+    A[iu, ju] = row
+    A[ju, iu] = row
+    return A
+
+
+def test_dense_n500_m2000():
+    n, m = 500, 2000
+    a = PR.dense_random(n, m, 3, as_arrays=True)
+    d = PR.dense_random(n, m, 3)
+    iu, ju = a["iu"], a["ju"]
+    # sampled bMat entries at a random positive definite X, Z
+    g = sa.GpuContext(0).load(d)
+    assert list(g.formula_counts()) == [m, 0, 0]
+    rng = np.random.default_rng(8)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Xb = (q * rng.uniform(1, 5, n)) @ q.T
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Zb = (q * rng.uniform(1, 5, n)) @ q.T
+    g.set("X", Xb.reshape(-1, order="F")); g.set("Z", Zb.reshape(-1, order="F"))
+    assert g.inv_chol() == 0
+    Zi = g.get("Zinv").reshape(n, n, order="F")
+    g.make_bmat()
+    B = g.bmat()
+    g.close()
+    cols = rng.choice(m, 20, replace=False)
+    worst = 0.0
+    for j in cols:
+        G = Xb @ dense_block(a["Aup"][j], n, iu, ju) @ Zi
+        for i in rng.choice(m, 10, replace=False):
+            want = np.sum(dense_block(a["Aup"][i], n, iu, ju) * G.T)
+            have = B[max(i, j), min(i, j)]
+            worst = max(worst, abs(have - want) / max(abs(want), 1e-300))
+            assert have == pytest.approx(want, rel=1e-10, abs=1e-12 * np.abs(B).max())
+    print(f"dense n=500 m=2000: 200 sampled bMat entries, max relative difference {worst:.2e}")
+    del B
+    # the solve, and its final point checked on the host
+    s = sa.Solver(0).load(d)
+    r = s.solve()
+    print(f"dense n=500 m=2000: {r.phase} in {r.iterations} iterations, objectives "
+          f"{r.primal_obj!r} {r.dual_obj!r}, gap {r.relative_gap:.2e}")
+    assert r.phase == "pdOPT"
+    assert r.relative_gap <= 1e-7
+    X, Z, y = s.iterate(d.packed_size(), d.m)
+    s.close()
+    Xb = X.reshape(n, n, order="F"); Zb = Z.reshape(n, n, order="F")
+    w = np.where(iu == ju, 1.0, 2.0)
+    primal = a["b"] - a["Aup"] @ (w * Xb[iu, ju])   # internal primal residual b - A(X)
+    dual = a["Cup"] - a["Aup"].T @ y - Zb[iu, ju]   # C - sum y_i A_i - Z (upper triangle)
+    print(f"dense n=500 m=2000: host residuals {np.abs(primal).max():.2e} {np.abs(dual).max():.2e}")
+    assert np.abs(primal).max() <= 1e-7
+    assert np.abs(dual).max() <= 1e-7
+    assert np.linalg.eigvalsh((Xb + Xb.T) / 2).min() >= -1e-9
+    assert np.linalg.eigvalsh((Zb + Zb.T) / 2).min() >= -1e-9

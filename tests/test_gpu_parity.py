@@ -82,6 +82,13 @@ CASES = {
     # blocks above 64 (per-block kernels adding to B) beside small ones
     "multiblock_sizes": lambda: PR.multi_block(n_sdp_blocks=6, nmin=10, nmax=120, lp=10, m=150,
                                                density=0.01, seed=13),
+    # the benchmark workloads at full size: Lovasz theta G(200, 0.3), m = 6011
+    # (two 4096-row segments of the column-block F3 kernel plus the trace
+    # row), and the 200-block problem, m = 5000 (the all-blocks F3 kernel with
+    # lists of more than 512 constraints per block, the LP term, dense F1
+    # owners in the small blocks)
+    "lovasz_m6011": lambda: PR.lovasz(200, 0.3, 2),
+    "multiblock_m5000": lambda: PR.multi_block(),
 }
 
 
@@ -94,14 +101,21 @@ def test_bmat_matches_oracle(name):
     g.set("X", X); g.set("Zinv", Zi)
     g.make_bmat()
     B = g.bmat()
-    want = O.Problem(d).make_bmat(X, Zi)
+    P = O.Problem(d)
+    want = P.make_bmat(X, Zi)
     tril = np.tril_indices(d.m)
     scale = np.abs(want).max()
     np.testing.assert_allclose(B[tril], want[tril], rtol=1e-11, atol=1e-13 * scale)
+    # changeToDense and computeFormula_SDP: the same F1/F2/F3 choice per
+    # (constraint, block) pair, and the same dense/sparse Schur decision
+    assert list(g.formula_counts()) == list(P.formula_counts()), name
+    assert bool(g.info[9]) == P.schur_dense, name
     print(name, "formulas", g.formula_counts())
 
 
-@pytest.mark.parametrize("m", [2, 5, 64, 65, 130, 300, 700])
+# 1300 and 2100: the look-ahead schedule's side-stream launches (m > 768) and
+# the wide-tile trailing update (more than 256 columns left, m > 1024)
+@pytest.mark.parametrize("m", [2, 5, 64, 65, 130, 300, 700, 1300, 2100])
 def test_factor_and_solve_match_oracle(m):
     rng = np.random.default_rng(m)
     d = PR.maxcut(m, 0.5, seed=3)  # any problem with this m; only bMat is used
