@@ -43,6 +43,26 @@ int sdpa_batch_solve(int device, int count, const int* n, const int* m, const lo
                      const double* ev, long nent, const double* lambda, const double* param,
                      int threads, double* out, double* kernel_ms);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same solve as sdpa_batch_solve, same arguments and
+ * results, with no hipMalloc/hipFree and no stream or event creation.  The
+ * instance data are staged, and the launch's queue, per-work-group work
+ * slabs and results kept, in one fixed device array of the library
+ * (sdpa_batch_noalloc_capacity() bytes, 128 MiB, held on each device the
+ * library runs on until the process ends).  stream: the HIP stream to run on
+ * (NULL: the null stream); the call returns after the results are copied
+ * back.  Returns the codes of sdpa_batch_solve, plus -9 when the batch does
+ * not fit the array (about 9 MB of fixed slabs per 8 CUs, then roughly
+ * 8 n^2 + 16 entries + 100 bytes per instance; split the batch).  Calls that
+ * overlap on one device share the array and race: serialize them. */
+int sdpa_batch_solve_noalloc(int device, int count, const int* n, const int* m, const long* coff,
+                             const double* C, long nC, const long* boff, const double* b, long nb,
+                             const long* pptr, const int* cptr, long ncptr, const int* er,
+                             const int* ec, const double* ev, long nent, const double* lambda,
+                             const double* param, int threads, double* out, void* stream);
+long sdpa_batch_noalloc_capacity(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,45 @@ class BatchResult:
         self.kernel_ms = kernel_ms
 
 
+def _bind_noalloc():
+    L = lib()
+    f = L.sdpa_batch_solve_noalloc
+    if f.argtypes is None:
+        f.argtypes = [C.c_int, C.c_int, _ip, _ip, _lp, _dp, C.c_long, _lp, _dp, C.c_long, _lp, _ip,
+                      C.c_long, _ip, _ip, _dp, C.c_long, _dp, _dp, C.c_int, _dp, C.c_void_p]
+        L.sdpa_batch_noalloc_capacity.restype = C.c_long
+        L.sdpa_batch_noalloc_capacity.argtypes = []
+    return f
+
+
+def _args(problems, lambdas, packed, params):
+    P = packed if packed is not None else Packed(problems)
+    lam = np.full(P.count, 100.0) if lambdas is None else np.ascontiguousarray(lambdas, np.float64)
+    if lam.shape != (P.count,):
+        raise ValueError("one lambda per instance")
+    pr = dict(DEFAULT)
+    for k, v in params.items():
+        if k not in pr:
+            raise ValueError(f"unknown parameter {k}")
+        pr[k] = v
+    return P, lam, np.array([float(pr[k]) for k in _ORDER])
+
+
+def solve_noalloc(problems, lambdas=None, device: int = 0, threads: int = 256,
+                  packed: Packed | None = None, **params) -> BatchResult:
+    """The same solve through sdpa_batch_solve_noalloc: no device allocation
+    (the library's fixed scratch array holds the launch's buffers; batches
+    that do not fit it raise).  kernel_ms of the result is 0."""
+    P, lam, par = _args(problems, lambdas, packed, params)
+    out = np.zeros(8 * P.count)
+    rc = _bind_noalloc()(device, P.count, P.n, P.m, P.coff, P.C, len(P.C), P.boff, P.b, len(P.b),
+                         P.pptr, P.cptr, len(P.cptr), P.er, P.ec, P.ev, len(P.ev), lam, par,
+                         threads, out, None)
+    if rc != 0:
+        raise GpuError(f"sdpa_batch_solve_noalloc failed ({rc})")
+    return BatchResult(out, 0.0)
+
+
 def solve(problems, lambdas=None, device: int = 0, threads: int = 256, packed: Packed | None = None,
           **params) -> BatchResult:
     """Solve every instance from X0 = Z0 = lambda_k I, y0 = 0 (lambda_k defaults

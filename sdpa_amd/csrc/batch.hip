@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -66,7 +67,8 @@ struct BData {
   const int* ec;
   const double* ev;
   const double* lambda;
-  const long* woff;   // work slab of each instance (kMats n^2 doubles)
+  const long* woff;   // work slab of each instance (kMats n^2 doubles), or null:
+                      // one slab of kMats NMAX^2 doubles per work-group
   const int* order;   // instances, largest first
   unsigned* next;     // queue head (zero at launch)
   double* work;
@@ -727,7 +729,8 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
     const int n = Dt.n[k], m = Dt.m[k], nn = n * n;
     const double* Cm = Dt.C + Dt.coff[k];
     const int ne = stage(Dt, k, n, m, sh);
-    double* wk = Dt.work + Dt.woff[k];
+    double* wk = Dt.woff ? Dt.work + Dt.woff[k]
+                         : Dt.work + (size_t)blockIdx.x * kMats * NMAX * NMAX;
     double* X = wk;
     double* Z = wk + nn;
     double* Zi = wk + 2 * nn;
@@ -1051,18 +1054,15 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
 
 using namespace sdpa_amd;
 
-extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* m,
-                                const long* coff, const double* C, long nC, const long* boff,
-                                const double* b, long nb, const long* pptr, const int* cptr,
-                                long ncptr, const int* er, const int* ec, const double* ev,
-                                long nent, const double* lambda, const double* param,
-                                int threads, double* out, double* kernel_ms) {
-  if (count <= 0 || !n || !m || !coff || !C || !boff || !b || !pptr || !cptr || !er || !ec ||
-      !ev || !lambda || !param || !out)
-    return -1;
-  if (threads != 128 && threads != 256) return -2;
-  std::vector<long> woff(count);
-  long wtot = 0;
+// Checks of the instance arrays shared by both entry points (the error codes
+// of include/sdpa_batch.h); fills the largest-first queue order and, when
+// woff is given, the per-instance slab offsets (returns their total).
+static int check_batch(int count, const int* n, const int* m, const long* coff, long nC,
+                       const long* boff, long nb, const long* pptr, const int* cptr, long ncptr,
+                       const int* er, const int* ec, long nent, const double* lambda,
+                       std::vector<int>& order, std::vector<long>* woff, long* wtot) {
+  long tot = 0;
+  if (woff) woff->assign(count, 0);
   for (int k = 0; k < count; ++k) {
     if (n[k] < 1 || n[k] > NMAX || m[k] < 1 || m[k] > MMAX) return -3;
     if (coff[k] < 0 || coff[k] + (long)n[k] * n[k] > nC || boff[k] < 0 || boff[k] + m[k] > nb ||
@@ -1074,21 +1074,41 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
     }
     if (cptr[pptr[k] + m[k]] - cptr[pptr[k]] > kEnt) return -7;
     if (!(lambda[k] > 0.0)) return -8;
-    woff[k] = wtot;
-    wtot += (long)kMats * n[k] * n[k];
+    if (woff) (*woff)[k] = tot;
+    tot += (long)kMats * n[k] * n[k];
   }
   for (int k = 0; k < count; ++k)
     for (int e = cptr[pptr[k]]; e < cptr[pptr[k] + m[k]]; ++e)
       if (er[e] < 0 || ec[e] < er[e] || ec[e] >= n[k]) return -6;
   // largest instances first (work ~ m^3 / 3 + n^3 per iteration): the
   // queue then ends on small ones and the work-groups finish together
-  std::vector<int> order(count);
+  order.resize(count);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
     const double wa = (double)m[a] * m[a] * m[a] / 3.0 + (double)n[a] * n[a] * n[a];
     const double wc = (double)m[c] * m[c] * m[c] / 3.0 + (double)n[c] * n[c] * n[c];
     return wa > wc;
   });
+  if (wtot) *wtot = tot;
+  return 0;
+}
+
+extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* m,
+                                const long* coff, const double* C, long nC, const long* boff,
+                                const double* b, long nb, const long* pptr, const int* cptr,
+                                long ncptr, const int* er, const int* ec, const double* ev,
+                                long nent, const double* lambda, const double* param,
+                                int threads, double* out, double* kernel_ms) {
+  if (count <= 0 || !n || !m || !coff || !C || !boff || !b || !pptr || !cptr || !er || !ec ||
+      !ev || !lambda || !param || !out)
+    return -1;
+  if (threads != 128 && threads != 256) return -2;
+  std::vector<long> woff;
+  std::vector<int> order;
+  long wtot = 0;
+  if (int e = check_batch(count, n, m, coff, nC, boff, nb, pptr, cptr, ncptr, er, ec, nent,
+                          lambda, order, &woff, &wtot))
+    return e;
   if (hipSetDevice(device) != hipSuccess) return -100;
   hipStream_t s;
   if (hipStreamCreate(&s) != hipSuccess) return -100;
@@ -1173,4 +1193,145 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
     if (p) (void)hipFree(p);
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// The same solve with no device allocation: every device buffer of the
+// launch (the staged instance data, the queue, the per-work-group slabs and
+// the results) lives in one file-scope device array.  Host code never uses
+// the array's name as an address: its device address is looked up once per
+// device (hipGetSymbolAddress, with that device current) and kept.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+int scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -100;
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return 0;
+}
+
+__global__ void k_zero_u32(unsigned* p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
+// bump allocator over the scratch array (256-byte aligned pieces)
+struct Carve {
+  unsigned char* base;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t count) {
+    T* p = reinterpret_cast<T*>(base + used);
+    used += (sizeof(T) * (count ? count : 1) + 255) & ~(size_t)255;
+    return p;
+  }
+};
+
+}  // namespace
+
+extern "C" long sdpa_batch_noalloc_capacity(void) { return (long)kScratchBytes; }
+
+extern "C" int sdpa_batch_solve_noalloc(int device, int count, const int* n, const int* m,
+                                        const long* coff, const double* C, long nC,
+                                        const long* boff, const double* b, long nb,
+                                        const long* pptr, const int* cptr, long ncptr,
+                                        const int* er, const int* ec, const double* ev,
+                                        long nent, const double* lambda, const double* param,
+                                        int threads, double* out, void* stream) {
+  if (count <= 0 || !n || !m || !coff || !C || !boff || !b || !pptr || !cptr || !er || !ec ||
+      !ev || !lambda || !param || !out)
+    return -1;
+  if (threads != 128 && threads != 256) return -2;
+  if (device < 0 || device >= kMaxDevices) return -100;
+  std::vector<int> order;
+  if (int e = check_batch(count, n, m, coff, nC, boff, nb, pptr, cptr, ncptr, er, ec, nent,
+                          lambda, order, nullptr, nullptr))
+    return e;
+  if (hipSetDevice(device) != hipSuccess) return -100;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return -100;
+  const int grid = count < ncu ? count : ncu;
+  // the layout of this call in the scratch array; reject what does not fit
+  Carve cv{nullptr};
+  const size_t o_n = cv.used; cv.take<int>(count);
+  const size_t o_m = cv.used; cv.take<int>(count);
+  const size_t o_co = cv.used; cv.take<long>(count);
+  const size_t o_C = cv.used; cv.take<double>(nC);
+  const size_t o_bo = cv.used; cv.take<long>(count);
+  const size_t o_b = cv.used; cv.take<double>(nb);
+  const size_t o_pp = cv.used; cv.take<long>(count);
+  const size_t o_cp = cv.used; cv.take<int>(ncptr);
+  const size_t o_er = cv.used; cv.take<int>(nent);
+  const size_t o_ec = cv.used; cv.take<int>(nent);
+  const size_t o_ev = cv.used; cv.take<double>(nent);
+  const size_t o_l = cv.used; cv.take<double>(count);
+  const size_t o_ord = cv.used; cv.take<int>(count);
+  const size_t o_next = cv.used; cv.take<unsigned>(1);
+  const size_t o_out = cv.used; cv.take<double>(8 * (size_t)count);
+  const size_t o_w = cv.used; cv.take<double>((size_t)grid * kMats * NMAX * NMAX);
+  if (cv.used > kScratchBytes) return -9;
+  void* base = nullptr;
+  if (scratch_address(device, &base)) return -100;
+  unsigned char* B0 = static_cast<unsigned char*>(base);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto put = [&](size_t off, const void* src, size_t bytes) -> int {
+    if (bytes && hipMemcpyAsync(B0 + off, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+      return -100;
+    return 0;
+  };
+  int rc = 0;
+  if ((rc = put(o_n, n, sizeof(int) * count)) || (rc = put(o_m, m, sizeof(int) * count)) ||
+      (rc = put(o_co, coff, sizeof(long) * count)) || (rc = put(o_C, C, sizeof(double) * nC)) ||
+      (rc = put(o_bo, boff, sizeof(long) * count)) || (rc = put(o_b, b, sizeof(double) * nb)) ||
+      (rc = put(o_pp, pptr, sizeof(long) * count)) || (rc = put(o_cp, cptr, sizeof(int) * ncptr)) ||
+      (rc = put(o_er, er, sizeof(int) * nent)) || (rc = put(o_ec, ec, sizeof(int) * nent)) ||
+      (rc = put(o_ev, ev, sizeof(double) * nent)) ||
+      (rc = put(o_l, lambda, sizeof(double) * count)) ||
+      (rc = put(o_ord, order.data(), sizeof(int) * count)))
+    return rc;
+  // the queue head starts at zero on every call (a kernel, not a memset, so
+  // that the call can sit in a captured graph)
+  unsigned* dnext = reinterpret_cast<unsigned*>(B0 + o_next);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, dnext);
+  BData D{count,
+          reinterpret_cast<const int*>(B0 + o_n),
+          reinterpret_cast<const int*>(B0 + o_m),
+          reinterpret_cast<const long*>(B0 + o_co),
+          reinterpret_cast<const double*>(B0 + o_C),
+          reinterpret_cast<const long*>(B0 + o_bo),
+          reinterpret_cast<const double*>(B0 + o_b),
+          reinterpret_cast<const long*>(B0 + o_pp),
+          reinterpret_cast<const int*>(B0 + o_cp),
+          reinterpret_cast<const int*>(B0 + o_er),
+          reinterpret_cast<const int*>(B0 + o_ec),
+          reinterpret_cast<const double*>(B0 + o_ev),
+          reinterpret_cast<const double*>(B0 + o_l),
+          nullptr,  // per-work-group slabs
+          reinterpret_cast<const int*>(B0 + o_ord),
+          dnext,
+          reinterpret_cast<double*>(B0 + o_w),
+          reinterpret_cast<double*>(B0 + o_out)};
+  BParams P{(int)param[0], param[1], param[2], param[3], param[4], param[5], param[6], param[7],
+            param[8]};
+  if (threads == 256)
+    hipLaunchKernelGGL(k_batch_solve<256>, dim3(grid), dim3(256), 0, s, D, P);
+  else
+    hipLaunchKernelGGL(k_batch_solve<128>, dim3(grid), dim3(128), 0, s, D, P);
+  if (hipGetLastError() != hipSuccess) return -101;
+  if (hipMemcpyAsync(out, B0 + o_out, sizeof(double) * 8 * count, hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return -100;
+  if (hipStreamSynchronize(s) != hipSuccess) return -101;
+  return 0;
 }
