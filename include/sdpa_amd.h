@@ -59,8 +59,12 @@ class SDPA {
   const char* getParameterPrintYMat() const { return YPrint_.c_str(); }
   const char* getParameterPrintInformation() const { return infPrint_.c_str(); }
   ParameterType getParameterType() const { return paramType_; }
-  // threads of the reference's OpenMP bMat; the GPU path has no use for it
-  void setNumThreads(int = 0) {}
+  // Newton::setNumThreads (sdpa_newton.cpp:2127-2149): 0 takes OMP_NUM_THREADS
+  // (1 if unset); the count is echoed to the display and the result file.
+  // It sized the reference's OpenMP bMat loops; the GPU path keeps it only to
+  // report it.
+  void setNumThreads(int numThreads = 0);
+  int getNumThreads() const { return numThreads_; }
   int getParameterMaxIteration() const { return param_.maxIteration; }
   double getParameterEpsilonStar() const { return param_.epsilonStar; }
   double getParameterLambdaStar() const { return param_.lambdaStar; }
@@ -73,6 +77,8 @@ class SDPA {
   double getParameterEpsilonDash() const { return param_.epsilonDash; }
   void setDisplay(FILE* Display = stdout) { display_ = Display; }
   void setResultFile(FILE* fpout = stdout) { fpout_ = fpout; }
+  FILE* getDisplay() const { return display_; }
+  FILE* getResultFile() const { return fpout_; }
   void setInitPoint(bool isInitPoint) { isInitPoint_ = isInitPoint; }
   bool getInitPoint() const { return isInitPoint_; }
   void setKappa(double kappa) { kappa_ = kappa; }
@@ -89,6 +95,12 @@ class SDPA {
   void initializeUpperTriangleSpace();
   void initializeUpperTriangle(bool inputTwiceCheck = false);
   void initializeSolve();
+  // SDPA::resetParameters (sdpa_solve.cpp:45-56) re-derives mu, the initial
+  // residuals, beta, theta, the solve information and the phase from the
+  // current parameters.  solve() derives all of them at its start from the
+  // parameters set at that time, so a reset between solves has nothing left
+  // to do.
+  void resetParameters() {}
   void solve();
 
   double* getResultXVec() { return xVec_.data(); }
@@ -162,6 +174,7 @@ class SDPA {
   sdpa_gpu_ctx* ctx_ = nullptr;
   FILE* display_ = nullptr;
   FILE* fpout_ = nullptr;
+  int numThreads_ = 1;
   bool isInitPoint_ = false;
   bool timing_ = false;
   int precision_ = 0;

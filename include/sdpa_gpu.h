@@ -73,6 +73,11 @@ int sdpa_gpu_inv_chol(sdpa_gpu_ctx* ctx);
 int sdpa_gpu_make_bmat(sdpa_gpu_ctx* ctx, int precision);
 /* Lal::choleskyFactorWithAdjust -> rATL_dpotrfL (sdpa_linear.cpp:762-790,
  * sdpa_dpotrf.cpp:218-265): lower Cholesky of bMat with SDPA's pivot rule.
+ * Where Chordal::ordering_bMat would leave the dense path (problem_info
+ * schur_dense = 0), bMat + 1e-10 I is factored instead, the matrix the
+ * reference's sparse branch factors (Chordal::factorizeSchur,
+ * sdpa_chordal.cpp:526-534); the shift is added to the stored bMat once per
+ * assembled (or set) matrix.
  * precision 0: FP64, in place.  precision 1 (mixed; no counterpart in the
  * reference): FP32 factor of the column-equilibrated bMat on FP32 MFMA, bMat
  * kept in FP64 for refinement; where that factorization fails or m <= 512 it
@@ -126,11 +131,14 @@ int sdpa_gpu_set_buffer(sdpa_gpu_ctx* ctx, int which, const double* src, long le
  * bMat, Cholesky, solve, eigen, inverse. */
 int sdpa_gpu_timing(sdpa_gpu_ctx* ctx, int enable, double* out10);
 /* Instrumentation with no counterpart in the reference: while timing is
- * enabled, HIP events on the side stream bracket every launch of the wide
- * trailing update of the Schur Cholesky (its dominant kernel).  out[0] = ms
- * summed over those launches, out[1] = launches, out[2] = algorithmic flops
- * (2 K per lower-triangle element updated), out[3] = 0. */
-int sdpa_gpu_kernel_timing(sdpa_gpu_ctx* ctx, double* out4);
+ * enabled, HIP events bracket every launch of the Schur Cholesky's two main
+ * kernels -- the wide trailing update on the side stream and the
+ * single-launch panel factorization on the context's stream.  out[0..3]:
+ * trailing update ms summed over its launches, launches, algorithmic flops
+ * (2 K per lower-triangle element updated), 0; out[4..7]: the same for the
+ * panel launches (flops: the previous panel's update of the panel, 2 K per
+ * element, plus the panel's Cholesky, w^2 h - w^3/3). */
+int sdpa_gpu_kernel_timing(sdpa_gpu_ctx* ctx, double* out8);
 int sdpa_gpu_synchronize(sdpa_gpu_ctx* ctx);
 
 #ifdef __cplusplus

@@ -1105,6 +1105,15 @@ typedef struct {
   double *X0, *Z0;
 } ostate;
 
+/* Chordal::factorizeSchur (sdpa_chordal.cpp:526-534): when ordering_bMat
+ * leaves the dense path, the reference factors bMat + PLUS_ADJUST_DIAGONAL I
+ * (1e-10, sdpa_chordal.cpp:25) by its sparse solver; the same shifted matrix
+ * is factored here (densely). */
+static void schur_shift(const oprob *P, double *B) {
+  if (P->schurDense) return;
+  for (int i = 0; i < P->m; i++) B[i + (size_t)P->m * i] += 1.0e-10;
+}
+
 /* Residuals::compute, sdpa_dataset.cpp:585-608 */
 static void residuals(ostate *S) {
   const oprob *P = S->P;
@@ -1399,6 +1408,7 @@ int orc_solve(const oprob *P, const oparam *prm, const double *X0, const double 
     res->time_bmat += t1 - t0;
     double tr = 0.0;
     for (int i = 0; i < m; i++) tr += S->B[i + (size_t)m * i];
+    schur_shift(P, S->B);
     int info = orc_chol_adjust(m, S->B, m);
     double t2 = now_sec();
     res->time_chol += t2 - t1;
@@ -1504,6 +1514,7 @@ int orc_direction(const oprob *P, const double *X, const double *y, const double
   S->betaVal = betaPred;
   make_gvec(S, 0);
   orc_make_bmat(P, S->X, S->Zinv, S->B);
+  schur_shift(P, S->B);
   rc = orc_chol_adjust(m, S->B, m);
   if (rc) goto done;
   for (int pass = 0; pass < 2; pass++) {

@@ -379,14 +379,16 @@ class Solver:
         lib().sdpa_solver_phase_times(self.h, out)
         return dict(zip(("bmat", "chol", "solve", "eig", "inv"), out))
 
-    def trailing_update_timing(self):
-        """HIP-event timing of the Schur Cholesky's wide trailing update
-        kernel over the last timed solve: (ms total, launches, flops)."""
+    def kernel_timing(self):
+        """HIP-event timing of the Schur Cholesky's two main kernels over the
+        last timed solve: {"trailing": (ms total, launches, flops),
+        "panel": (ms total, launches, flops)}."""
         ctx = lib().sdpa_solver_context(self.h)
-        out = np.zeros(4)
+        out = np.zeros(8)
         if ctx:
             lib().sdpa_gpu_kernel_timing(ctx, out.ctypes.data_as(C.c_void_p))
-        return float(out[0]), int(out[1]), float(out[2])
+        return {"trailing": (float(out[0]), int(out[1]), float(out[2])),
+                "panel": (float(out[4]), int(out[5]), float(out[6]))}
 
     def mixed_stats(self):
         """Mixed-precision counters of this solver's context (cumulative):

@@ -25,7 +25,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -1198,28 +1197,14 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
 // ---------------------------------------------------------------------------
 // The same solve with no device allocation: every device buffer of the
 // launch (the staged instance data, the queue, the per-work-group slabs and
-// the results) lives in one file-scope device array.  Host code never uses
-// the array's name as an address: its device address is looked up once per
-// device (hipGetSymbolAddress, with that device current) and kept.
+// the results) lives in one fixed device array, batch_scratch.hip's (a code
+// object of its own, so that only callers of this entry point load it).
+namespace sdpa_amd {
+size_t noalloc_scratch_bytes();
+int noalloc_scratch_address(int device, void** out);
+}  // namespace sdpa_amd
+
 namespace {
-
-constexpr size_t kScratchBytes = (size_t)128 << 20;
-__device__ __align__(256) unsigned char scratch[kScratchBytes];
-
-constexpr int kMaxDevices = 64;
-std::mutex g_scratch_mu;
-void* g_scratch[kMaxDevices] = {};
-
-int scratch_address(int device, void** out) {
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
-  if (!g_scratch[device]) {
-    void* p = nullptr;
-    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -100;
-    g_scratch[device] = p;
-  }
-  *out = g_scratch[device];
-  return 0;
-}
 
 __global__ void k_zero_u32(unsigned* p) {
   if (threadIdx.x == 0) *p = 0u;
@@ -1239,7 +1224,7 @@ struct Carve {
 
 }  // namespace
 
-extern "C" long sdpa_batch_noalloc_capacity(void) { return (long)kScratchBytes; }
+extern "C" long sdpa_batch_noalloc_capacity(void) { return (long)noalloc_scratch_bytes(); }
 
 extern "C" int sdpa_batch_solve_noalloc(int device, int count, const int* n, const int* m,
                                         const long* coff, const double* C, long nC,
@@ -1252,7 +1237,7 @@ extern "C" int sdpa_batch_solve_noalloc(int device, int count, const int* n, con
       !ev || !lambda || !param || !out)
     return -1;
   if (threads != 128 && threads != 256) return -2;
-  if (device < 0 || device >= kMaxDevices) return -100;
+  if (device < 0) return -100;
   std::vector<int> order;
   if (int e = check_batch(count, n, m, coff, nC, boff, nb, pptr, cptr, ncptr, er, ec, nent,
                           lambda, order, nullptr, nullptr))
@@ -1280,9 +1265,9 @@ extern "C" int sdpa_batch_solve_noalloc(int device, int count, const int* n, con
   const size_t o_next = cv.used; cv.take<unsigned>(1);
   const size_t o_out = cv.used; cv.take<double>(8 * (size_t)count);
   const size_t o_w = cv.used; cv.take<double>((size_t)grid * kMats * NMAX * NMAX);
-  if (cv.used > kScratchBytes) return -9;
+  if (cv.used > noalloc_scratch_bytes()) return -9;
   void* base = nullptr;
-  if (scratch_address(device, &base)) return -100;
+  if (noalloc_scratch_address(device, &base)) return -100;
   unsigned char* B0 = static_cast<unsigned char*>(base);
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto put = [&](size_t off, const void* src, size_t bytes) -> int {

@@ -114,21 +114,29 @@ struct DenseWork {
   // clear the give-up flag (device word, stream-ordered, and its pinned copy)
   // once a failure has been reported, so later solves start clean
   int clear_err(hipStream_t s);
+  // single-launch panel factorization: epoch-tagged exchange words
+  // (kPanelTagBlocks blocks of 8192, zeroed when allocated) and the last epoch
+  unsigned long long* d_ptag = nullptr;
+  unsigned ptag_epoch = 0;
+  int ensure_panel_tags();
   // look-ahead Cholesky: side stream (default priority) and per-panel events
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> evp, evn;
   hipEvent_t evjoin = nullptr;
   int side_waves = 0;  // grid bound of the side stream's updates
   int ensure_side(int npanels);
-  // Optional timing of the wide trailing updates (the factorization's
-  // dominant kernel): HIP events around each launch on the side stream.
+  // Optional timing of the factorization's two main kernels: HIP events
+  // around each launch of the wide trailing update on the side stream
+  // (evt*) and of the single-launch panel factorization on the main stream
+  // (evq*), with the algorithmic flops of each launch.
   bool time_trailing = false;
-  std::vector<hipEvent_t> evt0, evt1;
-  std::vector<double> evt_flops;
-  int trail_pending = 0;
-  double trail_ms = 0.0, trail_flops = 0.0;
-  long trail_launches = 0;
+  std::vector<hipEvent_t> evt0, evt1, evq0, evq1;
+  std::vector<double> evt_flops, evq_flops;
+  int trail_pending = 0, panel_pending = 0;
+  double trail_ms = 0.0, trail_flops = 0.0, panel_ms = 0.0, panel_flops = 0.0;
+  long trail_launches = 0, panel_launches = 0;
   int ensure_trailing_events(int n);
+  int ensure_panel_events(int n);
   int collect_trailing();  // call once the factorization has completed
   int ensure_items(int n);
   int ensure_vec(size_t n);

@@ -64,6 +64,7 @@ struct Ctx {
   std::vector<int> blockType, blockNumber, blockStruct;
   int nF[3] = {0, 0, 0};
   int schurDense = 1;
+  bool bshifted = false;  // the sparse branch's diagonal shift is in d_B
   bool bmatStoreFirst = false;  // single block covering all constraints, all F3
   // F3 of the SDP blocks of n <= kMultiN by one launch (several blocks,
   // m <= kF3MultiMaxM; larger blocks keep their per-block kernels): per
@@ -140,7 +141,6 @@ struct Ctx {
   int factor_mixed = 0;       // the current factor is the mixed one (B intact)
   double mx_anorm = 0.0;      // ||S B S||_inf of that factorization
   long mx_factors = 0, mx_fallbacks = 0, mx_solves = 0, mx_refine = 0, mx_stalls = 0;
-  void* blas = nullptr;  // rocBLAS handle (made on first use)
   // timing (ms), accumulated since the last reset
   hipEvent_t ev[8];
   double t_bmat = 0, t_chol = 0, t_solve = 0, t_eig = 0, t_inv = 0;
@@ -171,16 +171,15 @@ int triple_product(Ctx* c, const double* A, const double* B, const double* C, do
 int lp_min_step(Ctx* c, const double* D, const double* L, double* out);
 // eig.hip
 int min_eig(Ctx* c, double* lamX, double* lamZ);
-// blas.cpp (rocBLAS, column-major; t = 1 transposes).  Single blocks of at
-// least kBlasMinN rows take the library GEMMs; many small blocks stay in one
-// batched launch of the own kernel.
+// gemm.hip: C_b = alpha op(A_b) op(B_b) + beta C_b for b < batch (FP64 MFMA,
+// column-major; t = 1 transposes; s* are the batch strides in elements, 0
+// for an operand shared by the batch).  Single blocks of at least kBlasMinN
+// rows take these products; many small blocks stay in one batched launch of
+// the small-tile kernel.
 constexpr int kBlasMinN = 128;
-int blas_gemm_strided(Ctx* c, int ta, int tb, int M, int N, int K, double alpha, const double* A,
-                      int lda, long sA, const double* B, int ldb, long sB, double beta, double* C,
-                      int ldc, long sC, int batch);
-int blas_gemm(Ctx* c, int ta, int tb, int M, int N, int K, double alpha, const double* A, int lda,
-              const double* B, int ldb, double beta, double* C, int ldc);
-void blas_release(Ctx* c);
+int dgemm(hipStream_t s, int ta, int tb, int M, int N, int K, double alpha, const double* A,
+          int lda, long sA, const double* B, int ldb, long sB, double beta, double* C, int ldc,
+          long sC, int batch);
 // mixed.hip
 int factor_mixed(Ctx* c);  // 0 ok; otherwise the FP64 path must be used
 int solve_mixed(Ctx* c);   // 0 converged (dy set); 2 refinement stalled

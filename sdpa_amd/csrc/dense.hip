@@ -212,10 +212,15 @@ void DenseWork::release() {
   if (side) (void)hipStreamSynchronize(side);
   for (auto e : evt0) (void)hipEventDestroy(e);
   for (auto e : evt1) (void)hipEventDestroy(e);
+  for (auto e : evq0) (void)hipEventDestroy(e);
+  for (auto e : evq1) (void)hipEventDestroy(e);
   evt0.clear();
   evt1.clear();
+  evq0.clear();
+  evq1.clear();
   evt_flops.clear();
-  trail_pending = 0;
+  evq_flops.clear();
+  trail_pending = panel_pending = 0;
   for (auto e : evp) (void)hipEventDestroy(e);
   for (auto e : evn) (void)hipEventDestroy(e);
   evp.clear();
@@ -232,6 +237,8 @@ void DenseWork::release() {
   d_flags = nullptr; d_err = nullptr; h_err = nullptr; cap_flags = 0; epoch = 0;
   if (d_ll) (void)hipFree(d_ll);
   d_ll = nullptr; cap_ll = 0; ll_max_nb = ll_max_nb32 = -1; ll_epoch = 0;
+  if (d_ptag) (void)hipFree(d_ptag);
+  d_ptag = nullptr; ptag_epoch = 0;
   if (d_items) (void)hipFree(d_items);
   if (d_mats) (void)hipFree(d_mats);
   if (d_dinv) (void)hipFree(d_dinv);
@@ -328,60 +335,36 @@ __device__ __forceinline__ void offdiag16_inverse(const double* T, double* W, do
   for (int v = 0; v < 4; ++v) W[(16 * (h + 1) + li) + TLD * (16 * h + 4 * v + lk)] = -x[v];
 }
 
-// The diagonal-block work of one item (256 threads).  kp >= 0: first apply
-// the update by the 64 columns [kp, kp + 64) of L to the block,
-// A[k0.., k0..] -= L[k0.., kp..] L[k0.., kp..]^T (the block's share of the
-// previous step's rest-of-panel update).
+// LDS of the 64x64 diagonal-block factorization: T holds the block (its
+// lower part becomes L), W receives inv(L) (it must be zero on entry: only
+// the lower blocks are written), S/S1/S2 are scratch, rdiag the reciprocal
+// pivots, failc the 1-based failing column (0 if none).
+struct Chol64Lds {
+  double T[TB * TLD];
+  double W[TB * TLD];
+  double S[32 * 33];
+  double S1[16 * 17];
+  double S2[16 * 17];
+  double rdiag[TB];
+  int failc;
+};
+
+// Factor the block in L.T (256 threads; L.failc must be 0 and W zero, both
+// visible to the work-group): 16-column mini-panels on wave 0, trailing
+// 16x16 tiles on MFMA, and the first three diagonal 16x16 inverses and
+// W[1][0] built alongside on waves 1 and 2.  MODE 0: LAPACK (a pivot that
+// is not > 0 fails); MODE 1: SDPA's rule (sdpa_dpotrf.cpp:31-33: < -1e-6
+// fails, < 1e-14 becomes 1e100).  kb: the block's true size (the padding
+// beyond it is the identity).  Returns L.failc.
 template <int MODE>
-__device__ __forceinline__ void diag_block(const MatDesc& M, int b, int k0, int kp,
-                                           int* __restrict__ info, double* __restrict__ dinv,
-                                           int nslot) {
-  const int kb = min(TB, M.n - k0);
-  constexpr int SLD = 33;
-  __shared__ double T[TB * TLD];
-  __shared__ double W[TB * TLD];
-  __shared__ double S[32 * SLD];
-  __shared__ double S1[16 * 17];
-  __shared__ double S2[16 * 17];
-  __shared__ double rdiag[TB];
-  __shared__ int failc_s;
+__device__ __forceinline__ int chol64_factor(Chol64Lds& L, int kb) {
+  double* T = L.T;
+  double* W = L.W;
+  double* S1 = L.S1;
+  double* rdiag = L.rdiag;
+  int& failc_s = L.failc;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
-  double* A = M.ptr + k0 + (size_t)M.ld * k0;
-  {
-    // a thread's 16 loads in flight at once, then the LDS stores
-    double a[16];
-    const int r = t & 63, cq = t >> 6;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = cq + 4 * q;
-      a[q] = (r < kb && c < kb) ? A[r + (size_t)M.ld * c] : (r == c ? 1.0 : 0.0);
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) T[r + TLD * (cq + 4 * q)] = a[q];
-    const double* X = M.ptr + k0 + (size_t)M.ld * (kp < 0 ? 0 : kp);
-#pragma unroll
-    for (int q = 0; q < 16; ++q) a[q] = (kp >= 0 && r < kb) ? X[r + (size_t)M.ld * (cq + 4 * q)] : 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = a[q];
-  }
-  if (t == 0) failc_s = 0;
-  __syncthreads();
-  if (kp >= 0) {
-    // T[bi][bj] -= X[bi] X[bj]^T on the ten lower 16x16 tiles (K = 64)
-    for (int q = wv; q < 10; q += 4) {
-      int bj = 0, rem = q;
-      while (rem >= 4 - bj) { rem -= 4 - bj; ++bj; }
-      const int bi = bj + rem;
-      const v4d acc = mma16_lds<1>(&W[16 * bi], TLD, &W[16 * bj], TLD, 64);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) T[(16 * bi + li) + TLD * (16 * bj + 4 * v + lk)] -= acc[v];
-    }
-    __syncthreads();
-    const int r = t & 63, cq = t >> 6;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = 0.0;
-    __syncthreads();
-  }
+  (void)t;
 #pragma unroll 1
   for (int p = 0; p < 4; ++p) {
     const int c0 = 16 * p;
@@ -450,19 +433,21 @@ __device__ __forceinline__ void diag_block(const MatDesc& M, int b, int k0, int 
     }
     if (p < 3) __syncthreads();
   }
-  if (failc_s) {
-    if (t == 0) info[b] = k0 + failc_s;
-    return;
-  }
-  // write the factor back (lower part of the block)
-  {
-    const int r = t & 63, cq = t >> 6;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = cq + 4 * q;
-      if (r >= c && r < kb && c < kb) A[r + (size_t)M.ld * c] = T[r + TLD * c];
-    }
-  }
+  return failc_s;
+}
+
+// Complete W = inv(L) of a block factored by chol64_factor (256 threads;
+// T is only read, so the factor may be written out concurrently).  Ends
+// with a barrier: W is complete for every thread.
+__device__ __forceinline__ void chol64_inverse_finish(Chol64Lds& L) {
+  double* T = L.T;
+  double* W = L.W;
+  double* S = L.S;
+  double* S2 = L.S2;
+  double* rdiag = L.rdiag;
+  constexpr int SLD = 33;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  (void)t;
   // D_3 (wave 1) alongside U = L[B][A] W[A][A] (A = 0:32, B = 32:64; waves
   // 0, 2, 3 share its four 16x16 tiles)
   if (wv == 1) {
@@ -487,6 +472,72 @@ __device__ __forceinline__ void diag_block(const MatDesc& M, int b, int k0, int 
     for (int v = 0; v < 4; ++v) W[(32 + 16 * ti + li) + TLD * (16 * tj + 4 * v + lk)] = -x[v];
   }
   __syncthreads();
+}
+
+// The diagonal-block work of one item (256 threads).  kp >= 0: first apply
+// the update by the 64 columns [kp, kp + 64) of L to the block,
+// A[k0.., k0..] -= L[k0.., kp..] L[k0.., kp..]^T (the block's share of the
+// previous step's rest-of-panel update).
+template <int MODE>
+__device__ __forceinline__ void diag_block(const MatDesc& M, int b, int k0, int kp,
+                                           int* __restrict__ info, double* __restrict__ dinv,
+                                           int nslot) {
+  const int kb = min(TB, M.n - k0);
+  __shared__ Chol64Lds L;
+  double* T = L.T;
+  double* W = L.W;
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+  double* A = M.ptr + k0 + (size_t)M.ld * k0;
+  {
+    // a thread's 16 loads in flight at once, then the LDS stores
+    double a[16];
+    const int r = t & 63, cq = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cq + 4 * q;
+      a[q] = (r < kb && c < kb) ? A[r + (size_t)M.ld * c] : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[r + TLD * (cq + 4 * q)] = a[q];
+    const double* X = M.ptr + k0 + (size_t)M.ld * (kp < 0 ? 0 : kp);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = (kp >= 0 && r < kb) ? X[r + (size_t)M.ld * (cq + 4 * q)] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = a[q];
+  }
+  if (t == 0) L.failc = 0;
+  __syncthreads();
+  if (kp >= 0) {
+    // T[bi][bj] -= X[bi] X[bj]^T on the ten lower 16x16 tiles (K = 64)
+    for (int q = wv; q < 10; q += 4) {
+      int bj = 0, rem = q;
+      while (rem >= 4 - bj) { rem -= 4 - bj; ++bj; }
+      const int bi = bj + rem;
+      const v4d acc = mma16_lds<1>(&W[16 * bi], TLD, &W[16 * bj], TLD, 64);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) T[(16 * bi + li) + TLD * (16 * bj + 4 * v + lk)] -= acc[v];
+    }
+    __syncthreads();
+    const int r = t & 63, cq = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = 0.0;
+    __syncthreads();
+  }
+  const int failc = chol64_factor<MODE>(L, kb);
+  if (failc) {
+    if (t == 0) info[b] = k0 + failc;
+    return;
+  }
+  // write the factor back (lower part of the block)
+  {
+    const int r = t & 63, cq = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cq + 4 * q;
+      if (r >= c && r < kb && c < kb) A[r + (size_t)M.ld * c] = T[r + TLD * c];
+    }
+  }
+  chol64_inverse_finish(L);
   double* Dv = dinv + ((size_t)b * nslot + k0 / TB) * TB * TB;
   for (int e = t; e < TB * TB; e += 256) Dv[e] = W[(e & 63) + TLD * (e >> 6)];
 }
@@ -720,6 +771,389 @@ static void launch_syrk(hipStream_t s, const MatDesc* d_mats, const int* d_info,
     launch_syrk_t<2, 2, 4>(s, d_mats, d_info, nitems, rows, cols, k0, kw, cb, ce, maxwaves);
 }
 
+// ---------------------------------------------------------------------------
+// One outer panel [K0, K0 + 256) of a single large matrix in ONE launch.
+// Work-group b owns the 64 rows r0 = K0 + 64 b .. r0 + 63 of the panel, held
+// in registers for the whole launch: wave w keeps rows r0 + 16 w + (lane %
+// 16) and columns K0 + 4 s + lane / 16 (s = 0..63) in P[s] -- exactly the
+// MFMA operand and result layout, so the solves and updates of the row
+// block never leave the registers.  Step k (64 columns) of the panel:
+//   - work-group k factors its (updated) 64x64 diagonal block in LDS (the
+//     shared chol64 routines, SDPA's pivot rule in MODE 1) and publishes the
+//     block's inverse W_k;
+//   - every work-group b > k solves its block of column block k,
+//     L_bk = A_bk W_k^T (MFMA), stores it, publishes it if b < 4 (the rows
+//     of the panel's later diagonal blocks), and updates its column blocks
+//     k < j <= min(b, 3): A_bj -= L_bk L_jk^T.
+// Work-group b only ever waits for work-groups with a smaller index (k < b,
+// j < b), which the dispatcher has started before it, so the launch needs no
+// co-residency.  Values pass between work-groups as epoch-tagged 64-bit
+// words (low and high halves of a double, each with the launch's epoch in
+// the upper 32 bits, written and read with agent-scope atomics): a reader
+// that sees its epoch in both words has the value, with no fence and no
+// cache maintenance.  The chain from one diagonal block to the next is then
+// a single work-group's: fetch W_k, solve its own 64 x 64, update its
+// diagonal block, factor it -- without a kernel boundary.
+// tw layout (8192 words per 64x64 block): W_k at block k (k < 4), L_jk at
+// block 4 + 4 j + k; block 20 is the control word (WG 0's go / skip).
+constexpr int kPanelTagBlocks = 21;
+constexpr int kTagSpin = 1 << 22;  // bounded waits: a timed-out wait sets *err
+
+typedef unsigned long long u64;
+
+__device__ __forceinline__ void tag_store(u64* dst, double x, unsigned epoch) {
+  const u64 e = (u64)epoch << 32;
+  __hip_atomic_store(dst, (u64)(unsigned)__double2loint(x) | e, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(dst + 1, (u64)(unsigned)__double2hiint(x) | e, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The 64x64 block published at src (column-major element e = row + 64 col
+// in words 2e, 2e + 1) into LDS T (ld TLD); 256 threads, 16 elements each,
+// all loads issued before any wait.
+__device__ __forceinline__ void tag_fetch64(const u64* src, unsigned epoch, double* T, int* err) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {  // two rounds of 8 elements: 16 words in flight
+    u64 w[16];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = t + 256 * (8 * h + q);
+      w[2 * q] = __hip_atomic_load(src + 2 * e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      w[2 * q + 1] =
+          __hip_atomic_load(src + 2 * e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = t + 256 * (8 * h + q);
+      int spins = 0;
+      while ((unsigned)(w[2 * q] >> 32) != epoch || (unsigned)(w[2 * q + 1] >> 32) != epoch) {
+        // after one timed-out wait anywhere the launch stops waiting (its
+        // results are then discarded: the give-up flag is reported)
+        if (++spins > kTagSpin ||
+            ((spins & 1023) == 0 &&
+             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w[2 * q] = __hip_atomic_load(src + 2 * e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w[2 * q + 1] =
+            __hip_atomic_load(src + 2 * e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      T[(e & 63) + TLD * (e >> 6)] =
+          __hiloint2double((int)(unsigned)w[2 * q + 1], (int)(unsigned)w[2 * q]);
+    }
+  }
+}
+
+// acc[c] = sum over the 64 columns of register block KB of this wave's rows
+// times the rows 16 c + (lane % 16) of the 64x64 LDS operand X:
+// acc[c][v] = sum_t P_row[t] X[16 c + 4 v + lane / 16][t], i.e. the 16 x 64
+// block (rows of the wave) x X^T in the register layout of P.
+template <int KB>
+__device__ __forceinline__ void rows_times_xt(const double (&P)[64], const double* X, v4d acc[4]) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[(16 * c + li) + TLD * (4 * s + lk)],
+                                                    P[16 * KB + s], acc[c], 0, 0, 0);
+  }
+}
+
+// acc[c] += the same product (no zeroing).
+template <int KB>
+__device__ __forceinline__ void rows_times_xt_acc(const double (&P)[64], const double* X,
+                                                  v4d acc[4]) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[(16 * c + li) + TLD * (4 * s + lk)],
+                                                    P[16 * KB + s], acc[c], 0, 0, 0);
+  }
+}
+
+// The register block KB of the wave's rows into LDS T (the 64 x 64 block of
+// the work-group's rows, column-major, ld TLD).
+template <int KB>
+__device__ __forceinline__ void rows_to_lds(const double (&P)[64], double* T) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) T[(16 * w + li) + TLD * (4 * s + lk)] = P[16 * KB + s];
+}
+
+template <int KB>
+__device__ __forceinline__ void rows_publish(const double (&P)[64], u64* dst, unsigned epoch) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int e = (16 * w + li) + 64 * (4 * s + lk);
+    tag_store(dst + 2 * e, P[16 * KB + s], epoch);
+  }
+}
+
+template <int KB>
+__device__ __forceinline__ void rows_store(const double (&P)[64], double* A, int ld, int r0, int m,
+                                           int K0) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int row = r0 + 16 * w + li;
+  if (row >= m) return;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int col = K0 + 64 * KB + 4 * s + lk;
+    if (col < m) A[row + (size_t)ld * col] = P[16 * KB + s];
+  }
+}
+
+// Step K of the panel for a work-group below the diagonal block (b > K).
+template <int K, int MODE>
+__device__ __forceinline__ void panel_step_rows(double (&P)[64], Chol64Lds& L, int b, int nsteps,
+                                                double* A, int ld, int r0, int m, int K0,
+                                                const u64* tw, u64* twm, unsigned epoch,
+                                                int* err) {
+  // L_bK = A_bK W_K^T
+  tag_fetch64(tw + (size_t)K * 8192, epoch, L.T, err);
+  __syncthreads();
+  {
+    v4d acc[4];
+    rows_times_xt<K>(P, L.T, acc);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) P[16 * K + 4 * c + v] = acc[c][v];
+  }
+  rows_store<K>(P, A, ld, r0, m, K0);
+  if (b < 4) rows_publish<K>(P, twm + (size_t)(4 + 4 * b + K) * 8192, epoch);
+  // A_bj -= L_bK L_jK^T for the panel's later column blocks j <= b
+#pragma unroll
+  for (int j = K + 1; j < 4; ++j) {
+    if (j < nsteps && j <= b) {
+      __syncthreads();  // the LDS operand of the previous product is consumed
+      if (j < b) tag_fetch64(tw + (size_t)(4 + 4 * j + K) * 8192, epoch, L.T, err);
+      else rows_to_lds<K>(P, L.T);
+      __syncthreads();
+      v4d acc[4];
+      rows_times_xt<K>(P, L.T, acc);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) P[16 * j + 4 * c + v] -= acc[c][v];
+    }
+  }
+  __syncthreads();
+}
+
+// Step K for work-group K itself: factor the diagonal block of its rows.
+template <int K, int MODE>
+__device__ __forceinline__ void panel_step_diag(double (&P)[64], Chol64Lds& L, double* A, int ld,
+                                                int r0, int m, int K0, int* info, double* dinv,
+                                                u64* twm, unsigned epoch) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int kb = min(TB, m - r0);
+  // the block (identity beyond kb) into T, W cleared
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int r = 16 * w + li, c = 4 * s + lk;
+    L.T[r + TLD * c] = (r < kb && c < kb) ? P[16 * K + s] : (r == c ? 1.0 : 0.0);
+  }
+  for (int e = t; e < TB * TB; e += 256) L.W[(e & 63) + TLD * (e >> 6)] = 0.0;
+  if (t == 0) L.failc = 0;
+  __syncthreads();
+  const int failc = chol64_factor<MODE>(L, kb);
+  if (failc && t == 0) atomicCAS(info, 0, r0 + failc);
+  // the factor (lower part) to the matrix, beside the inverse's last steps
+  {
+    const int r = t & 63, cq = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cq + 4 * q;
+      if (r >= c && r < kb && c < kb) A[(r0 + r) + (size_t)ld * (r0 + c)] = L.T[r + TLD * c];
+    }
+  }
+  chol64_inverse_finish(L);
+  double* Dv = dinv + (size_t)(r0 / TB) * TB * TB;
+  u64* dst = twm + (size_t)K * 8192;
+  for (int e = t; e < TB * TB; e += 256) {
+    const double x = L.W[(e & 63) + TLD * (e >> 6)];
+    tag_store(dst + 2 * e, x, epoch);
+    Dv[e] = x;
+  }
+}
+
+// The previous panel's update of one work-group's rows (the left-looking
+// prologue): A_bj -= L_b,prev L_j,prev^T for the column blocks j <= b of the
+// panel, K = the previous panel's 256 columns [Kp, Kp + 256) in 64-column
+// chunks; L_b,prev sits in registers (Y, the layout of P), each 64 x 64
+// chunk of L_j,prev is staged in LDS, the next chunk loaded while the current
+// one is multiplied.
+__device__ __forceinline__ void chunk_load(const double* A, int ld, int m, int row0, int col0,
+                                           double (&pre)[16]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = t + 256 * q, r = row0 + (e & 63), c = col0 + (e >> 6);
+    pre[q] = r < m ? A[r + (size_t)ld * c] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void chunk_to_lds(const double (&pre)[16], double* T) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = t + 256 * q;
+    T[(e & 63) + TLD * (e >> 6)] = pre[q];
+  }
+}
+
+__device__ __forceinline__ void panel_prologue(double (&P)[64], Chol64Lds& L, const double* A,
+                                               int ld, int m, int K0, int Kp, int r0, int b,
+                                               int nsteps) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int nj = min(b + 1, nsteps);  // column blocks below the diagonal
+  double Y[64];
+  {
+    const int row = r0 + 16 * w + li;
+    const bool rin = row < m;
+    const double* src = A + (rin ? row : 0) + (size_t)ld * Kp;
+#pragma unroll
+    for (int s2 = 0; s2 < 64; ++s2) Y[s2] = rin ? src[(size_t)ld * (4 * s2 + lk)] : 0.0;
+  }
+  double pre[16];
+  chunk_load(A, ld, m, K0, Kp, pre);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j < nj) {
+      v4d acc[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        __syncthreads();  // the previous chunk is consumed
+        chunk_to_lds(pre, L.T);
+        __syncthreads();
+        // the next chunk: (j, q + 1), or (j + 1, 0)
+        const int nq = q < 3 ? q + 1 : 0, njj = q < 3 ? j : j + 1;
+        if (njj < nj) chunk_load(A, ld, m, K0 + 64 * njj, Kp + 64 * nq, pre);
+        if (q == 0) rows_times_xt_acc<0>(Y, L.T, acc);
+        if (q == 1) rows_times_xt_acc<1>(Y, L.T, acc);
+        if (q == 2) rows_times_xt_acc<2>(Y, L.T, acc);
+        if (q == 3) rows_times_xt_acc<3>(Y, L.T, acc);
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) P[16 * j + 4 * c + v] -= acc[c][v];
+    }
+  }
+  __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_panel_rows(double* __restrict__ A, int ld, int m,
+                                                       int K0, int Kp, int* __restrict__ info,
+                                                       double* __restrict__ dinv,
+                                                       u64* __restrict__ tw, unsigned epoch,
+                                                       int* __restrict__ err) {
+  __shared__ Chol64Lds L;
+  __shared__ int skip_s;
+  const int b = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int r0 = K0 + 64 * b;
+  const int ncols = min(256, m - K0);
+  const int nsteps = (ncols + 63) / 64;
+  // go / skip: work-group 0 reads the factorization's info once (an earlier
+  // launch may have failed) and publishes the decision; the others take it
+  // from there, so all of them see the same decision
+  u64* ctrl = tw + (size_t)20 * 8192;
+  if (t == 0) {
+    int skip = 0;
+    if (b == 0) {
+      skip = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      __hip_atomic_store(ctrl, (u64)(unsigned)skip | ((u64)epoch << 32), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      u64 c = __hip_atomic_load(ctrl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int spins = 0;
+      while ((unsigned)(c >> 32) != epoch) {
+        if (++spins > kTagSpin) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          c = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        c = __hip_atomic_load(ctrl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      skip = (int)(c & 1);
+    }
+    skip_s = skip;
+  }
+  // the row block into registers (rows beyond m and columns beyond the
+  // matrix read as zero)
+  double P[64];
+  {
+    const int row = r0 + 16 * w + li;
+    const bool rin = row < m;
+    const double* src = A + (rin ? row : 0) + (size_t)ld * K0;
+#pragma unroll
+    for (int s = 0; s < 64; ++s) {
+      const int col = 4 * s + lk;
+      P[s] = (rin && col < ncols) ? src[(size_t)ld * col] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (skip_s) return;
+  if (Kp >= 0) panel_prologue(P, L, A, ld, m, K0, Kp, r0, b, nsteps);
+#define PANEL_STEP(K)                                                                      \
+  if (K < nsteps) {                                                                        \
+    if (b == K) {                                                                          \
+      /* the work-group's last step: its later column blocks are above the diagonal */    \
+      panel_step_diag<K, MODE>(P, L, A, ld, r0, m, K0, info, dinv, tw, epoch);             \
+      return;                                                                              \
+    }                                                                                      \
+    panel_step_rows<K, MODE>(P, L, b, nsteps, A, ld, r0, m, K0, tw, tw, epoch, err);       \
+  }
+  PANEL_STEP(0)
+  PANEL_STEP(1)
+  PANEL_STEP(2)
+  PANEL_STEP(3)
+#undef PANEL_STEP
+}
+
+int DenseWork::ensure_panel_tags() {
+  if (d_ptag) return 0;
+  SDPA_HIP_CHECK(hipMalloc(&d_ptag, sizeof(unsigned long long) * kPanelTagBlocks * 8192));
+  SDPA_HIP_CHECK(hipMemset(d_ptag, 0, sizeof(unsigned long long) * kPanelTagBlocks * 8192));
+  SDPA_HIP_CHECK(hipDeviceSynchronize());
+  ptag_epoch = 0;
+  return 0;
+}
+
+// Factor the outer panel [K0, K0 + 256) of the single m x m matrix A by one
+// launch of k_panel_rows, after applying the previous panel [Kp, Kp + 256)
+// to it (Kp < 0: none).  A's 64x64 diagonal inverses go to dinv, slot
+// k0 / 64.
+static int factor_panel_rows(hipStream_t s, DenseWork* w, double* A, int ld, int m, int K0,
+                             int Kp, int mode, int* d_info, double* dinv) {
+  if (int e = w->ensure_panel_tags()) return e;
+  if (int e = w->ensure_flags(0)) return e;  // the give-up flag
+  if (++w->ptag_epoch == 0) ++w->ptag_epoch;
+  const int R = (m - K0 + 63) / 64;
+  if (mode == 0)
+    hipLaunchKernelGGL(k_panel_rows<0>, dim3(R), dim3(256), 0, s, A, ld, m, K0, Kp, d_info,
+                       dinv, w->d_ptag, w->ptag_epoch, w->d_err);
+  else
+    hipLaunchKernelGGL(k_panel_rows<1>, dim3(R), dim3(256), 0, s, A, ld, m, K0, Kp, d_info,
+                       dinv, w->d_ptag, w->ptag_epoch, w->d_err);
+  return 0;
+}
+
 // Factor the outer panel [K0, K0 + PW): 64-column steps.  Step 0 is the
 // diagonal block alone; every later step is one fused launch (the previous
 // step's rest-of-panel update with the next diagonal block); each step ends
@@ -802,6 +1236,18 @@ int DenseWork::ensure_trailing_events(int n) {
   return 0;
 }
 
+int DenseWork::ensure_panel_events(int n) {
+  while ((int)evq0.size() < n) {
+    hipEvent_t a, b;
+    SDPA_HIP_CHECK(hipEventCreate(&a));
+    SDPA_HIP_CHECK(hipEventCreate(&b));
+    evq0.push_back(a);
+    evq1.push_back(b);
+    evq_flops.push_back(0.0);
+  }
+  return 0;
+}
+
 int DenseWork::collect_trailing() {
   for (int q = 0; q < trail_pending; ++q) {
     float ms = 0.f;
@@ -811,6 +1257,14 @@ int DenseWork::collect_trailing() {
     ++trail_launches;
   }
   trail_pending = 0;
+  for (int q = 0; q < panel_pending; ++q) {
+    float ms = 0.f;
+    SDPA_HIP_CHECK(hipEventElapsedTime(&ms, evq0[q], evq1[q]));
+    panel_ms += ms;
+    panel_flops += evq_flops[q];
+    ++panel_launches;
+  }
+  panel_pending = 0;
   return 0;
 }
 
@@ -840,24 +1294,44 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
     }
   } else {
     // Look-ahead.  Panel p's trailing update is split by columns: panel
-    // p+1's (on `s`, right after the panel: the next panel is factored on
-    // the same stream without a cross-stream wait), panel p+2's and the rest
-    // (both on the CU-masked side stream, beside the next panels).  Every
-    // update of a column block is ordered after the earlier ones that write
-    // it: panel p+1's columns are last written on the side by panel p-1's
-    // second launch (evn[p-1], which the side stream's order puts after all
-    // earlier side launches); the side launches of panel p read the panel
-    // only after it is factored (evp[p]).
+    // p+1's is the prologue of panel p+1's own launch on `s` (no separate
+    // launch, no cross-stream wait in front of the next panel), panel p+2's
+    // and the rest run on the CU-masked side stream beside the next panels.
+    // Every update of a column block is ordered after the earlier ones that
+    // write it; the side launches of panel p read the panel only after it is
+    // factored (evp[p]).
     if (int e = w->ensure_side(np)) return e;
     SDPA_HIP_CHECK(hipEventRecord(w->evjoin, s));
     SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evjoin, 0));
     for (int p = 0; p < np; ++p) {
       const int K0 = p * PW;
-      factor_panel(s, w->d_mats, 1, maxn, K0, PW, mode, d_info, dinv, nslot);
+      // panel p's columns carry the updates of panels 0 .. p-2 once the side
+      // stream has run panel p-2's launch for them (evn[p-2], which the side
+      // stream's order puts after every earlier side launch); panel p-1's
+      // update is the panel kernel's own prologue
+      if (p >= 2) SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 2], 0));
+      if (w->time_trailing) {
+        if (int e = w->ensure_panel_events(w->panel_pending + 1)) return e;
+        SDPA_HIP_CHECK(hipEventRecord(w->evq0[w->panel_pending], s));
+      }
+      if (int e = factor_panel_rows(s, w, h_items[0].ptr, h_items[0].ld, maxn, K0,
+                                    p > 0 ? K0 - PW : -1, mode, d_info, dinv))
+        return e;
+      if (w->time_trailing) {
+        SDPA_HIP_CHECK(hipEventRecord(w->evq1[w->panel_pending], s));
+        // algorithmic flops of the launch: the previous panel's update of this
+        // panel's lower part (2 K per element, K = 256) and the panel's own
+        // Cholesky (w^2 h - w^3 / 3 for an h x w panel)
+        const double h = maxn - K0, wc = std::min(PW, maxn - K0);
+        const double elems = wc * h - wc * (wc - 1.0) / 2.0;
+        w->evq_flops[w->panel_pending] = (p > 0 ? 2.0 * PW * elems : 0.0) + wc * wc * h -
+                                         wc * wc * wc / 3.0;
+        ++w->panel_pending;
+      }
       if (K0 + PW >= maxn) break;
       SDPA_HIP_CHECK(hipEventRecord(w->evp[p], s));
-      if (p > 0) SDPA_HIP_CHECK(hipStreamWaitEvent(s, w->evn[p - 1], 0));
-      launch_syrk(s, w->d_mats, d_info, 1, maxn, K0, PW, K0 + PW, K0 + 2 * PW);
+      // the side stream, once panel p is factored: the panel-after-next's
+      // columns, then the rest
       SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
       launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, K0 + 3 * PW,
                   w->side_waves);

@@ -205,8 +205,7 @@ int min_eig(Ctx* c, double* lamX, double* lamZ) {
       // one large block: plain GEMMs through the library
       for (int k = 0; k < 4; ++k) {
         const GemmItem& g = it[k];
-        if (int e = blas_gemm(c, 0, k < 2 ? 1 : 0, g.M, g.N, g.K, 1.0, g.A, g.lda, g.B, g.ldb,
-                              0.0, g.C, g.ldc))
+        if (int e = dgemm(c->stream, 0, k < 2 ? 1 : 0, g.M, g.N, g.K, 1.0, g.A, g.lda, 0, g.B, g.ldb, 0, 0.0, g.C, g.ldc, 0, 1))
           return e;
       }
     } else {

@@ -235,7 +235,7 @@ int triple_product(Ctx* c, const double* A, const double* B, const double* Cm, d
     if (nb == 1 && maxn >= kBlasMinN) {
       for (int k = 0; k < 2; ++k) {  // one large block: plain GEMMs through the library
         const GemmItem& g = it[k];
-        if (int e = blas_gemm(c, 0, 0, g.M, g.N, g.K, 1.0, g.A, g.lda, g.B, g.ldb, 0.0, g.C, g.ldc))
+        if (int e = dgemm(c->stream, 0, 0, g.M, g.N, g.K, 1.0, g.A, g.lda, 0, g.B, g.ldb, 0, 0.0, g.C, g.ldc, 0, 1))
           return e;
       }
     } else {
@@ -426,7 +426,7 @@ int inverse_chol(Ctx* c) {
     }
     if (nb == 1 && maxn >= kBlasMinN) {
       const GemmItem& g = it[0];  // one large block: a plain GEMM through the library
-      if (int e = blas_gemm(c, 1, 0, g.M, g.N, g.K, 1.0, g.A, g.lda, g.B, g.ldb, 0.0, g.C, g.ldc))
+      if (int e = dgemm(c->stream, 1, 0, g.M, g.N, g.K, 1.0, g.A, g.lda, 0, g.B, g.ldb, 0, 0.0, g.C, g.ldc, 0, 1))
         return e;
     } else {
       if (int e = c->dw.ensure_items(nb)) return e;

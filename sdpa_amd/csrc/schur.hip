@@ -604,6 +604,8 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
   // chunk so that densified A, X*A and G each fit in a 3 x 512 MiB budget
   size_t budget = (size_t)512 << 20;
   int chunk = (int)std::max<size_t>(1, budget / (nn * sizeof(double)));
+  // whole 128-column tiles of the Abar^T Gbar product when the chunk is wide
+  if (chunk >= 128) chunk -= chunk % 128;
   if (chunk > T.nF1) chunk = T.nF1;
   size_t need = 3 * (size_t)chunk * nn;
   if (need > c->capG) {
@@ -642,11 +644,11 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
                            T.d_estart, T.d_ecnt, T.d_dpos, c->d_er, c->d_ec, c->d_ev, T.d_dense, n,
                            Ad);
     }
-    // XA_t = X A_t and G_t = XA_t Z^-1 for the chunk: plain GEMMs (rocBLAS)
-    if (int e = blas_gemm_strided(c, 0, 0, n, n, n, 1.0, X, n, 0, At, n, (long)nn, 0.0, XA, n,
+    // XA_t = X A_t and G_t = XA_t Z^-1 for the chunk (strided-batched MFMA GEMMs)
+    if (int e = dgemm(c->stream, 0, 0, n, n, n, 1.0, X, n, 0, At, n, (long)nn, 0.0, XA, n,
                                   (long)nn, cnt))
       return e;
-    if (int e = blas_gemm_strided(c, 0, 0, n, n, n, 1.0, XA, n, (long)nn, Zi, n, 0, 0.0, G, n,
+    if (int e = dgemm(c->stream, 0, 0, n, n, n, 1.0, XA, n, (long)nn, Zi, n, 0, 0.0, G, n,
                                   (long)nn, cnt))
       return e;
     if (T.ndense > 0) {
@@ -660,8 +662,7 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
       int ks = (int)std::min<long>(32, std::max<long>(1, 1024 / tiles));
       if (K / ks < 1024) ks = (int)std::max<long>(1, K / 1024);
       if (ks <= 1) {
-        if (int e = blas_gemm(c, 1, 0, M, N, (int)K, 1.0, T.d_dense, (int)nn, G, (int)nn, 0.0,
-                              c->d_F1C, M))
+        if (int e = dgemm(c->stream, 1, 0, M, N, (int)K, 1.0, T.d_dense, (int)nn, 0, G, (int)nn, 0, 0.0, c->d_F1C, M, 0, 1))
           return e;
       } else {
         const long Kc = K / ks, rem = K - Kc * ks;
@@ -672,12 +673,11 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
           c->capF1P = (size_t)nparts * len;
           SDPA_HIP_CHECK(hipMalloc(&c->d_F1P, c->capF1P * sizeof(double)));
         }
-        if (int e = blas_gemm_strided(c, 1, 0, M, N, (int)Kc, 1.0, T.d_dense, (int)nn, Kc, G,
+        if (int e = dgemm(c->stream, 1, 0, M, N, (int)Kc, 1.0, T.d_dense, (int)nn, Kc, G,
                                       (int)nn, Kc, 0.0, c->d_F1P, M, (long)len, ks))
           return e;
         if (rem > 0)
-          if (int e = blas_gemm(c, 1, 0, M, N, (int)rem, 1.0, T.d_dense + Kc * ks, (int)nn,
-                                G + Kc * ks, (int)nn, 0.0, c->d_F1P + (size_t)ks * len, M))
+          if (int e = dgemm(c->stream, 1, 0, M, N, (int)rem, 1.0, T.d_dense + Kc * ks, (int)nn, 0, G + Kc * ks, (int)nn, 0, 0.0, c->d_F1P + (size_t)ks * len, M, 0, 1))
             return e;
         hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((len + 255) / 256)), dim3(256), 0,
                            c->stream, c->d_F1P, nparts, len, c->d_F1C);

@@ -79,6 +79,11 @@ struct Ent {
   double v;
 };
 
+__global__ void k_shift_diag(double* B, int m, double shift) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < m) B[i + (size_t)m * i] += shift;
+}
+
 }  // namespace
 
 extern "C" {
@@ -107,7 +112,6 @@ int sdpa_gpu_destroy(sdpa_gpu_ctx* h) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_problem(c);
-  blas_release(c);
   c->dw.release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   if (c->h_red) (void)hipHostFree(c->h_red);
@@ -565,6 +569,7 @@ int sdpa_gpu_set_buffer(sdpa_gpu_ctx* h, int which, const double* src, long len)
   if (!p || len != L) return fail(c, -20, "buffer id or length mismatch");
   SDPA_HIP_CHECK(hipMemcpyAsync(p, src, sizeof(double) * L, hipMemcpyHostToDevice, c->stream));
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (which == 10) c->bshifted = false;  // a new Schur matrix
   return 0;
 }
 
@@ -654,6 +659,7 @@ int sdpa_gpu_make_bmat(sdpa_gpu_ctx* h, int precision) {
   if (precision != 0) return fail(c, -30, "only FP64 (precision 0) is implemented");
   tic(c);
   int r = make_bmat(c);
+  c->bshifted = false;
   c->t_bmat += toc(c);
   c->n_bmat++;
   return r;
@@ -665,6 +671,16 @@ int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* h, int precision) {
   if (precision != 0 && precision != 1) return fail(c, -30, "precision must be 0 (FP64) or 1 (mixed)");
   tic(c);
   c->factor_mixed = 0;
+  if (!c->schurDense && !c->bshifted) {
+    // Chordal::factorizeSchur (sdpa_chordal.cpp:526-534): where the reference
+    // leaves the dense path it factors bMat + 1e-10 I (PLUS_ADJUST_DIAGONAL)
+    // with its sparse solver; the same shifted matrix is factored here, once
+    // per assembled matrix (a refactorization after a stalled mixed solve
+    // sees it already shifted), and the mixed path refines against it
+    hipLaunchKernelGGL(k_shift_diag, dim3((c->m + 255) / 256), dim3(256), 0, c->stream, c->d_B,
+                       c->m, 1.0e-10);
+    c->bshifted = true;
+  }
   if (precision == 1) {
     const int r = factor_mixed(c);
     if (r < 0) return r;
@@ -780,17 +796,23 @@ int sdpa_gpu_timing(sdpa_gpu_ctx* h, int enable, double* out) {
     c->dw.time_trailing = c->timing;
     c->dw.trail_ms = c->dw.trail_flops = 0.0;
     c->dw.trail_launches = 0;
+    c->dw.panel_ms = c->dw.panel_flops = 0.0;
+    c->dw.panel_launches = 0;
   }
   return 0;
 }
 
-int sdpa_gpu_kernel_timing(sdpa_gpu_ctx* h, double* out4) {
-  if (!h || !out4) return -1;
+int sdpa_gpu_kernel_timing(sdpa_gpu_ctx* h, double* out8) {
+  if (!h || !out8) return -1;
   Ctx* c = &h->c;
-  out4[0] = c->dw.trail_ms;
-  out4[1] = (double)c->dw.trail_launches;
-  out4[2] = c->dw.trail_flops;
-  out4[3] = 0.0;
+  out8[0] = c->dw.trail_ms;
+  out8[1] = (double)c->dw.trail_launches;
+  out8[2] = c->dw.trail_flops;
+  out8[3] = 0.0;
+  out8[4] = c->dw.panel_ms;
+  out8[5] = (double)c->dw.panel_launches;
+  out8[6] = c->dw.panel_flops;
+  out8[7] = 0.0;
   return 0;
 }
 

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -63,6 +64,18 @@ void SDPA::terminate() {
     ctx_ = nullptr;
   }
   prepared_ = false;
+}
+
+// Newton::setNumThreads (sdpa_newton.cpp:2127-2149)
+void SDPA::setNumThreads(int numThreads) {
+  if (numThreads == 0) {
+    const char* env = std::getenv("OMP_NUM_THREADS");
+    numThreads_ = env ? std::atoi(env) : 1;
+  } else {
+    numThreads_ = numThreads;
+  }
+  if (display_) std::fprintf(display_, "NumThreads  is set as %d\n", numThreads_);
+  if (fpout_) std::fprintf(fpout_, "NumThreads  is set as %d\n", numThreads_);
 }
 
 // Parameter::setDefaultParameter (sdpa_parts.cpp:182-224)

@@ -128,3 +128,29 @@ def test_mixed_beyond_float_solve_range_uses_fp64():
     assert r1.iterations == r0.iterations == 2
     assert st["factors"] == 0 and st["fallbacks"] >= 1
     assert r1.primal_obj == r0.primal_obj and r1.dual_obj == r0.dual_obj
+
+
+def test_mixed_mu_switch_hands_over_to_fp64():
+    # muSwitch > 0: the Schur matrix is factored in FP32 (with refinement)
+    # while mu > muSwitch and in FP64 from the first iteration with mu at or
+    # below it (SDPA::setPrecision).  Lovasz n = 60 (m = 550) starts at
+    # mu = 1e4: with muSwitch = 1 about half of the 13 factorizations are
+    # mixed; the result is the oracle's, as with FP64 throughout.
+    d = PR.lovasz(60, 0.3, 2)
+    o = O.Problem(d).solve()
+    s = sa.Solver(0).load(d).set_param(precision=1, muSwitch=1.0)
+    r = s.solve()
+    st = s.mixed_stats()
+    mus = [t[0] for t in r.trace]
+    above = sum(1 for mu in mus[:r.iterations] if mu > 1.0)
+    print(d.m, r.phase, r.iterations, st, "iterations with mu > 1:", above)
+    # one mixed factorization (or FP64 fallback inside it) per iteration above the switch
+    assert st["factors"] + st["fallbacks"] == above
+    assert 0 < above < r.iterations
+    assert st["current"] == 0  # the last factorization was FP64
+    assert r.phase == o.phase == "pdOPT"
+    assert abs(r.iterations - o.iterations) <= 1
+    assert r.primal_obj == pytest.approx(o.primal_obj, rel=1e-6, abs=1e-8)
+    assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)
+    assert r.relative_gap <= 1e-7
+    assert r.primal_error <= 1e-7 and r.dual_error <= 1e-7

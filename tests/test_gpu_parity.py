@@ -158,6 +158,36 @@ def test_factor_adjust_and_failure_rules():
     assert g.factor_bmat() == info_o > 0
 
 
+@pytest.mark.parametrize("m", [1300, 2100])
+def test_factor_adjust_and_failure_rules_single_launch_panels(m):
+    # matrices beyond two 256-column panels take the single-launch panel
+    # factorization: zero rows/columns (pivots below 1e-14 become 1e100) at
+    # panel, step and block edges, and an indefinite matrix whose first
+    # failing pivot lies in a later panel
+    rng = np.random.default_rng(m)
+    d = PR.maxcut(m, 0.5, seed=3)
+    g = ctx_for(d)
+    A = spd(m, rng, 1e3)
+    zero = np.array([0, 63, 64, 255, 256, 300, 511, 700, m - 1])
+    A[zero, :] = 0.0
+    A[:, zero] = 0.0
+    g.set("B", A.reshape(-1, order="F"))
+    assert g.factor_bmat() == 0
+    L = g.bmat()
+    _, Lo = O.chol_adjust(A)
+    big = np.diag(L) > 1e40
+    assert (np.flatnonzero(big) == zero).all()
+    tril = np.tril_indices(m)
+    np.testing.assert_allclose(L[tril], Lo[tril], rtol=1e-10, atol=1e-12)
+    # indefinite: a negative pivot appears past the first panels
+    B = spd(m, rng, 1e3)
+    B[900:, 900:] -= 5e3 * np.eye(m - 900)
+    g.set("B", B.reshape(-1, order="F"))
+    info_o, _ = O.chol_adjust(B)
+    assert info_o > 900
+    assert g.factor_bmat() == info_o
+
+
 def test_inverse_cholesky_matches_oracle():
     d = PR.multi_block(n_sdp_blocks=5, nmin=2, nmax=140, lp=7, m=20, density=0.05, seed=4)
     rng = np.random.default_rng(2)
@@ -291,3 +321,32 @@ def test_init_point_solve(golden_dir):
     r = s.solve()
     assert r.phase == "pdOPT"
     assert r.primal_obj == pytest.approx(-41.9, rel=1e-6)
+
+
+@pytest.mark.parametrize("per_block,dense", [(60, False), (50, True)])
+def test_sparse_schur_decision_shift(per_block, dense):
+    # block-separated constraints: m = 120 leaves the reference's dense path
+    # (Chordal::ordering_bMat), m = 100 stays on it.  Off the dense path the
+    # reference factors bMat + 1e-10 I (sdpa_chordal.cpp:526-534); on a
+    # diagonal matrix that is a closed form: L_ii = sqrt(d_i + 1e-10), against
+    # the dense rule's sqrt(d_i) and 1e50 for a zero pivot (d < 1e-14 -> 1e100)
+    d = PR.block_separated(per_block=per_block)
+    g = ctx_for(d)
+    assert bool(g.info[9]) == dense == O.Problem(d).schur_dense
+    m = d.m
+    dg = np.ones(m)
+    dg[5] = 0.0
+    dg[50] = 1e-13
+    g.set("B", np.diag(dg).reshape(-1, order="F"))
+    assert g.factor_bmat() == 0
+    L = np.diag(g.bmat())
+    want = np.sqrt(dg + 1e-10) if not dense else np.where(dg < 1e-14, 1e50, np.sqrt(dg))
+    np.testing.assert_allclose(L, want, rtol=1e-14)
+    # the whole solve against the oracle (which applies the same rule)
+    r = sa.Solver(0).load(d).solve()
+    o = O.Problem(d).solve()
+    print(d.name, r.phase, r.iterations, o.iterations, r.primal_obj, o.primal_obj)
+    assert r.phase == o.phase == "pdOPT"
+    assert abs(r.iterations - o.iterations) <= 1
+    assert r.primal_obj == pytest.approx(o.primal_obj, rel=1e-6, abs=1e-8)
+    assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)

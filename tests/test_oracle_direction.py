@@ -127,3 +127,22 @@ def test_direction_at_the_identity_start_is_the_solvers_first_step():
     # the direction from the start makes A(X + Dx) = b exactly
     M, _ = packed_mats(d)
     np.testing.assert_allclose(M[1:] @ (X + pred["Dx"]), d.c, rtol=1e-10, atol=1e-9)
+
+
+def test_sparse_schur_decision_solves_the_shifted_system():
+    # where Chordal::ordering_bMat leaves the dense path the reference factors
+    # bMat + 1e-10 I (sdpa_chordal.cpp:25, 526-534): the direction then
+    # satisfies A(Dx) = p - 1e-10 dy instead of A(Dx) = p (the dense path's
+    # equations hold to the rounding level on the same data)
+    for per_block, dense in ((60, False), (50, True)):
+        # two blocks: m = 120 (sparse decision) or m = 100 (m <= 100: dense)
+        d = PR.block_separated(per_block=per_block)
+        P = O.Problem(d)
+        assert P.schur_dense == dense
+        rng = np.random.default_rng(7)
+        X, y, Z = random_iterate(d, rng)
+        pred, _ = P.direction(X, y, Z, True, 0.5, 0.2, 0.1)
+        M, _ = packed_mats(d)
+        r = M[1:] @ pred["Dx"] - (d.c - M[1:] @ X)
+        want = np.zeros_like(r) if dense else -1.0e-10 * pred["dy"]
+        np.testing.assert_allclose(r, want, rtol=0, atol=1e-14 * (1 + np.abs(pred["dy"]).max()))
