@@ -64,7 +64,8 @@ KERNEL_NAME = {
     "panel": "k_panel_rows<1> (Schur Cholesky: one 256-column panel per launch, FP64 MFMA)",
     "trailing": "k_syrk_update<2,4,3> (Schur Cholesky wide trailing update, FP64 MFMA)",
 }
-PMC_MATCH = {"panel": "k_panel_rows<1>", "trailing": "k_syrk_update<2, 4, 3>"}
+PMC_MATCH = {"panel": "k_panel_rows<1>", "trailing": "k_syrk_update<2, 4, 3>",
+             "f3cols": "k_f3_cols", "f3multi": "k_f3_multi"}
 COPY_BYTES = 256 << 20
 
 CONFIGS = {
@@ -313,6 +314,17 @@ def pmc_summary(fetch_dir, write_dir, which):
     print("\n".join(lines))
 
 
+def pmc_counters(d, which):
+    """Mean per dispatch of every counter in a PMC run, over the dispatches of
+    one kernel (a PMC_MATCH key)."""
+    rows = _pmc_rows(d)
+    name = PMC_MATCH[which]
+    counters = sorted({r.get("Counter_Name") for r in rows if name in r.get("Kernel_Name", "")})
+    for counter in counters:
+        vals = [v for (n, v) in _per_dispatch(rows, counter).values() if name in n]
+        print(f"{counter}: {len(vals)} dispatches of {name}, mean {float(np.mean(vals)):.6e}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,10 +339,15 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, one 256 MiB device copy (PMC scaling)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "KERNEL"),
-                    help="KERNEL: panel or trailing")
+                    help="KERNEL: " + ", ".join(PMC_MATCH))
+    ap.add_argument("--pmc-counters", nargs=2, metavar=("DIR", "KERNEL"),
+                    help="mean per dispatch of each counter of a PMC run, for KERNEL")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
+        return
+    if args.pmc_counters:
+        pmc_counters(*args.pmc_counters)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "

@@ -77,6 +77,9 @@ struct Ctx {
   // j's position in the block's list, estart, nzeff
   int* d_jent = nullptr;
   std::vector<BlockTable> blk;
+  // one allocation holding every block's small tables (d_con ... d_dlist,
+  // d_rmeta, d_rval: the BlockTable pointers point into it; d_dense does not)
+  void* d_blkarena = nullptr;
   // SDP sparse entry pool (local row/col, value)
   int* d_er = nullptr;
   int* d_ec = nullptr;

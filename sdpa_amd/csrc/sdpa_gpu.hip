@@ -5,9 +5,12 @@
 #include "ctx.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
+#include <thread>
 
 using namespace sdpa_amd;
 
@@ -25,14 +28,47 @@ int fail(Ctx* c, int code, const char* msg) {
 }
 
 template <class T>
-int upload(Ctx* c, T** dst, const std::vector<T>& src) {
+int upload_raw(Ctx* c, T** dst, const T* src, size_t count) {
+  (void)c;
   if (*dst) SDPA_HIP_CHECK(hipFree(*dst));
   *dst = nullptr;
-  size_t bytes = sizeof(T) * (src.empty() ? 1 : src.size());
-  SDPA_HIP_CHECK(hipMalloc((void**)dst, bytes));
-  if (!src.empty())
-    SDPA_HIP_CHECK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  SDPA_HIP_CHECK(hipMalloc((void**)dst, sizeof(T) * (count ? count : 1)));
+  if (count) SDPA_HIP_CHECK(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
   return 0;
+}
+
+template <class T>
+int upload(Ctx* c, T** dst, const std::vector<T>& src) {
+  return upload_raw(c, dst, src.data(), src.size());
+}
+
+// Host threads for the loader's passes over the input records: the
+// machine's hardware threads, at most 16, about one per 2^20 units of work.
+int loader_threads(long work) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const long nt = std::min<long>(hw ? (long)hw : 1L, 16L);
+  return (int)std::max<long>(1L, std::min<long>(nt, work >> 20));
+}
+
+// f(t, begin, end) for t = 0..nt-1 over contiguous, ordered chunks of [0, n),
+// one host thread each (a chunk whose thread cannot be started runs here)
+template <class F>
+void par_chunks(int nt, long n, const F& f) {
+  if (nt <= 1) {
+    f(0, 0L, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t) {
+    const long b = n * t / nt, e = n * (t + 1) / nt;
+    try {
+      th.emplace_back([&f, t, b, e] { f(t, b, e); });
+    } catch (...) {
+      f(t, b, e);
+    }
+  }
+  for (auto& x : th) x.join();
 }
 
 int dalloc(double** p, size_t n) {
@@ -43,15 +79,35 @@ int dalloc(double** p, size_t n) {
   return 0;
 }
 
-void free_problem(Ctx* c) {
-  for (auto& T : c->blk) {
-    for (void* p : {(void*)T.d_con, (void*)T.d_estart, (void*)T.d_ecnt, (void*)T.d_nzeff,
-                    (void*)T.d_form, (void*)T.d_dpos, (void*)T.d_heavy, (void*)T.d_f1own,
-                    (void*)T.d_dcon, (void*)T.d_dense, (void*)T.d_rmeta, (void*)T.d_rval,
-                    (void*)T.d_dlist})
-      if (p) (void)hipFree(p);
+// Many small device arrays in one allocation and one copy: add() places each
+// array at a 256-byte aligned offset of a host image, commit() uploads the
+// image and points every added pointer at its array.
+struct Arena {
+  std::vector<unsigned char> h;
+  std::vector<std::pair<void**, size_t>> fix;
+  template <class T>
+  void add(T** dst, const std::vector<T>& v) {
+    const size_t off = (h.size() + 255) & ~(size_t)255;
+    h.resize(off + sizeof(T) * (v.empty() ? 1 : v.size()), 0);
+    if (!v.empty()) std::memcpy(h.data() + off, v.data(), sizeof(T) * v.size());
+    fix.push_back({(void**)dst, off});
   }
+  int commit(void** base) {
+    if (*base) SDPA_HIP_CHECK(hipFree(*base));
+    *base = nullptr;
+    SDPA_HIP_CHECK(hipMalloc(base, h.empty() ? 1 : h.size()));
+    if (!h.empty()) SDPA_HIP_CHECK(hipMemcpy(*base, h.data(), h.size(), hipMemcpyHostToDevice));
+    for (auto& f : fix) *f.first = (unsigned char*)*base + f.second;
+    return 0;
+  }
+};
+
+void free_problem(Ctx* c) {
+  for (auto& T : c->blk)
+    if (T.d_dense) (void)hipFree(T.d_dense);
   c->blk.clear();
+  if (c->d_blkarena) (void)hipFree(c->d_blkarena);
+  c->d_blkarena = nullptr;
   void** ptrs[] = {(void**)&c->d_er, (void**)&c->d_ec, (void**)&c->d_ev, (void**)&c->d_lpptr,
                    (void**)&c->d_lpcon, (void**)&c->d_lpval, (void**)&c->d_cptr, (void**)&c->d_cp1,
                    (void**)&c->d_cp2, (void**)&c->d_cv, (void**)&c->d_ppos, (void**)&c->d_pptr,
@@ -73,11 +129,6 @@ void free_problem(Ctx* c) {
   free_mixed(c);
   c->loaded = false;
 }
-
-struct Ent {
-  int r, c;
-  double v;
-};
 
 __global__ void k_shift_diag(double* B, int m, double shift) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -168,46 +219,144 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   c->total = off + c->nLP;
   c->nDim = nd + c->nLP;
 
-  // group the records: per (k, SDP block) entry lists in input order; C dense
+  // Group the records.  The SDP entries of constraints 1..m go to buckets
+  // (SDP block q, constraint k), laid out block by block and constraint by
+  // constraint, each bucket's entries in input order: a stable counting sort
+  // over ordered chunks of the records, one host thread per chunk, which
+  // also validates them.  C = -F0 and the LP records are taken afterwards in
+  // input order (duplicate LP coordinates of a constraint are summed).
   const int nS = c->nSDP;
   std::vector<double> hC(c->total, 0.0);
-  std::vector<std::vector<std::vector<Ent>>> ent(nS, std::vector<std::vector<Ent>>(m));
   std::vector<std::vector<std::pair<int, double>>> lpent(m);  // (coordinate, value)
-  for (long e = 0; e < nnz; ++e) {
-    const int k = K[e], l = L[e] - 1;
-    if (k < 0 || k > m || l < 0 || l >= nBlock) return fail(c, -12, "record out of range");
-    if (c->blockType[l] == 1) {
-      const int q = c->blockNumber[l], n = c->sdpN[q];
-      const int i = I[e] - 1, j = J[e] - 1;
-      if (i < 0 || j < 0 || i >= n || j >= n) return fail(c, -13, "index out of range");
-      if (k == 0) {
+  const long nb = (long)nS * m;
+  int nt = loader_threads(nnz);
+  while (nt > 1 && (long)nt * nb > (32L << 20)) --nt;  // two counters per bucket per thread
+  std::vector<std::vector<int>> tcnt(nt), tdg(nt);
+  std::vector<std::vector<long>> tother(nt);  // F0 and LP records of each chunk
+  std::vector<long> terr(nt, -1);
+  std::vector<int> tcode(nt, 0);
+  {
+    const int* bType = c->blockType.data();
+    const int* bNum = c->blockNumber.data();
+    const int* sN = c->sdpN.data();
+    par_chunks(nt, nnz, [&](int t, long b0, long b1) {
+      std::vector<int>& cnt = tcnt[t];
+      std::vector<int>& dg = tdg[t];
+      cnt.assign(nb, 0);
+      dg.assign(nb, 0);
+      for (long e = b0; e < b1; ++e) {
+        const int k = K[e], l = L[e] - 1;
+        int code = 0;
+        if (k < 0 || k > m || l < 0 || l >= nBlock) {
+          code = -12;
+        } else if (bType[l] == 1) {
+          const int q = bNum[l], n = sN[q];
+          const int i = I[e] - 1, j = J[e] - 1;
+          if (i < 0 || j < 0 || i >= n || j >= n) {
+            code = -13;
+          } else if (k == 0) {
+            tother[t].push_back(e);
+          } else {
+            const long b = (long)q * m + (k - 1);
+            ++cnt[b];
+            dg[b] += (i == j);
+          }
+        } else {
+          const int size = blockStruct[l] < 0 ? -blockStruct[l] : 1;
+          if (I[e] != J[e]) code = -14;
+          else if (I[e] < 1 || I[e] > size) code = -13;
+          else tother[t].push_back(e);
+        }
+        if (code) {
+          terr[t] = e;
+          tcode[t] = code;
+          return;
+        }
+      }
+    });
+  }
+  // the chunks are in input order: the first chunk with an error holds the
+  // first erroneous record
+  for (int t = 0; t < nt; ++t) {
+    if (terr[t] < 0) continue;
+    if (tcode[t] == -12) return fail(c, -12, "record out of range");
+    if (tcode[t] == -14) return fail(c, -14, "LP entry must be diagonal");
+    return fail(c, -13, "index out of range");
+  }
+  for (int t = 0; t < nt; ++t)
+    for (long e : tother[t]) {
+      const int k = K[e], l = L[e] - 1;
+      if (c->blockType[l] == 1) {
+        const int q = c->blockNumber[l], n = c->sdpN[q];
+        const int i = I[e] - 1, j = J[e] - 1;
         double* Cq = hC.data() + c->sdpOff[q];
         Cq[i + (size_t)n * j] += -V[e];
         if (i != j) Cq[j + (size_t)n * i] += -V[e];
       } else {
-        ent[q][k - 1].push_back(Ent{i, j, V[e]});
-      }
-    } else {
-      if (I[e] != J[e]) return fail(c, -14, "LP entry must be diagonal");
-      const int t = c->blockNumber[l] + I[e] - 1;
-      if (I[e] < 1 || t >= c->nLP) return fail(c, -13, "LP index out of range");
-      if (k == 0) hC[c->lpOff + t] += -V[e];
-      else {
-        auto& le = lpent[k - 1];
-        bool found = false;
-        for (auto& pr : le)
-          if (pr.first == t) { pr.second += V[e]; found = true; }
-        if (!found) le.push_back({t, V[e]});
+        const int t2 = c->blockNumber[l] + I[e] - 1;
+        if (k == 0) {
+          hC[c->lpOff + t2] += -V[e];
+        } else {
+          auto& le = lpent[k - 1];
+          bool found = false;
+          for (auto& pr : le)
+            if (pr.first == t2) { pr.second += V[e]; found = true; }
+          if (!found) le.push_back({t2, V[e]});
+        }
       }
     }
+  tother.clear();
+  // bucket sizes, diagonal counts and starts; each chunk's write offsets
+  std::vector<long> bstart(nb + 1, 0);
+  std::vector<int> bcnt(nb), bdg(nb);
+  for (long b = 0; b < nb; ++b) {
+    long sz = 0, d = 0;
+    for (int t = 0; t < nt; ++t) { sz += tcnt[t][b]; d += tdg[t][b]; }
+    if (sz > INT_MAX) return fail(c, -15, "more than 2^31 - 1 SDP entries");
+    bcnt[b] = (int)sz;
+    bdg[b] = (int)d;
+    bstart[b + 1] = bstart[b] + sz;
   }
+  const long nentAll = bstart[nb];
+  if (nentAll > INT_MAX) return fail(c, -15, "more than 2^31 - 1 SDP entries");
+  for (long b = 0; b < nb; ++b) {
+    long o = bstart[b];
+    for (int t = 0; t < nt; ++t) {
+      const int x = tcnt[t][b];
+      tcnt[t][b] = (int)o;
+      o += x;
+    }
+  }
+  tdg.clear();
+  std::unique_ptr<int[]> ber(new int[nentAll ? nentAll : 1]), bec(new int[nentAll ? nentAll : 1]);
+  std::unique_ptr<double[]> bev(new double[nentAll ? nentAll : 1]);
+  int* h_er = ber.get();
+  int* h_ec = bec.get();
+  double* h_ev = bev.get();
+  {
+    const int* bType = c->blockType.data();
+    const int* bNum = c->blockNumber.data();
+    par_chunks(nt, nnz, [&](int t, long b0, long b1) {
+      int* off = tcnt[t].data();
+      for (long e = b0; e < b1; ++e) {
+        const int k = K[e];
+        if (k == 0) continue;
+        const int l = L[e] - 1;
+        if (bType[l] != 1) continue;
+        const int p = off[(long)bNum[l] * m + (k - 1)]++;
+        h_er[p] = I[e] - 1;
+        h_ec[p] = J[e] - 1;
+        h_ev[p] = V[e];
+      }
+    });
+  }
+  tcnt.clear();
 
   // per-block tables: changeToDense, index, formula (sdpa_newton.cpp:515-695)
   c->blk.assign(nS, BlockTable());
-  std::vector<int> h_er, h_ec;
-  std::vector<double> h_ev;
   c->nF[0] = c->nF[1] = c->nF[2] = 0;
   std::vector<int> maxNCon(1, 0);
+  Arena arena;  // the blocks' small tables, uploaded after the loop
   for (int q = 0; q < nS; ++q) {
     BlockTable& T = c->blk[q];
     T.n = c->sdpN[q];
@@ -215,12 +364,12 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     const int n = T.n;
     std::vector<int> con, ecnt, nzeff, isdense;
     for (int k = 0; k < m; ++k) {
-      if (ent[q][k].empty()) continue;
+      const long bk = (long)q * m + k;
+      if (bcnt[bk] == 0) continue;
       con.push_back(k);
-      int cnt = (int)ent[q][k].size();
-      bool dense = (double)cnt >= (double)n * n * 0.20;  // sdpa_struct.cpp:518
-      int ne = 0;
-      for (auto& x : ent[q][k]) ne += (x.r == x.c) ? 1 : 2;
+      const int cnt = bcnt[bk];
+      const bool dense = (double)cnt >= (double)n * n * 0.20;  // sdpa_struct.cpp:518
+      const int ne = 2 * cnt - bdg[bk];  // effective entries: off-diagonal ones count twice
       ecnt.push_back(cnt);
       nzeff.push_back(dense ? n * n : ne);
       isdense.push_back(dense ? 1 : 0);
@@ -264,16 +413,18 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       else if (f1 < f2 && f1 < f3) f = (nn <= 200 && 2.0 * nn >= jnz) ? 3 : 1;
       else if (f2 < f3) f = 2;
       else f = 3;
+      c->nF[f - 1]++;  // the reference's choice, reported by sdpa_gpu_problem_info
+      // The formula a pair is evaluated by: the reference's, except that a
+      // sparse F2 owner with at most kLightHost entries in a block of
+      // n <= kMultiN takes the F3 gather.  calF2 and calF3 evaluate the same
+      // sum A_i . (X A_j Z^-1) in other orders (sdpa_newton.cpp:905-936,
+      // 1319-1403); F2 here would run through G_j = X A_j Z^-1 as two n^3
+      // products per owner, a batched GEMM launch per block.
+      if (f == 2 && !isdense[k] && ecnt[k] <= kLightHost && n <= kMultiN) f = 3;
       form[k] = f;
-      c->nF[f - 1]++;
       if (f == 3) T.nF3++;
       else { T.nF1++; f1own.push_back(k); }
-      estart[k] = (int)h_er.size();
-      for (auto& x : ent[q][con[k]]) {
-        h_er.push_back(x.r);
-        h_ec.push_back(x.c);
-        h_ev.push_back(x.v);
-      }
+      estart[k] = (int)bstart[(long)q * m + con[k]];
       if (isdense[k]) { dpos[k] = T.ndense++; dcon.push_back(con[k]); }
       if (!isdense[k] && ecnt[k] > kLightHost) heavy.push_back(k);
     }
@@ -287,15 +438,15 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       for (int k = 0; k < nc; ++k) one += ecnt[k] == 1;
       T.f3cols = (n <= 256 && nc >= 64 && one * 10 >= (long)nc * 9) ? 1 : 0;
     }
-    if (int e = upload(c, &T.d_con, con)) return e;
-    if (int e = upload(c, &T.d_estart, estart)) return e;
-    if (int e = upload(c, &T.d_ecnt, ecnt)) return e;
-    if (int e = upload(c, &T.d_nzeff, nzeff)) return e;
-    if (int e = upload(c, &T.d_form, form)) return e;
-    if (int e = upload(c, &T.d_dpos, dpos)) return e;
-    if (int e = upload(c, &T.d_heavy, heavy)) return e;
-    if (int e = upload(c, &T.d_f1own, f1own)) return e;
-    if (int e = upload(c, &T.d_dcon, dcon)) return e;
+    arena.add(&T.d_con, con);
+    arena.add(&T.d_estart, estart);
+    arena.add(&T.d_ecnt, ecnt);
+    arena.add(&T.d_nzeff, nzeff);
+    arena.add(&T.d_form, form);
+    arena.add(&T.d_dpos, dpos);
+    arena.add(&T.d_heavy, heavy);
+    arena.add(&T.d_f1own, f1own);
+    arena.add(&T.d_dcon, dcon);
     if (n <= 256) {  // entry positions fit the record's 8-bit fields
       std::vector<int> rm(4 * (size_t)nc);
       std::vector<double> rv(nc, 0.0);
@@ -309,8 +460,8 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
                         (one ? (h_er[st] << 8) | (h_ec[st] << 16) : 0);
         if (one) rv[k] = h_ev[st];
       }
-      if (int e = upload(c, &T.d_rmeta, rm)) return e;
-      if (int e = upload(c, &T.d_rval, rv)) return e;
+      arena.add(&T.d_rmeta, rm);
+      arena.add(&T.d_rval, rv);
     }
     T.h_con = con;
     T.h_form = form;
@@ -323,24 +474,34 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       std::vector<int> dl(T.ndense, 0);
       for (int k = 0; k < nc; ++k)
         if (dpos[k] >= 0) dl[dpos[k]] = k;
-      if (int e = upload(c, &T.d_dlist, dl)) return e;
+      arena.add(&T.d_dlist, dl);
+      // one column of the pool per dense A_k (SparseMatrix::changeToDense,
+      // sdpa_struct.cpp:510-555: a later duplicate entry overwrites an
+      // earlier one), the columns filled by host threads
       const size_t nn2 = (size_t)n * n;
-      std::vector<double> pool(nn2 * T.ndense, 0.0);
-      for (int k = 0; k < nc; ++k) {
-        if (dpos[k] < 0) continue;
-        double* A = pool.data() + nn2 * dpos[k];
-        for (auto& x : ent[q][con[k]]) {
-          if (x.r == x.c) A[x.r + (size_t)n * x.c] = x.v;
-          else A[x.r + (size_t)n * x.c] = A[x.c + (size_t)n * x.r] = x.v;
+      std::unique_ptr<double[]> pool(new double[nn2 * T.ndense]);
+      double* pl = pool.get();
+      const int ntp = std::min(T.ndense, loader_threads((long)(nn2 * T.ndense)));
+      par_chunks(ntp, T.ndense, [&](int, long p0, long p1) {
+        for (long p = p0; p < p1; ++p) {
+          const int k = dl[p];
+          double* A = pl + nn2 * p;
+          std::fill(A, A + nn2, 0.0);
+          for (int e = estart[k]; e < estart[k] + ecnt[k]; ++e) {
+            const int r = h_er[e], cc = h_ec[e];
+            A[r + (size_t)n * cc] = h_ev[e];
+            if (r != cc) A[cc + (size_t)n * r] = h_ev[e];
+          }
         }
-      }
-      if (int e = upload(c, &T.d_dense, pool)) return e;
+      });
+      if (int e = upload_raw(c, &T.d_dense, pl, nn2 * T.ndense)) return e;
     }
   }
-  if (int e = upload(c, &c->d_er, h_er)) return e;
-  if (int e = upload(c, &c->d_ec, h_ec)) return e;
-  if (int e = upload(c, &c->d_ev, h_ev)) return e;
-  c->nent = (long)h_er.size();
+  if (int e = arena.commit(&c->d_blkarena)) return e;
+  if (int e = upload_raw(c, &c->d_er, h_er, (size_t)nentAll)) return e;
+  if (int e = upload_raw(c, &c->d_ec, h_ec, (size_t)nentAll)) return e;
+  if (int e = upload_raw(c, &c->d_ev, h_ev, (size_t)nentAll)) return e;
+  c->nent = nentAll;
 
   // constraint-major sparse terms (dense blocks excluded) and position-major transpose
   std::vector<int> cptr(m + 1, 0);
@@ -348,14 +509,14 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   std::vector<double> cv;
   for (int k = 0; k < m; ++k) {
     for (int q = 0; q < nS; ++q) {
-      const auto& es = ent[q][k];
-      if (es.empty()) continue;
+      const long b = (long)q * m + k;
+      if (bcnt[b] == 0) continue;
       const int n = c->sdpN[q];
-      if ((double)es.size() >= (double)n * n * 0.20) continue;  // dense: handled by the pool
-      for (auto& x : es) {
-        cp1.push_back(c->sdpOff[q] + x.r + (long)n * x.c);
-        cp2.push_back(c->sdpOff[q] + x.c + (long)n * x.r);
-        cv.push_back(x.v);
+      if ((double)bcnt[b] >= (double)n * n * 0.20) continue;  // dense: handled by the pool
+      for (long e = bstart[b]; e < bstart[b] + bcnt[b]; ++e) {
+        cp1.push_back(c->sdpOff[q] + h_er[e] + (long)n * h_ec[e]);
+        cp2.push_back(c->sdpOff[q] + h_ec[e] + (long)n * h_er[e]);
+        cv.push_back(h_ev[e]);
       }
     }
     for (auto& pr : lpent[k]) {
@@ -425,14 +586,18 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       // constraints sharing an SDP block or an LP coordinate, plus the
       // diagonal; dense when the full count exceeds 0.7 m^2
       const size_t words = ((size_t)m + 63) / 64;
-      std::vector<uint64_t> pat((size_t)m * words, 0ull);
+      std::vector<uint64_t> pat((size_t)m * words, 0ull), gs(words);
+      // row i of the pattern gains every member j <= i of the group g: the
+      // group's bit set, whole words below i's word and i's word masked
       auto mark = [&](const std::vector<int>& g) {
-        for (size_t a = 0; a < g.size(); ++a)
-          for (size_t b2 = 0; b2 <= a; ++b2) {
-            int i = g[a], j = g[b2];
-            if (i < j) std::swap(i, j);
-            pat[(size_t)i * words + (j >> 6)] |= 1ull << (j & 63);
-          }
+        std::fill(gs.begin(), gs.end(), 0ull);
+        for (int i : g) gs[i >> 6] |= 1ull << (i & 63);
+        for (int i : g) {
+          uint64_t* row = pat.data() + (size_t)i * words;
+          const int wi = i >> 6, bi = i & 63;
+          for (int w = 0; w < wi; ++w) row[w] |= gs[w];
+          row[wi] |= gs[wi] & (bi == 63 ? ~0ull : (2ull << bi) - 1);
+        }
       };
       for (int q = 0; q < nS; ++q) mark(c->blk[q].h_con);
       std::vector<int> g;
