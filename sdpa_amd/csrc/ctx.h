@@ -34,6 +34,11 @@ struct BlockTable {
   int* d_rmeta = nullptr;
   double* d_rval = nullptr;
   std::vector<int> h_con, h_form, h_dpos, h_ecnt, h_estart, h_f1own, h_nzeff;
+  // F1 chunks: the owners per chunk and, per chunk, the first dense-pool
+  // column that one of the chunk's owners owns (the Abar^T Gbar product
+  // starts there); set at the block's first assembly
+  int f1chunk = 0;
+  std::vector<int> f1rlo;
 };
 
 // one SDP block's F3 lists, for the kernel that takes all blocks at once
@@ -182,6 +187,10 @@ int min_eig(Ctx* c, double* lamX, double* lamZ);
 constexpr int kBlasMinN = 128;
 int dgemm(hipStream_t s, int ta, int tb, int M, int N, int K, double alpha, const double* A,
           int lda, long sA, const double* B, int ldb, long sB, double beta, double* C, int ldc,
+          long sC, int batch);
+// the same in FP32 (v_mfma_f32_16x16x4f32, FP32 accumulation)
+int sgemm(hipStream_t s, int ta, int tb, int M, int N, int K, float alpha, const float* A,
+          int lda, long sA, const float* B, int ldb, long sB, float beta, float* C, int ldc,
           long sC, int batch);
 // mixed.hip
 int factor_mixed(Ctx* c);  // 0 ok; otherwise the FP64 path must be used

@@ -20,6 +20,17 @@ namespace sdpa_amd {
 
 constexpr int kLight = 8;
 
+// Entry (u, v) of the symmetric n x n matrix M (column-major): read as
+// M[v + n u] (ByOwner = false: rows u of the owner's entries are the
+// columns) or, through the symmetry, as M[u + n v] (ByOwner = true: when the
+// lanes of a wave take consecutive owner entries, their reads of one partner
+// column are contiguous).
+template <bool ByOwner>
+__device__ __forceinline__ double sym_at(const double* __restrict__ M, int n, int u, int v) {
+  return ByOwner ? M[u + (size_t)n * v] : M[v + (size_t)n * u];
+}
+
+template <bool ByOwner = false>
 __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int* __restrict__ ec,
                                           const double* __restrict__ ev, int so, int co, int sp,
                                           int cp, int n, const double* __restrict__ X,
@@ -29,27 +40,24 @@ __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int*
   for (int e1 = e1_begin; e1 < co; e1 += e1_step) {
     const int a = er[so + e1], b = ec[so + e1];
     const double v1 = ev[so + e1];
-    const double* Xa = X + (size_t)n * a;
-    const double* Xb = X + (size_t)n * b;
-    const double* Za = Zi + (size_t)n * a;
-    const double* Zb = Zi + (size_t)n * b;
     double sum = 0.0;
-    if (a != b) {
-      for (int e2 = 0; e2 < cp; ++e2) {
-        const int g = er[sp + e2], d = ec[sp + e2];
-        double plu;
-        if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d] + Za[d] * Xb[g] + Za[g] * Xb[d];
-        else plu = Zb[d] * Xa[g] + Za[d] * Xb[g];
-        sum += ev[sp + e2] * plu;
+    for (int e2 = 0; e2 < cp; ++e2) {
+      const int g = er[sp + e2], d = ec[sp + e2];
+      // calF3's four terms: Zb[d] Xa[g] + Zb[g] Xa[d] + Za[d] Xb[g] + Za[g] Xb[d]
+      const double t1 = sym_at<ByOwner>(Zi, n, b, d) * sym_at<ByOwner>(X, n, a, g);
+      double plu;
+      if (a != b) {
+        if (g != d)
+          plu = t1 + sym_at<ByOwner>(Zi, n, b, g) * sym_at<ByOwner>(X, n, a, d) +
+                sym_at<ByOwner>(Zi, n, a, d) * sym_at<ByOwner>(X, n, b, g) +
+                sym_at<ByOwner>(Zi, n, a, g) * sym_at<ByOwner>(X, n, b, d);
+        else
+          plu = t1 + sym_at<ByOwner>(Zi, n, a, d) * sym_at<ByOwner>(X, n, b, g);
+      } else {
+        if (g != d) plu = t1 + sym_at<ByOwner>(Zi, n, b, g) * sym_at<ByOwner>(X, n, a, d);
+        else plu = t1;
       }
-    } else {
-      for (int e2 = 0; e2 < cp; ++e2) {
-        const int g = er[sp + e2], d = ec[sp + e2];
-        double plu;
-        if (g != d) plu = Zb[d] * Xa[g] + Zb[g] * Xa[d];
-        else plu = Zb[d] * Xa[g];
-        sum += ev[sp + e2] * plu;
-      }
+      sum += ev[sp + e2] * plu;
     }
     ret += v1 * sum;
   }
@@ -386,7 +394,7 @@ __global__ __launch_bounds__(256) void k_f3_heavy(
   const int o = (nzi > nzj) ? ki : kj;
   const int p = (o == ki) ? kj : ki;
   if (form[o] != 3) return;
-  double v = f3_pair(er, ec, ev, estart[o], ecnt[o], estart[p], ecnt[p], n, X, Zi, lane, 64);
+  double v = f3_pair<true>(er, ec, ev, estart[o], ecnt[o], estart[p], ecnt[p], n, X, Zi, lane, 64);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if (lane == 0) B[(size_t)con[ki] + (size_t)m * con[kj]] += v;
 }
@@ -556,23 +564,41 @@ __global__ __launch_bounds__(256) void k_f1_sparse(
   }
 }
 
-// Dense partners: P = Abar^T Gbar (ndense x chunk) was formed by a GEMM;
-// add the owned entries.  grid = (ceil(ndense/256), chunk).
+// Dense partners: P = Abar^T Gbar over the pool columns rlo..ndense-1
+// ((ndense - rlo) x chunk) was formed by a GEMM; add the owned entries.
+// grid = (ceil((ndense - rlo)/256), chunk).
 __global__ __launch_bounds__(256) void k_f1_scatter(const int* __restrict__ own, int first,
                                                     const int* __restrict__ dlist,
                                                     const int* __restrict__ con,
                                                     const int* __restrict__ nzeff, int ndense,
-                                                    const double* __restrict__ P,
+                                                    int rlo, const double* __restrict__ P,
                                                     double* __restrict__ B, int m) {
   const int jc = blockIdx.y;
-  const int id = blockIdx.x * 256 + threadIdx.x;
+  const int id = rlo + blockIdx.x * 256 + threadIdx.x;
   if (id >= ndense) return;
   const int j = own[first + jc];
   const int i = dlist[id];
   if (!owns(j, i, nzeff)) return;
   int gi = con[i], gj = con[j];
   int a = gi > gj ? gi : gj, b = gi > gj ? gj : gi;
-  B[(size_t)a + (size_t)m * b] += P[id + (size_t)ndense * jc];
+  B[(size_t)a + (size_t)m * b] += P[(id - rlo) + (size_t)(ndense - rlo) * jc];
+}
+
+// The first dense-pool column that an owner of the chunk [first, first + cnt)
+// of T's F1 owners owns (ndense if none): the rule of owns() on the host.
+static int f1_first_owned(const BlockTable& T, int first, int cnt) {
+  std::vector<int> local(T.ndense, -1);
+  for (int k = 0; k < T.ncon; ++k)
+    if (T.h_dpos[k] >= 0) local[T.h_dpos[k]] = k;
+  for (int id = 0; id < T.ndense; ++id) {
+    const int i = local[id];
+    for (int t = first; t < first + cnt; ++t) {
+      const int j = T.h_f1own[t];
+      const int jnz = T.h_nzeff[j], inz = T.h_nzeff[i];
+      if (!((jnz < inz) || (inz == jnz && i < j))) return id;
+    }
+  }
+  return T.ndense;
 }
 
 // ---- LP ---------------------------------------------------------------------
@@ -627,6 +653,12 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
   // dense-member list (local indices of dense constraints, in pool order),
   // built at load time
   const int* d_dlist = T.d_dlist;
+  if (T.ndense > 0 && (T.f1chunk != chunk || T.f1rlo.empty())) {
+    T.f1chunk = chunk;
+    T.f1rlo.clear();
+    for (int first = 0; first < T.nF1; first += chunk)
+      T.f1rlo.push_back(f1_first_owned(T, first, std::min(chunk, T.nF1 - first)));
+  }
   for (int first = 0; first < T.nF1; first += chunk) {
     const int cnt = std::min(chunk, T.nF1 - first);
     // the chunk's A_t: straight from the dense pool when its owners are
@@ -651,18 +683,23 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
     if (int e = dgemm(c->stream, 0, 0, n, n, n, 1.0, XA, n, (long)nn, Zi, n, 0, 0.0, G, n,
                                   (long)nn, cnt))
       return e;
-    if (T.ndense > 0) {
-      // P = Abar^T * Gbar : (ndense x n^2) * (n^2 x cnt).  The output has few
-      // tiles for its long inner dimension: the n^2 rows are split into ks
-      // slices (one batched call, one partial product each) and the partials
-      // summed, so that the GEMM fills the chip.
-      const int M = T.ndense, N = cnt;
+    const int rlo = T.ndense > 0 ? T.f1rlo[first / chunk] : 0;
+    if (T.ndense > rlo) {
+      // P = Abar^T * Gbar : (ndense x n^2) * (n^2 x cnt), over the pool
+      // columns from rlo on (the ones before are owned by no owner of the
+      // chunk: for dense constraints of equal size the pair belongs to the
+      // smaller index, so a chunk needs only the triangle below it).  The
+      // output has few tiles for its long inner dimension: the n^2 rows are
+      // split into ks slices (one batched call, one partial product each) and
+      // the partials summed, so that the GEMM fills the chip.
+      const double* Abar = T.d_dense + (size_t)rlo * nn;
+      const int M = T.ndense - rlo, N = cnt;
       const long K = (long)nn;
       const long tiles = (long)((M + 127) / 128) * ((N + 127) / 128);
       int ks = (int)std::min<long>(32, std::max<long>(1, 1024 / tiles));
       if (K / ks < 1024) ks = (int)std::max<long>(1, K / 1024);
       if (ks <= 1) {
-        if (int e = dgemm(c->stream, 1, 0, M, N, (int)K, 1.0, T.d_dense, (int)nn, 0, G, (int)nn, 0, 0.0, c->d_F1C, M, 0, 1))
+        if (int e = dgemm(c->stream, 1, 0, M, N, (int)K, 1.0, Abar, (int)nn, 0, G, (int)nn, 0, 0.0, c->d_F1C, M, 0, 1))
           return e;
       } else {
         const long Kc = K / ks, rem = K - Kc * ks;
@@ -673,17 +710,17 @@ static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
           c->capF1P = (size_t)nparts * len;
           SDPA_HIP_CHECK(hipMalloc(&c->d_F1P, c->capF1P * sizeof(double)));
         }
-        if (int e = dgemm(c->stream, 1, 0, M, N, (int)Kc, 1.0, T.d_dense, (int)nn, Kc, G,
+        if (int e = dgemm(c->stream, 1, 0, M, N, (int)Kc, 1.0, Abar, (int)nn, Kc, G,
                                       (int)nn, Kc, 0.0, c->d_F1P, M, (long)len, ks))
           return e;
         if (rem > 0)
-          if (int e = dgemm(c->stream, 1, 0, M, N, (int)rem, 1.0, T.d_dense + Kc * ks, (int)nn, 0, G + Kc * ks, (int)nn, 0, 0.0, c->d_F1P + (size_t)ks * len, M, 0, 1))
+          if (int e = dgemm(c->stream, 1, 0, M, N, (int)rem, 1.0, Abar + Kc * ks, (int)nn, 0, G + Kc * ks, (int)nn, 0, 0.0, c->d_F1P + (size_t)ks * len, M, 0, 1))
             return e;
         hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((len + 255) / 256)), dim3(256), 0,
                            c->stream, c->d_F1P, nparts, len, c->d_F1C);
       }
-      hipLaunchKernelGGL(k_f1_scatter, dim3((T.ndense + 255) / 256, cnt), dim3(256), 0, c->stream,
-                         T.d_f1own, first, d_dlist, T.d_con, T.d_nzeff, T.ndense, c->d_F1C,
+      hipLaunchKernelGGL(k_f1_scatter, dim3((M + 255) / 256, cnt), dim3(256), 0, c->stream,
+                         T.d_f1own, first, d_dlist, T.d_con, T.d_nzeff, T.ndense, rlo, c->d_F1C,
                          c->d_B, c->m);
     }
     if (T.ndense < T.ncon)
