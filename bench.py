@@ -53,6 +53,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 METRIC = "time-to-1e-7 gap; bMat assembly GB/s & MFMA util; mixed vs FP64 GPU, m=6000"
 # committed PMC summaries (FETCH_SIZE + WRITE_SIZE per launch) of the Schur
 # Cholesky's two main kernels on the headline workload
@@ -127,6 +128,33 @@ def other_config(key, steps):
     rt = s.run()
     pt = s.phase_times()
     s.enable_timing(False)
+    split = None
+    if key == "dense":
+        # the split-precision path on the same workload: F1/F2 products in
+        # FP32 on MFMA, the Schur matrix factored in FP32 with FP64
+        # refinement, FP64 from the first refinement stall on
+        s.set_param(precision=1)
+        st0 = s.mixed_stats()
+        mval, rm = timed_solves(s, steps, 1, True, name + " split precision")
+        st1 = s.mixed_stats()
+        s.enable_timing(True)
+        rmt = s.run()
+        mpt = s.phase_times()
+        s.enable_timing(False)
+        s.set_param(precision=0)
+        mits = max(1, rmt.iterations)
+        fl = d.m * 4.0 * 500 ** 3 + d.m * (d.m + 1.0) * 500 ** 2
+        # the assembly runs in FP32 while the factorization is mixed: the
+        # iterations before the first stall
+        nrun = steps + 1
+        mixed_its = (st1["factors"] - st0["factors"]) / nrun
+        split = {"time_to_1e7_s": mval, "fp64_over_split": val / mval, "phase": rm.phase,
+                 "iterations": rm.iterations, "relative_gap": rm.relative_gap,
+                 "primal_error": rm.primal_error, "dual_error": rm.dual_error,
+                 "objective_rel_diff_vs_fp64": max(abs(rm.primal_obj - r.primal_obj) / abs(r.primal_obj),
+                                                   abs(rm.dual_obj - r.dual_obj) / abs(r.dual_obj)),
+                 "fp32_iterations_per_solve": mixed_its,
+                 "phase_ms_per_iteration": {k: v / mits for k, v in mpt.items()}}
     s.close()
     its = max(1, rt.iterations)
     out = {"workload": name, "m": d.m, "time_to_1e7_s": val, "phase": r.phase,
@@ -140,6 +168,26 @@ def other_config(key, steps):
         ms = pt["bmat"] / its
         out["bmat_f1"] = {"flops": fl, "ms": ms, "TFLOPs": fl / (ms * 1e-3) / 1e12,
                           "frac_fp64_peak": fl / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        out["split_precision"] = split
+        # the assembly alone in both precisions (HIP-event phase timing, the
+        # mean of 3 assemblies at the identity start X = Z = 100 I)
+        g = sdpa_amd.GpuContext(0).load(d)
+        g.identity_start(100.0)
+        asm = {}
+        for prec in (0, 1):
+            g.timing(1)
+            g.make_bmat(prec)  # untimed: the FP32 pool is made by the first FP32 assembly
+            g.timing(1)
+            for _ in range(3):
+                g.make_bmat(prec)
+            t = g.timing(0)
+            ams = t[0] / max(1.0, t[1])
+            peak = FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS
+            asm["fp64" if prec == 0 else "fp32"] = {
+                "ms": ams, "TFLOPs": fl / (ams * 1e-3) / 1e12,
+                "frac_peak": fl / (ams * 1e-3) / 1e12 / peak, "peak": peak}
+        g.close()
+        out["bmat_f1_by_precision"] = asm
     del d
     return out
 

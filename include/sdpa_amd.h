@@ -150,12 +150,12 @@ class SDPA {
   void setInternalInitPoint(const double* X, const double* Z, const double* y);
   double phaseTimes[5] = {0, 0, 0, 0, 0};  // bmat, chol, solve, eig, inv (ms; when timing on)
   void enableTiming(bool on) { timing_ = on; }
-  // Extension with no counterpart in the reference: precision 1 factors the
-  // Schur matrix in mixed precision (FP32 factor of the equilibrated matrix
-  // on FP32 MFMA, solutions refined to FP64 accuracy against the FP64
-  // matrix) while mu > muSwitch and the refinement converges; from the first
-  // stall (or FP32 pivot failure) on, FP64 as the reference.  0 (default):
-  // FP64 only.
+  // Extension with no counterpart in the reference: precision 1 assembles
+  // the Schur matrix with its F1/F2 products in FP32 and factors it in mixed
+  // precision (FP32 factor of the equilibrated matrix on FP32 MFMA, solutions
+  // refined to FP64 accuracy against the FP64 matrix) while mu > muSwitch and
+  // the refinement converges; from the first stall (or FP32 pivot failure)
+  // on, FP64 as the reference.  0 (default): FP64 only.
   void setPrecision(int precision, double muSwitch = 0.0) {
     precision_ = precision;
     muSwitch_ = muSwitch;

@@ -68,8 +68,13 @@ int sdpa_gpu_keep_initial_point(sdpa_gpu_ctx* ctx);
  * a factorization fails. */
 int sdpa_gpu_inv_chol(sdpa_gpu_ctx* ctx);
 
-/* Newton::Make_bMat (sdpa_newton.cpp:2634-2670) into the device bMat (FP64;
- * precision is accepted for symmetry with the factorization and ignored). */
+/* Newton::Make_bMat (sdpa_newton.cpp:2634-2670) into the device bMat (FP64).
+ * precision 0: FP64 throughout.  precision 1 (split precision; no
+ * counterpart in the reference): the F1/F2 products (G_j = X A_j Z^-1 and
+ * A_i . G_j for dense partners) run on FP32 MFMA with FP32 accumulation, over
+ * X, Z^-1 and the dense A_i rounded to FP32; their long inner sums are split
+ * into slices added in FP64, and the F3 and LP terms stay FP64.  Other values
+ * return -30. */
 int sdpa_gpu_make_bmat(sdpa_gpu_ctx* ctx, int precision);
 /* Lal::choleskyFactorWithAdjust -> rATL_dpotrfL (sdpa_linear.cpp:762-790,
  * sdpa_dpotrf.cpp:218-265): lower Cholesky of bMat with SDPA's pivot rule.

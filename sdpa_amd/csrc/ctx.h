@@ -27,6 +27,7 @@ struct BlockTable {
   int* d_f1own = nullptr;  // local indices of F1/F2 owners (nF1)
   int* d_dcon = nullptr;   // global constraint of each dense pool column (ndense)
   double* d_dense = nullptr;  // n*n x ndense dense pool (column per dense A_k)
+  float* d_dense32 = nullptr; // the pool in FP32 (made by the first FP32 assembly)
   int* d_dlist = nullptr;     // local index of each dense pool column's constraint (ndense)
   // blocks with n <= 256: one 16-byte record per constraint (con, estart, nzeff,
   // min(ecnt, 9) | form << 4 | a << 8 | b << 16 with (a, b) its entry when it
@@ -129,6 +130,8 @@ struct Ctx {
   double* d_hpart = nullptr;  // F3 heavy-pair partial sums (nheavy^2 x slices)
   size_t capHpart = 0;
   size_t capF1P = 0;
+  float* d_xz32 = nullptr;  // FP32 assembly: one block's X and Z^-1 in FP32
+  size_t capXZ32 = 0;
   int* d_info = nullptr;   // factorization info (one per batch item)
   double* d_red = nullptr;  // reduction partials
   double* h_red = nullptr;  // pinned host mirror of reduction results
@@ -160,7 +163,7 @@ struct Ctx {
 // schur.hip
 constexpr int kF3MultiMaxM = 8192;  // the all-blocks F3 kernel keeps a Schur column in LDS
 constexpr int kMultiN = 64;         // ... and takes the blocks up to this size
-int make_bmat(Ctx* c);
+int make_bmat(Ctx* c, int precision);  // 1: F1/F2 products in FP32
 int f3multi_prepare(int m);  // LDS size of that kernel for m constraints
 // ops.hip
 enum DotCode {

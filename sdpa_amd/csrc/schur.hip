@@ -493,6 +493,9 @@ __global__ __launch_bounds__(256) void k_f3_heavy_sum(
 // ---- F1/F2 ----------------------------------------------------------------
 // Dense copy of the owners' A_j (sparse owners are expanded, dense ones copied
 // from the pool) for one chunk.  grid = (chunk), block = 256.
+// T is the element type of the products (double, or float for the FP32
+// assembly); the pool is in the same type.
+template <class T>
 __global__ __launch_bounds__(256) void k_densify(const int* __restrict__ own, int first,
                                                  const int* __restrict__ estart,
                                                  const int* __restrict__ ecnt,
@@ -500,35 +503,43 @@ __global__ __launch_bounds__(256) void k_densify(const int* __restrict__ own, in
                                                  const int* __restrict__ er,
                                                  const int* __restrict__ ec,
                                                  const double* __restrict__ ev,
-                                                 const double* __restrict__ pool, int n,
-                                                 double* __restrict__ out) {
+                                                 const T* __restrict__ pool, int n,
+                                                 T* __restrict__ out) {
   const int j = own[first + blockIdx.x];
-  double* A = out + (size_t)blockIdx.x * n * n;
+  T* A = out + (size_t)blockIdx.x * n * n;
   const size_t nn = (size_t)n * n;
   if (dpos[j] >= 0) {
-    const double* src = pool + (size_t)dpos[j] * nn;
+    const T* src = pool + (size_t)dpos[j] * nn;
     for (size_t e = threadIdx.x; e < nn; e += 256) A[e] = src[e];
     return;
   }
-  for (size_t e = threadIdx.x; e < nn; e += 256) A[e] = 0.0;
+  for (size_t e = threadIdx.x; e < nn; e += 256) A[e] = T(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int e = 0; e < ecnt[j]; ++e) {
       int r = er[estart[j] + e], c = ec[estart[j] + e];
-      A[r + (size_t)n * c] += ev[estart[j] + e];
-      if (r != c) A[c + (size_t)n * r] += ev[estart[j] + e];
+      A[r + (size_t)n * c] += (T)ev[estart[j] + e];
+      if (r != c) A[c + (size_t)n * r] += (T)ev[estart[j] + e];
     }
   }
 }
 
-// C[i] = sum over s of P[s * len + i], in order of s
-__global__ __launch_bounds__(256) void k_sum_slices(const double* __restrict__ P, int ns,
-                                                    size_t len, double* __restrict__ C) {
+// C[i] = sum over s of P[s * len + i], in order of s (in FP64)
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_slices(const T* __restrict__ P, int ns, size_t len,
+                                                    double* __restrict__ C) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= len) return;
   double a = 0.0;
-  for (int s = 0; s < ns; ++s) a += P[(size_t)s * len + i];
+  for (int s = 0; s < ns; ++s) a += (double)P[(size_t)s * len + i];
   C[i] = a;
+}
+
+// dst = float(src), n elements
+__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ src,
+                                                float* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    dst[i] = (float)src[i];
 }
 
 __device__ __forceinline__ bool owns(int jloc, int iloc, const int* nzeff) {
@@ -538,22 +549,24 @@ __device__ __forceinline__ bool owns(int jloc, int iloc, const int* nzeff) {
 
 // Sparse partners of F1 owners: one wave per (owner in chunk, partner).
 // grid = (ceil(ncon/4), chunk).
+template <class T>
 __global__ __launch_bounds__(256) void k_f1_sparse(
     const int* __restrict__ own, int first, const int* __restrict__ con,
     const int* __restrict__ estart, const int* __restrict__ ecnt, const int* __restrict__ nzeff,
     const int* __restrict__ dpos, const int* __restrict__ er, const int* __restrict__ ec,
-    const double* __restrict__ ev, int ncon, int n, const double* __restrict__ G,
+    const double* __restrict__ ev, int ncon, int n, const T* __restrict__ G,
     double* __restrict__ B, int m) {
   const int jc = blockIdx.y;
   const int j = own[first + jc];
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= ncon || dpos[i] >= 0 || !owns(j, i, nzeff)) return;
-  const double* Gj = G + (size_t)jc * n * n;
+  const T* Gj = G + (size_t)jc * n * n;
   double v = 0.0;
   for (int e = lane; e < ecnt[i]; e += 64) {
     int r = er[estart[i] + e], c = ec[estart[i] + e];
-    double g = (r == c) ? Gj[r + (size_t)n * c] : Gj[r + (size_t)n * c] + Gj[c + (size_t)n * r];
+    const double g = (r == c) ? (double)Gj[r + (size_t)n * c]
+                              : (double)Gj[r + (size_t)n * c] + (double)Gj[c + (size_t)n * r];
     v += ev[estart[i] + e] * g;
   }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -624,116 +637,167 @@ __global__ void k_lp_bmat(const int* __restrict__ ptr, const int* __restrict__ l
 
 struct F1Scratch;
 
-static int f1_block(Ctx* c, BlockTable& T, const double* X, const double* Zi) {
-  const int n = T.n;
+// The products of f1_block in the element type T.
+static int gemm_t(hipStream_t s, int ta, int tb, int M, int N, int K, const double* A, int lda,
+                  long sA, const double* B, int ldb, long sB, double* C, int ldc, long sC,
+                  int batch) {
+  return dgemm(s, ta, tb, M, N, K, 1.0, A, lda, sA, B, ldb, sB, 0.0, C, ldc, sC, batch);
+}
+static int gemm_t(hipStream_t s, int ta, int tb, int M, int N, int K, const float* A, int lda,
+                  long sA, const float* B, int ldb, long sB, float* C, int ldc, long sC,
+                  int batch) {
+  return sgemm(s, ta, tb, M, N, K, 1.0f, A, lda, sA, B, ldb, sB, 0.0f, C, ldc, sC, batch);
+}
+
+// F1/F2 owners of one block: G_j = X A_j Z^-1 for chunks of owners, then
+// A_i . G_j for their partners.  T = double: FP64 throughout.  T = float
+// (the split-precision assembly): X, Z^-1 and the dense pool are rounded to
+// FP32 and the three products run on FP32 MFMA with FP32 accumulation; the
+// slices of the long Abar^T Gbar product are summed in FP64, the sparse
+// partners' gathers are FP64 sums of FP32 values, and B stays FP64.
+template <class T>
+static int f1_block(Ctx* c, BlockTable& T_, const double* X64, const double* Zi64) {
+  BlockTable& Tb = T_;
+  const int n = Tb.n;
   const size_t nn = (size_t)n * n;
+  hipStream_t s = c->stream;
   // chunk so that densified A, X*A and G each fit in a 3 x 512 MiB budget
+  // (in FP64; the FP32 assembly takes the same chunks)
   size_t budget = (size_t)512 << 20;
   int chunk = (int)std::max<size_t>(1, budget / (nn * sizeof(double)));
   // whole 128-column tiles of the Abar^T Gbar product when the chunk is wide
   if (chunk >= 128) chunk -= chunk % 128;
-  if (chunk > T.nF1) chunk = T.nF1;
+  if (chunk > Tb.nF1) chunk = Tb.nF1;
   size_t need = 3 * (size_t)chunk * nn;
   if (need > c->capG) {
     if (c->d_G) SDPA_HIP_CHECK(hipFree(c->d_G));
     SDPA_HIP_CHECK(hipMalloc(&c->d_G, need * sizeof(double)));
     c->capG = need;
   }
-  double* Ad = c->d_G;
-  double* XA = c->d_G + (size_t)chunk * nn;
-  double* G = c->d_G + 2 * (size_t)chunk * nn;
-  if (T.ndense > 0) {
-    size_t needP = (size_t)T.ndense * chunk;
+  T* Ad = reinterpret_cast<T*>(c->d_G);
+  T* XA = Ad + (size_t)chunk * nn;
+  T* G = Ad + 2 * (size_t)chunk * nn;
+  if (Tb.ndense > 0) {
+    size_t needP = (size_t)Tb.ndense * chunk;
     if (needP > c->capF1C) {
       if (c->d_F1C) SDPA_HIP_CHECK(hipFree(c->d_F1C));
       SDPA_HIP_CHECK(hipMalloc(&c->d_F1C, needP * sizeof(double)));
       c->capF1C = needP;
     }
   }
+  // the operands in T
+  const T* X;
+  const T* Zi;
+  const T* pool;
+  if constexpr (sizeof(T) == 8) {
+    X = X64;
+    Zi = Zi64;
+    pool = Tb.d_dense;
+  } else {
+    if (2 * nn > c->capXZ32) {
+      if (c->d_xz32) SDPA_HIP_CHECK(hipFree(c->d_xz32));
+      SDPA_HIP_CHECK(hipMalloc(&c->d_xz32, 2 * nn * sizeof(float)));
+      c->capXZ32 = 2 * nn;
+    }
+    const unsigned gb = (unsigned)std::min<size_t>(1024, (nn + 255) / 256);
+    hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, s, X64, c->d_xz32, nn);
+    hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, s, Zi64, c->d_xz32 + nn, nn);
+    X = c->d_xz32;
+    Zi = c->d_xz32 + nn;
+    if (Tb.ndense > 0 && !Tb.d_dense32) {
+      // the pool in FP32, made at the block's first FP32 assembly
+      const size_t len = nn * Tb.ndense;
+      SDPA_HIP_CHECK(hipMalloc(&Tb.d_dense32, len * sizeof(float)));
+      hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<size_t>(8192, (len + 255) / 256)),
+                         dim3(256), 0, s, Tb.d_dense, Tb.d_dense32, len);
+    }
+    pool = Tb.d_dense32;
+  }
   // dense-member list (local indices of dense constraints, in pool order),
   // built at load time
-  const int* d_dlist = T.d_dlist;
-  if (T.ndense > 0 && (T.f1chunk != chunk || T.f1rlo.empty())) {
-    T.f1chunk = chunk;
-    T.f1rlo.clear();
-    for (int first = 0; first < T.nF1; first += chunk)
-      T.f1rlo.push_back(f1_first_owned(T, first, std::min(chunk, T.nF1 - first)));
+  const int* d_dlist = Tb.d_dlist;
+  if (Tb.ndense > 0 && (Tb.f1chunk != chunk || Tb.f1rlo.empty())) {
+    Tb.f1chunk = chunk;
+    Tb.f1rlo.clear();
+    for (int first = 0; first < Tb.nF1; first += chunk)
+      Tb.f1rlo.push_back(f1_first_owned(Tb, first, std::min(chunk, Tb.nF1 - first)));
   }
-  for (int first = 0; first < T.nF1; first += chunk) {
-    const int cnt = std::min(chunk, T.nF1 - first);
+  for (int first = 0; first < Tb.nF1; first += chunk) {
+    const int cnt = std::min(chunk, Tb.nF1 - first);
     // the chunk's A_t: straight from the dense pool when its owners are
     // consecutive pool columns, else densified into Ad
-    const double* At = Ad;
+    const T* At = Ad;
     {
-      const int p0 = T.h_dpos[T.h_f1own[first]];
+      const int p0 = Tb.h_dpos[Tb.h_f1own[first]];
       bool consecutive = p0 >= 0;
       for (int t = 1; t < cnt && consecutive; ++t)
-        consecutive = T.h_dpos[T.h_f1own[first + t]] == p0 + t;
+        consecutive = Tb.h_dpos[Tb.h_f1own[first + t]] == p0 + t;
       if (consecutive)
-        At = T.d_dense + (size_t)p0 * nn;
+        At = pool + (size_t)p0 * nn;
       else
-        hipLaunchKernelGGL(k_densify, dim3(cnt), dim3(256), 0, c->stream, T.d_f1own, first,
-                           T.d_estart, T.d_ecnt, T.d_dpos, c->d_er, c->d_ec, c->d_ev, T.d_dense, n,
+        hipLaunchKernelGGL(k_densify<T>, dim3(cnt), dim3(256), 0, s, Tb.d_f1own, first,
+                           Tb.d_estart, Tb.d_ecnt, Tb.d_dpos, c->d_er, c->d_ec, c->d_ev, pool, n,
                            Ad);
     }
     // XA_t = X A_t and G_t = XA_t Z^-1 for the chunk (strided-batched MFMA GEMMs)
-    if (int e = dgemm(c->stream, 0, 0, n, n, n, 1.0, X, n, 0, At, n, (long)nn, 0.0, XA, n,
-                                  (long)nn, cnt))
+    if (int e = gemm_t(s, 0, 0, n, n, n, X, n, 0, At, n, (long)nn, XA, n, (long)nn, cnt))
       return e;
-    if (int e = dgemm(c->stream, 0, 0, n, n, n, 1.0, XA, n, (long)nn, Zi, n, 0, 0.0, G, n,
-                                  (long)nn, cnt))
+    if (int e = gemm_t(s, 0, 0, n, n, n, XA, n, (long)nn, Zi, n, 0, G, n, (long)nn, cnt))
       return e;
-    const int rlo = T.ndense > 0 ? T.f1rlo[first / chunk] : 0;
-    if (T.ndense > rlo) {
+    const int rlo = Tb.ndense > 0 ? Tb.f1rlo[first / chunk] : 0;
+    if (Tb.ndense > rlo) {
       // P = Abar^T * Gbar : (ndense x n^2) * (n^2 x cnt), over the pool
       // columns from rlo on (the ones before are owned by no owner of the
       // chunk: for dense constraints of equal size the pair belongs to the
       // smaller index, so a chunk needs only the triangle below it).  The
       // output has few tiles for its long inner dimension: the n^2 rows are
       // split into ks slices (one batched call, one partial product each) and
-      // the partials summed, so that the GEMM fills the chip.
-      const double* Abar = T.d_dense + (size_t)rlo * nn;
-      const int M = T.ndense - rlo, N = cnt;
+      // the partials summed in FP64, so that the GEMM fills the chip.
+      const T* Abar = pool + (size_t)rlo * nn;
+      const int M = Tb.ndense - rlo, N = cnt;
       const long K = (long)nn;
       const long tiles = (long)((M + 127) / 128) * ((N + 127) / 128);
       int ks = (int)std::min<long>(32, std::max<long>(1, 1024 / tiles));
       if (K / ks < 1024) ks = (int)std::max<long>(1, K / 1024);
-      if (ks <= 1) {
-        if (int e = dgemm(c->stream, 1, 0, M, N, (int)K, 1.0, Abar, (int)nn, 0, G, (int)nn, 0, 0.0, c->d_F1C, M, 0, 1))
+      const size_t len = (size_t)M * N;
+      if (ks <= 1 && sizeof(T) == 8) {
+        if (int e = gemm_t(s, 1, 0, M, N, (int)K, Abar, (int)nn, 0, G, (int)nn, 0,
+                           reinterpret_cast<T*>(c->d_F1C), M, 0, 1))
           return e;
       } else {
         const long Kc = K / ks, rem = K - Kc * ks;
         const int nparts = ks + (rem > 0 ? 1 : 0);
-        const size_t len = (size_t)M * N;
         if ((size_t)nparts * len > c->capF1P) {
           if (c->d_F1P) SDPA_HIP_CHECK(hipFree(c->d_F1P));
           c->capF1P = (size_t)nparts * len;
           SDPA_HIP_CHECK(hipMalloc(&c->d_F1P, c->capF1P * sizeof(double)));
         }
-        if (int e = dgemm(c->stream, 1, 0, M, N, (int)Kc, 1.0, Abar, (int)nn, Kc, G,
-                                      (int)nn, Kc, 0.0, c->d_F1P, M, (long)len, ks))
+        T* P = reinterpret_cast<T*>(c->d_F1P);
+        if (int e = gemm_t(s, 1, 0, M, N, (int)Kc, Abar, (int)nn, Kc, G, (int)nn, Kc, P, M,
+                           (long)len, ks))
           return e;
         if (rem > 0)
-          if (int e = dgemm(c->stream, 1, 0, M, N, (int)rem, 1.0, Abar + Kc * ks, (int)nn, 0, G + Kc * ks, (int)nn, 0, 0.0, c->d_F1P + (size_t)ks * len, M, 0, 1))
+          if (int e = gemm_t(s, 1, 0, M, N, (int)rem, Abar + Kc * ks, (int)nn, 0, G + Kc * ks,
+                             (int)nn, 0, P + (size_t)ks * len, M, 0, 1))
             return e;
-        hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((len + 255) / 256)), dim3(256), 0,
-                           c->stream, c->d_F1P, nparts, len, c->d_F1C);
+        hipLaunchKernelGGL(k_sum_slices<T>, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s,
+                           P, nparts, len, c->d_F1C);
       }
-      hipLaunchKernelGGL(k_f1_scatter, dim3((M + 255) / 256, cnt), dim3(256), 0, c->stream,
-                         T.d_f1own, first, d_dlist, T.d_con, T.d_nzeff, T.ndense, rlo, c->d_F1C,
-                         c->d_B, c->m);
+      hipLaunchKernelGGL(k_f1_scatter, dim3((M + 255) / 256, cnt), dim3(256), 0, s, Tb.d_f1own,
+                         first, d_dlist, Tb.d_con, Tb.d_nzeff, Tb.ndense, rlo, c->d_F1C, c->d_B,
+                         c->m);
     }
-    if (T.ndense < T.ncon)
-      hipLaunchKernelGGL(k_f1_sparse, dim3((T.ncon + 3) / 4, cnt), dim3(256), 0, c->stream,
-                         T.d_f1own, first, T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff, T.d_dpos,
-                         c->d_er, c->d_ec, c->d_ev, T.ncon, n, G, c->d_B, c->m);
+    if (Tb.ndense < Tb.ncon)
+      hipLaunchKernelGGL(k_f1_sparse<T>, dim3((Tb.ncon + 3) / 4, cnt), dim3(256), 0, s,
+                         Tb.d_f1own, first, Tb.d_con, Tb.d_estart, Tb.d_ecnt, Tb.d_nzeff,
+                         Tb.d_dpos, c->d_er, c->d_ec, c->d_ev, Tb.ncon, n, G, c->d_B, c->m);
     // the next chunk rewrites the descriptor slots and scratch: stream order keeps it safe
   }
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-int make_bmat(Ctx* c) {
+int make_bmat(Ctx* c, int precision) {
   hipStream_t s = c->stream;
   const int m = c->m;
   if (c->f3multi) {
@@ -751,7 +815,8 @@ int make_bmat(Ctx* c) {
     const double* X = c->d_X + T.off;
     const double* Zi = c->d_Zinv + T.off;
     if (T.nF1 > 0)
-      if (int e = f1_block(c, T, X, Zi)) return e;
+      if (int e = precision == 1 ? f1_block<float>(c, T, X, Zi) : f1_block<double>(c, T, X, Zi))
+        return e;
     if (T.nF3 > 0) {
       if (c->f3multi && T.n <= kMultiN) {
         // light pairs done by k_f3_multi

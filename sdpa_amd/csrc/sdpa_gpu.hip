@@ -103,8 +103,10 @@ struct Arena {
 };
 
 void free_problem(Ctx* c) {
-  for (auto& T : c->blk)
+  for (auto& T : c->blk) {
     if (T.d_dense) (void)hipFree(T.d_dense);
+    if (T.d_dense32) (void)hipFree(T.d_dense32);
+  }
   c->blk.clear();
   if (c->d_blkarena) (void)hipFree(c->d_blkarena);
   c->d_blkarena = nullptr;
@@ -118,14 +120,14 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
                    (void**)&c->d_f3blk, (void**)&c->d_jptr, (void**)&c->d_jent,
-                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P, (void**)&c->d_hpart,
+                   (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P, (void**)&c->d_hpart, (void**)&c->d_xz32,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
   for (void** p : ptrs) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
-  c->capG = c->capF1C = c->capF1P = c->capHpart = 0;
+  c->capG = c->capF1C = c->capF1P = c->capHpart = c->capXZ32 = 0;
   free_mixed(c);
   c->loaded = false;
 }
@@ -821,9 +823,9 @@ int sdpa_gpu_inv_chol(sdpa_gpu_ctx* h) {
 int sdpa_gpu_make_bmat(sdpa_gpu_ctx* h, int precision) {
   if (!h || !h->c.loaded) return -1;
   Ctx* c = &h->c;
-  if (precision != 0) return fail(c, -30, "only FP64 (precision 0) is implemented");
+  if (precision != 0 && precision != 1) return fail(c, -30, "precision must be 0 or 1");
   tic(c);
-  int r = make_bmat(c);
+  int r = make_bmat(c, precision);
   c->bshifted = false;
   c->t_bmat += toc(c);
   c->n_bmat++;

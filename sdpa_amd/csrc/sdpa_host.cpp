@@ -858,8 +858,9 @@ void SDPA::solve() {
 
     // Newton::Mehrotra(PREDICTOR)
     check(sdpa_gpu_make_gvec(g, 0, betaVal * muCur, infeasible), g, "make_gvec");
-    check(sdpa_gpu_make_bmat(g, 0), g, "make_bmat");
     const bool mixedNow = mixedOn && muCur > muSwitch_;
+    // in mixed precision the F1/F2 products of the assembly run in FP32 too
+    check(sdpa_gpu_make_bmat(g, mixedNow ? 1 : 0), g, "make_bmat");
     int info = sdpa_gpu_factor_bmat(g, mixedNow ? 1 : 0);
     check(info, g, "factor_bmat");
     if (info > 0) { failure_ = 1; break; }

@@ -154,3 +154,68 @@ def test_mixed_mu_switch_hands_over_to_fp64():
     assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)
     assert r.relative_gap <= 1e-7
     assert r.primal_error <= 1e-7 and r.dual_error <= 1e-7
+
+
+def _sparse_f1_block():
+    # one 30 x 30 block, constraints with 1..60 entries: F1 owners with many
+    # entries and sparse partners (the FP32 path's densify and gather kernels)
+    from test_gpu_parity import sparse_mixed
+    return sparse_mixed()
+
+
+FP32_CASES = {
+    "dense_n70_m40": lambda: PR.dense_random(n=70, m=40, seed=6),
+    "dense_n130_m200": lambda: PR.dense_random(n=130, m=200, seed=7),
+    "sparse_f1_owners": _sparse_f1_block,
+}
+
+
+@pytest.mark.parametrize("name", list(FP32_CASES))
+def test_fp32_assembly_matches_oracle(name):
+    # precision 1 of sdpa_gpu_make_bmat: the F1/F2 products in FP32 (FP32
+    # accumulation; the slices of the long A_i . G_j sums added in FP64), the
+    # F3 terms in FP64.  Against the oracle's FP64 bMat: max |B32 - B64| at
+    # most 1e-5 max |B64| -- X, Z^-1 and A_i rounded to FP32 (2^-24 relative)
+    # and sums of up to n^2 terms; 1e-6 is typical.
+    from test_gpu_parity import packed_random
+    d = FP32_CASES[name]()
+    rng = np.random.default_rng(1)
+    X, Z, Zi = packed_random(d, rng)
+    g = sa.GpuContext(0).load(d)
+    g.set("X", X); g.set("Zinv", Zi)
+    g.make_bmat(1)
+    B32 = g.bmat()
+    g.make_bmat(0)
+    B64 = g.bmat()
+    g.close()
+    want = O.Problem(d).make_bmat(X, Zi)
+    tril = np.tril_indices(d.m)
+    scale = np.abs(want[tril]).max()
+    err32 = np.abs(B32[tril] - want[tril]).max() / scale
+    err64 = np.abs(B64[tril] - want[tril]).max() / scale
+    print(f"{name}: FP32 assembly error {err32:.2e}, FP64 {err64:.2e} (relative to max |B|)")
+    assert err32 <= 1e-5
+    assert err64 <= 1e-12
+    assert err32 > 0.0  # the FP32 products were used
+
+
+def test_split_precision_dense_solve():
+    # the split-precision path on an F1-dominated problem with m > 512 (the
+    # mixed factorization's range): F1/F2 products in FP32, the Schur matrix
+    # factored in FP32 with FP64 refinement until the first stall, FP64 after
+    # it.  The answer is the FP64 path's (itself checked against the oracle):
+    # pdOPT, objectives within 1e-6 relative, gap and feasibility at the 1e-7
+    # stopping level.
+    d = PR.dense_random(n=60, m=600, seed=8)
+    s = sa.Solver(0).load(d).prepare()
+    r0 = s.set_param(precision=0).run()
+    r1 = s.set_param(precision=1).run()
+    st = s.mixed_stats()
+    print(d.m, r0.iterations, r1.iterations, r0.primal_obj, r1.primal_obj, st)
+    assert st["factors"] >= 1
+    assert r0.phase == r1.phase == "pdOPT"
+    assert abs(r1.iterations - r0.iterations) <= 3
+    assert r1.primal_obj == pytest.approx(r0.primal_obj, rel=1e-6, abs=1e-8)
+    assert r1.dual_obj == pytest.approx(r0.dual_obj, rel=1e-6, abs=1e-8)
+    assert r1.relative_gap <= 1e-7
+    assert r1.primal_error <= 1e-7 and r1.dual_error <= 1e-7
