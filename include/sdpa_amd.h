@@ -149,6 +149,14 @@ class SDPA {
   // setInitPoint(true) + inputInit* would.
   void setInternalInitPoint(const double* X, const double* Z, const double* y);
   double phaseTimes[5] = {0, 0, 0, 0, 0};  // bmat, chol, solve, eig, inv (ms; when timing on)
+  // the buckets of the reference's ComputeTime (sdpa_parts.h:48-86) this
+  // build measures, seconds of the last solve (device ones when timing on)
+  struct ComputeTimes {
+    double predictor = 0, corrector = 0, stepPredictor = 0, stepCorrector = 0, mainLoop = 0;
+    double makebMat = 0, bF1 = 0, bF3 = 0, cholesky = 0, solve = 0, eigXZ = 0, invXZ = 0;
+    double makegVec = 0, makedX = 0, updateRes = 0;
+  };
+  const ComputeTimes& getComputeTime() const { return computeTime_; }
   void enableTiming(bool on) { timing_ = on; }
   // Extension with no counterpart in the reference: precision 1 assembles
   // the Schur matrix with its F1/F2 products in FP32 and factors it in mixed
@@ -175,6 +183,7 @@ class SDPA {
   FILE* display_ = nullptr;
   FILE* fpout_ = nullptr;
   int numThreads_ = 1;
+  ComputeTimes computeTime_;
   bool isInitPoint_ = false;
   bool timing_ = false;
   int precision_ = 0;
@@ -251,7 +260,8 @@ int sdpa_solver_write_input(sdpa_solver* s, const char* path, const char* printF
 int sdpa_solver_write_init(sdpa_solver* s, const char* path, const char* printFormat);
 /* SDPA::getDimacsError (sdpa_call.cpp:849-853): out[1..6] (out[0] = 0). */
 int sdpa_solver_dimacs(sdpa_solver* s, double* out7);
-/* printResultXVec / XMat / YMat (which = 0, 1, 2) into the file at path. */
+/* printResultXVec / XMat / YMat (which = 0, 1, 2) or printComputationTime
+ * (which = 3) into the file at path. */
 int sdpa_solver_print_result(sdpa_solver* s, int which, const char* path);
 
 #ifdef __cplusplus

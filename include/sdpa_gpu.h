@@ -144,6 +144,13 @@ int sdpa_gpu_timing(sdpa_gpu_ctx* ctx, int enable, double* out10);
  * panel launches (flops: the previous panel's update of the panel, 2 K per
  * element, plus the panel's Cholesky, w^2 h - w^3/3). */
 int sdpa_gpu_kernel_timing(sdpa_gpu_ctx* ctx, double* out8);
+/* The finer buckets of the same timing, cumulative ms since timing was last
+ * enabled (the device-side counterparts of the reference's ComputeTime,
+ * sdpa_parts.h:48-86): out[0] bMat, [1] of it the F1/F2 products, [2]
+ * Cholesky, [3] solve, [4] eigen (step lengths), [5] inverse Cholesky,
+ * [6] gVec, [7] direction (Dz, Dx), [8] residuals, [9] update, [10] inner
+ * products.  Returns the count written (at most cap). */
+int sdpa_gpu_timing_detail(sdpa_gpu_ctx* ctx, double* out, int cap);
 int sdpa_gpu_synchronize(sdpa_gpu_ctx* ctx);
 
 #ifdef __cplusplus

@@ -155,6 +155,9 @@ struct Ctx {
   // timing (ms), accumulated since the last reset
   hipEvent_t ev[8];
   double t_bmat = 0, t_chol = 0, t_solve = 0, t_eig = 0, t_inv = 0;
+  // finer buckets (ms): the F1/F2 share of bMat, gVec, direction,
+  // residuals, update, inner products
+  double t_f1 = 0, t_gvec = 0, t_dir = 0, t_res = 0, t_upd = 0, t_dots = 0;
   int n_bmat = 0, n_chol = 0, n_solve = 0;
   bool timing = false;
   std::string last_error;

@@ -814,9 +814,19 @@ int make_bmat(Ctx* c, int precision) {
     if (T.ncon == 0) continue;
     const double* X = c->d_X + T.off;
     const double* Zi = c->d_Zinv + T.off;
-    if (T.nF1 > 0)
+    if (T.nF1 > 0) {
+      if (c->timing) SDPA_HIP_CHECK(hipEventRecord(c->ev[2], s));
       if (int e = precision == 1 ? f1_block<float>(c, T, X, Zi) : f1_block<double>(c, T, X, Zi))
         return e;
+      if (c->timing) {
+        // the F1/F2 share of the assembly (timing synchronizes)
+        SDPA_HIP_CHECK(hipEventRecord(c->ev[3], s));
+        SDPA_HIP_CHECK(hipEventSynchronize(c->ev[3]));
+        float ms = 0.f;
+        SDPA_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        c->t_f1 += ms;
+      }
+    }
     if (T.nF3 > 0) {
       if (c->f3multi && T.n <= kMultiN) {
         // light pairs done by k_f3_multi

@@ -500,10 +500,17 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     }
   }
   if (int e = arena.commit(&c->d_blkarena)) return e;
-  if (int e = upload_raw(c, &c->d_er, h_er, (size_t)nentAll)) return e;
-  if (int e = upload_raw(c, &c->d_ec, h_ec, (size_t)nentAll)) return e;
-  if (int e = upload_raw(c, &c->d_ev, h_ev, (size_t)nentAll)) return e;
-  c->nent = nentAll;
+  // The device reads the entry pool only for sparse (not changeToDense)
+  // constraints: the F3 kernels, the sparse partners of F1 owners and the
+  // densify kernel.  Where every constraint of every block is dense (the
+  // dense pool holds them all) it stays on the host.
+  bool needEntries = false;
+  for (const BlockTable& T : c->blk) needEntries = needEntries || T.ndense < T.ncon;
+  const size_t nup = needEntries ? (size_t)nentAll : 0;
+  if (int e = upload_raw(c, &c->d_er, h_er, nup)) return e;
+  if (int e = upload_raw(c, &c->d_ec, h_ec, nup)) return e;
+  if (int e = upload_raw(c, &c->d_ev, h_ev, nup)) return e;
+  c->nent = (long)nup;
 
   // constraint-major sparse terms (dense blocks excluded) and position-major transpose
   std::vector<int> cptr(m + 1, 0);
@@ -903,23 +910,37 @@ int sdpa_gpu_min_eig(sdpa_gpu_ctx* h, double* lamX, double* lamZ) {
 
 int sdpa_gpu_residuals(sdpa_gpu_ctx* h, double* nP, double* nD) {
   if (!h || !h->c.loaded) return -1;
-  return residuals(&h->c, nP, nD);
+  Ctx* c = &h->c;
+  tic(c);
+  const int r = residuals(c, nP, nD);
+  c->t_res += toc(c);
+  return r;
 }
 
 int sdpa_gpu_make_gvec(sdpa_gpu_ctx* h, int corrector, double target, int infeasible) {
   if (!h || !h->c.loaded) return -1;
-  return make_gvec(&h->c, corrector, target, infeasible);
+  Ctx* c = &h->c;
+  tic(c);
+  const int r = make_gvec(c, corrector, target, infeasible);
+  c->t_gvec += toc(c);
+  return r;
 }
 
 int sdpa_gpu_make_direction(sdpa_gpu_ctx* h, int infeasible) {
   if (!h || !h->c.loaded) return -1;
-  return make_direction(&h->c, infeasible);
+  Ctx* c = &h->c;
+  tic(c);
+  const int r = make_direction(c, infeasible);
+  c->t_dir += toc(c);
+  return r;
 }
 
 int sdpa_gpu_dots(sdpa_gpu_ctx* h, int nq, const int* codes, double* out) {
   if (!h || !h->c.loaded) return -1;
   if (nq <= 0 || nq > 32) return -2;
+  tic(&h->c);
   const int r = compute_dots(&h->c, nq, codes, out);
+  h->c.t_dots += toc(&h->c);
   // compute_dots synchronizes the stream: the failure flag of the last
   // cooperative Schur solve (copied to pinned memory behind it) is current
   if (r == 0 && h->c.dw.h_err && *h->c.dw.h_err) {
@@ -943,7 +964,21 @@ int sdpa_gpu_mixed_stats(sdpa_gpu_ctx* h, long* out6) {
 
 int sdpa_gpu_update(sdpa_gpu_ctx* h, double aP, double aD) {
   if (!h || !h->c.loaded) return -1;
-  return update_point(&h->c, aP, aD);
+  Ctx* c = &h->c;
+  tic(c);
+  const int r = update_point(c, aP, aD);
+  c->t_upd += toc(c);
+  return r;
+}
+
+int sdpa_gpu_timing_detail(sdpa_gpu_ctx* h, double* out, int cap) {
+  if (!h || !out) return -1;
+  const Ctx* c = &h->c;
+  const double v[11] = {c->t_bmat, c->t_f1,  c->t_chol, c->t_solve, c->t_eig, c->t_inv,
+                        c->t_gvec, c->t_dir, c->t_res,  c->t_upd,   c->t_dots};
+  const int n = cap < 11 ? cap : 11;
+  for (int t = 0; t < n; ++t) out[t] = v[t];
+  return n;
 }
 
 int sdpa_gpu_timing(sdpa_gpu_ctx* h, int enable, double* out) {
@@ -959,6 +994,7 @@ int sdpa_gpu_timing(sdpa_gpu_ctx* h, int enable, double* out) {
   if (enable >= 0) {
     c->timing = enable != 0;
     c->t_bmat = c->t_chol = c->t_solve = c->t_eig = c->t_inv = 0;
+    c->t_f1 = c->t_gvec = c->t_dir = c->t_res = c->t_upd = c->t_dots = 0;
     c->n_bmat = c->n_chol = c->n_solve = 0;
     c->dw.time_trailing = c->timing;
     c->dw.trail_ms = c->dw.trail_flops = 0.0;

@@ -538,20 +538,59 @@ void SDPA::printResultXVec(FILE* fp) { print_vector(fp, xVec_.data(), (int)xVec_
 void SDPA::printResultXMat(FILE* fp) { printBlocks(fp, Z_, XPrint_.c_str()); }
 void SDPA::printResultYMat(FILE* fp) { printBlocks(fp, X_, YPrint_.c_str()); }
 
+// ComputeTime::display (sdpa_parts.cpp:78-162): the reference's rows and
+// format.  Host rows are wall-clock seconds of this solve; the device rows
+// (bMat and its parts, gVec, Cholesky, solve, makedX, the inverses, the
+// eigenvalue estimates, updateRes) are HIP-event times, recorded when timing
+// is enabled (sdpa_gpu_timing), else 0.  Here F2 owners run through the F1
+// products (bF1 covers both), the diagonal terms are part of the F3 and F1
+// kernels (bDia 0), X and Z are inverted and their step lengths estimated in
+// one launch each (the X+Z rows), and file input is not part of solve().
 void SDPA::printComputationTime(FILE* fp) {
   if (!fp) return;
-  // the reference's ComputeTime table (sdpa_parts.cpp:78-190) with the buckets
-  // this build measures: device phase times when timing is enabled
-  const double main = solveTime_ > 0 ? solveTime_ : 1.0;
+  const ComputeTimes& T = computeTime_;
+  const double main = T.mainLoop > 0 ? T.mainLoop : 1.0;
+  const double eig = T.eigXZ;
+  const struct { const char* name; double v; } rows[] = {
+      {" Predictor time  =       ", T.predictor},
+      {" Corrector time  =       ", T.corrector},
+      {" Make bMat time  =       ", T.makebMat},
+      {" Make bDia time  =       ", 0.0},
+      {" Make bF1  time  =       ", T.bF1},
+      {" Make bF2  time  =       ", 0.0},
+      {" Make bF3  time  =       ", T.bF3},
+      {" Make bPRE time  =       ", 0.0},
+      {" Make rMat time  =       ", 0.0},
+      {" Make gVec Mul   =       ", 0.0},
+      {" Make gVec time  =       ", T.makegVec},
+      {" Cholesky bMat   =       ", T.cholesky},
+      {" Ste Pre time    =       ", T.stepPredictor},
+      {" Ste Cor time    =       ", T.stepCorrector},
+      {" solve           =       ", T.solve},
+      {" sumDz           =       ", 0.0},
+      {" makedX          =       ", T.makedX},
+      {" symmetriseDx    =       ", 0.0},
+      {" makedXdZ        =       ", 0.0},
+      {" xMatTime        =       ", 0.0},
+      {" zMatTime        =       ", 0.0},
+      {" invzMatTime     =       ", 0.0},
+      {" xMatzMatTime    =       ", T.invXZ},
+      {" EigxMatTime     =       ", 0.0},
+      {" EigzMatTime     =       ", 0.0},
+      {" EigxMatzMatTime =       ", eig},
+      {" updateRes       =       ", T.updateRes},
+      {" EigTime         =       ", eig},
+      {" sub_total_bMat  =       ", T.mainLoop - T.makebMat},
+      {" Main Loop       =       ", T.mainLoop},
+      {" File Check      =       ", 0.0},
+      {" File Change     =       ", 0.0},
+      {" File Read       =       ", 0.0},
+      {" Total           =       ", T.mainLoop},
+  };
   std::fprintf(fp, "\n");
   std::fprintf(fp, "                         Time(sec)  Ratio(%% : MainLoop) \n");
-  const char* names[5] = {" Make bMat time  =       ", " Cholesky bMat   =       ",
-                          " solve           =       ", " EigxMatTime     =       ",
-                          " invzMatTime     =       "};
-  for (int t = 0; t < 5; ++t)
-    std::fprintf(fp, "%s%f,  %f\n", names[t], phaseTimes[t] * 1e-3, phaseTimes[t] * 1e-3 / main * 100.0);
-  std::fprintf(fp, " Main Loop       =       %f,  %f\n", solveTime_, 100.0);
-  std::fprintf(fp, " Total           =       %f,  %f\n", solveTime_, 100.0);
+  for (const auto& r : rows) std::fprintf(fp, "%s%f,  %f\n", r.name, r.v, r.v / main * 100.0);
+  std::fprintf(fp, "\n");
 }
 
 // IO::computeDimacs (sdpa_io.cpp:775-813).  The reference's "getOneNorm" is
@@ -850,7 +889,14 @@ void SDPA::solve() {
     return true;
   };
   int it = 0;
+  // host wall-clock buckets of ComputeTime (sdpa_parts.h:48-86)
+  using clk = std::chrono::steady_clock;
+  auto secs = [](clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+  };
+  double hPred = 0.0, hCorr = 0.0, hStepP = 0.0, hStepC = 0.0;
   while (update_phase() && it < P.maxIteration) {
+    const clk::time_point tPred = clk::now();
     const bool affine = !(phase == noINFO || phase == pFEAS || phase == dFEAS);
     if (phase == pdFEAS) betaVal = 0.0;
     else betaVal = affine ? 2.0 : P.betaBar;
@@ -873,6 +919,7 @@ void SDPA::solve() {
     check(sdpa_gpu_make_direction(g, infeasible), g, "make_direction");
 
     // StepLength::MehrotraPredictor + DirectionParameter::MehrotraCorrector
+    const clk::time_point tStepP = clk::now();
     double q[7];
     {
       int codes[5] = {D_CDX, D_BDY, D_XDZ, D_DXZ, D_DXDZ};
@@ -905,11 +952,15 @@ void SDPA::solve() {
     }
 
     // Newton::Mehrotra(CORRECTOR)
+    const clk::time_point tCorr = clk::now();
+    hStepP += secs(tStepP, tCorr);
+    hPred += secs(tPred, tCorr);
     check(sdpa_gpu_make_gvec(g, 1, betaVal * muCur, infeasible), g, "make_gvec");
     if (!solve_schur()) break;
     check(sdpa_gpu_make_direction(g, infeasible), g, "make_direction");
 
     // StepLength::MehrotraCorrector
+    const clk::time_point tStepC = clk::now();
     double lamX, lamZ;
     check(sdpa_gpu_min_eig(g, &lamX, &lamZ), g, "min_eig");
     const double alphaBD = 100.0;
@@ -950,6 +1001,11 @@ void SDPA::solve() {
       double incD = q[1] * alphaD, incP = q[0] * alphaP;
       double maxRatio = (q[5] - q[6]) / (incP - incD);
       if (maxRatio > 0.0 && maxRatio < 1.0) { alphaP *= maxRatio; alphaD *= maxRatio; }
+    }
+    {
+      const clk::time_point tEnd = clk::now();
+      hStepC += secs(tStepC, tEnd);
+      hCorr += secs(tCorr, tEnd);
     }
     trace_.push_back(IterationTrace{muCur, thetaP, thetaD, objP, objD, alphaP, alphaD, betaVal});
     print_iteration(fpout_, it, trace_.back());
@@ -1019,6 +1075,28 @@ void SDPA::solve() {
   phaseTimes[0] = tm[0]; phaseTimes[1] = tm[2]; phaseTimes[2] = tm[4]; phaseTimes[3] = tm[6];
   phaseTimes[4] = tm[8];
   solveTime_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  {
+    double dt[11] = {0};
+    check(sdpa_gpu_timing_detail(g, dt, 11), g, "timing_detail");
+    ComputeTimes& T = computeTime_;
+    T = ComputeTimes();
+    T.predictor = hPred;
+    T.corrector = hCorr;
+    T.stepPredictor = hStepP;
+    T.stepCorrector = hStepC;
+    T.mainLoop = solveTime_;
+    // device buckets (ms -> s), known when timing is enabled
+    T.makebMat = dt[0] * 1e-3;
+    T.bF1 = dt[1] * 1e-3;
+    T.bF3 = (dt[0] - dt[1]) * 1e-3;
+    T.cholesky = dt[2] * 1e-3;
+    T.solve = dt[3] * 1e-3;
+    T.eigXZ = dt[4] * 1e-3;
+    T.invXZ = dt[5] * 1e-3;
+    T.makegVec = dt[6] * 1e-3;
+    T.makedX = dt[7] * 1e-3;
+    T.updateRes = (dt[8] + dt[9]) * 1e-3;
+  }
   // IO::printLastInfo + printSolution (sdpa_solve.cpp:207-232)
   print_iteration(fpout_, it, trace_.back());
   print_iteration(display_, it, trace_.back());
@@ -1191,7 +1269,8 @@ int sdpa_solver_print_result(sdpa_solver* h, int which, const char* path) {
     if (!f) throw std::runtime_error(std::string("cannot open ") + path);
     if (which == 0) h->s->printResultXVec(f);
     else if (which == 1) h->s->printResultXMat(f);
-    else h->s->printResultYMat(f);
+    else if (which == 2) h->s->printResultYMat(f);
+    else h->s->printComputationTime(f);
     std::fclose(f);
   })
 }

@@ -89,3 +89,39 @@ def test_schur_dense_decision_matches_oracle():
     for d in cases:
         g = sa.GpuContext(0).load(d)
         assert bool(g.info[9]) == O.Problem(d).schur_dense, d.name
+
+
+COMPUTE_TIME_ROWS = [
+    "Predictor time", "Corrector time", "Make bMat time", "Make bDia time", "Make bF1  time",
+    "Make bF2  time", "Make bF3  time", "Make bPRE time", "Make rMat time", "Make gVec Mul",
+    "Make gVec time", "Cholesky bMat", "Ste Pre time", "Ste Cor time", "solve", "sumDz",
+    "makedX", "symmetriseDx", "makedXdZ", "xMatTime", "zMatTime", "invzMatTime",
+    "xMatzMatTime", "EigxMatTime", "EigzMatTime", "EigxMatzMatTime", "updateRes", "EigTime",
+    "sub_total_bMat", "Main Loop", "File Check", "File Change", "File Read", "Total"]
+
+
+def test_computation_time_table(golden_dir, tmp_path):
+    # printComputationTime (ComputeTime::display, sdpa_parts.cpp:78-162): the
+    # reference's rows in its order and format; with timing enabled the
+    # device buckets are filled and add up
+    s = sa.Solver(0).read_input(os.path.join(golden_dir, "example1.dat-s"))
+    s.enable_timing(True)
+    r = s.solve()
+    assert r.phase == "pdOPT"
+    out = tmp_path / "times.txt"
+    s.print_result(3, out)
+    lines = [l for l in open(out).read().splitlines() if "=" in l]
+    got = {}
+    for want, line in zip(COMPUTE_TIME_ROWS, lines):
+        name, vals = line.split("=")
+        assert name.strip() == want.strip(), (name, want)
+        t, ratio = (float(v) for v in vals.split(","))
+        got[want] = t
+    assert len(lines) == len(COMPUTE_TIME_ROWS)
+    for k in ("Make bMat time", "Cholesky bMat", "solve", "EigxMatzMatTime", "Main Loop"):
+        assert got[k] > 0.0, k
+    assert got["Make bF1  time"] + got["Make bF3  time"] == pytest.approx(got["Make bMat time"],
+                                                                          rel=1e-5, abs=2e-6)
+    assert got["Predictor time"] + got["Corrector time"] <= got["Main Loop"]
+    assert got["sub_total_bMat"] == pytest.approx(got["Main Loop"] - got["Make bMat time"],
+                                                  abs=2e-6)
