@@ -79,6 +79,10 @@ struct Ctx {
   int f3multi = 0;
   F3Blk* d_f3blk = nullptr;
   int* d_jptr = nullptr;  // m + 1
+  // per jent record: the first four entries of A_j in the block, a | b << 8
+  // each (int4), and their values (double4)
+  int* d_jab = nullptr;
+  double* d_jv = nullptr;
   // per (block q, constraint j) one record: q | min(ecnt, 15) << 20 | form << 24,
   // j's position in the block's list, estart, nzeff
   int* d_jent = nullptr;

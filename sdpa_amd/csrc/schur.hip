@@ -92,11 +92,16 @@ __device__ __forceinline__ double f3_plu(bool ab, bool gd, double xag, double xa
 // lower part is stored once.  Against one launch per block this writes each
 // entry of B once, needs no zero-fill, and reads X and Z^-1 from LDS.  F1/F2
 // and heavy-pair terms are added afterwards.
-// The next block's record and descriptor are loaded one pass ahead.
+// The next block's record, descriptor and A_j's entries there (jab: a | b << 8
+// of the first kMultiE, jv their values) are loaded one pass ahead; a pass
+// issues its first round of row records and the loads of its staged rows
+// together, before its first barrier, so that it waits for one round trip.
 // Dynamic LDS: m doubles (acc) + kMultiE * 4 * kMultiN doubles (rows).
 __global__ __launch_bounds__(kMultiT) void k_f3_multi(const F3Blk* __restrict__ blks,
                                                   const int* __restrict__ jptr,
                                                   const int* __restrict__ jent,
+                                                  const int4* __restrict__ jab,
+                                                  const double4* __restrict__ jv,
                                                   const int* __restrict__ er,
                                                   const int* __restrict__ ec,
                                                   const double* __restrict__ ev,
@@ -111,12 +116,24 @@ __global__ __launch_bounds__(kMultiT) void k_f3_multi(const F3Blk* __restrict__ 
   for (int r = j + tid; r < m; r += kMultiT) acc[r] = 0.0;
   const int t0 = jptr[j], t1 = jptr[j + 1];
   int4 jr = t0 < t1 ? reinterpret_cast<const int4*>(jent)[t0] : make_int4(0, 0, 0, 0);
+  int4 jq = t0 < t1 ? jab[t0] : make_int4(0, 0, 0, 0);
+  double4 jw = t0 < t1 ? jv[t0] : make_double4(0.0, 0.0, 0.0, 0.0);
   F3Blk Bq = blks[jr.x & 0xfffff];
+  // a thread's share of the staged rows: at most kMultiE * 2 * kMultiN
+  // elements of X and of Z^-1 over the work-group
+  constexpr int kStg = (kMultiE * 2 * kMultiN + kMultiT - 1) / kMultiT;
   for (int t = t0; t < t1; ++t) {
     const int pj = jr.y, sj = jr.z, nzj = jr.w, cj = (jr.x >> 20) & 15, fj = (jr.x >> 24) & 3;
+    // entry e of A_j in the block (selects: no local arrays)
+    const int4 abq = jq;
+    const double4 jwq = jw;
+    auto ab_of = [&](int e) { return e == 0 ? abq.x : e == 1 ? abq.y : e == 2 ? abq.z : abq.w; };
+    auto v_of = [&](int e) { return e == 0 ? jwq.x : e == 1 ? jwq.y : e == 2 ? jwq.z : jwq.w; };
     const F3Blk Bc = Bq;
     if (t + 1 < t1) {
       jr = reinterpret_cast<const int4*>(jent)[t + 1];
+      jq = jab[t + 1];
+      jw = jv[t + 1];
       Bq = blks[jr.x & 0xfffff];
     }
     if (cj > kLight) continue;  // every pair of this block is heavy (k_f3_heavy)
@@ -124,18 +141,44 @@ __global__ __launch_bounds__(kMultiT) void k_f3_multi(const F3Blk* __restrict__ 
     const double* Xq = X + Bc.off;
     const double* Zq = Zi + Bc.off;
     const bool staged = n <= kMultiN && cj <= kMultiE && Bc.rmeta != nullptr;
+    const int ncon = Bc.ncon;
+    // issued together: the first round of row records (one per thread) and
+    // the staged rows
+    constexpr int kR0 = 1;
+    int4 rq0[kR0];
+    double vq0[kR0];
+    double sx[kStg], sz[kStg];
+    if (staged) {
+#pragma unroll
+      for (int u = 0; u < kR0; ++u) {
+        const int kl = min(pj + tid + kMultiT * u, ncon - 1);
+        rq0[u] = Bc.rmeta[kl];
+        vq0[u] = Bc.rval[kl];
+      }
+#pragma unroll
+      for (int q = 0; q < kStg; ++q) {
+        const int x = tid + kMultiT * q;
+        const int e = x / (2 * n), h = (x / n) & 1, g = x % n;  // h: row a or row b
+        const int row = e < cj ? (ab_of(e) >> (8 * h)) & 255 : 0;
+        sx[q] = e < cj ? Xq[(size_t)n * row + g] : 0.0;
+        sz[q] = e < cj ? Zq[(size_t)n * row + g] : 0.0;
+      }
+    }
     __syncthreads();  // the previous pass is done with acc and the staged rows
     if (staged) {
       if (tid < cj) {
-        sja[tid] = er[sj + tid];
-        sjb[tid] = ec[sj + tid];
-        sjv[tid] = ev[sj + tid];
+        sja[tid] = ab_of(tid) & 255;
+        sjb[tid] = (ab_of(tid) >> 8) & 255;
+        sjv[tid] = v_of(tid);
       }
-      for (int x = tid; x < cj * 2 * n; x += kMultiT) {
-        const int e = x / (2 * n), h = (x / n) & 1, g = x % n;  // h: row a or row b
-        const int row = h ? ec[sj + e] : er[sj + e];
-        sR[(e * 4 + h) * kMultiN + g] = Xq[(size_t)n * row + g];
-        sR[(e * 4 + 2 + h) * kMultiN + g] = Zq[(size_t)n * row + g];
+#pragma unroll
+      for (int q = 0; q < kStg; ++q) {
+        const int x = tid + kMultiT * q;
+        const int e = x / (2 * n), h = (x / n) & 1, g = x % n;
+        if (e < cj) {
+          sR[(e * 4 + h) * kMultiN + g] = sx[q];
+          sR[(e * 4 + 2 + h) * kMultiN + g] = sz[q];
+        }
       }
     }
     __syncthreads();
@@ -145,7 +188,23 @@ __global__ __launch_bounds__(kMultiT) void k_f3_multi(const F3Blk* __restrict__ 
       const int co = jo ? cj : ci, cp = jo ? ci : cj, fo = jo ? fj : fi;
       if (!(fo == 3 && co <= kLight && cp <= kLight)) return;
       double val;
-      if (!staged) {
+      if (staged && cj == 1 && ci == 1) {
+        // one entry each, the common case: (a, b) of A_j staged, (g, d) of
+        // A_i in the row's record; the general loops' terms and order
+        const int g = (rec.w >> 8) & 255, d = (rec.w >> 16) & 255;
+        const double* R0 = sR;  // X row a, X row b, Z row a, Z row b of A_j's entry
+        if (jo) {
+          val = sjv[0] * (vi * f3_plu(sja[0] != sjb[0], g != d, R0[g], R0[d], R0[kMultiN + g],
+                                      R0[kMultiN + d], R0[2 * kMultiN + g],
+                                      R0[2 * kMultiN + d], R0[3 * kMultiN + g],
+                                      R0[3 * kMultiN + d]));
+        } else {
+          val = vi * (sjv[0] * f3_plu(g != d, sja[0] != sjb[0], R0[g], R0[kMultiN + g], R0[d],
+                                      R0[kMultiN + d], R0[2 * kMultiN + g],
+                                      R0[3 * kMultiN + g], R0[2 * kMultiN + d],
+                                      R0[3 * kMultiN + d]));
+        }
+      } else if (!staged) {
         val = jo ? f3_pair(er, ec, ev, sj, cj, si, ci, n, Xq, Zq, 0, 1)
                  : f3_pair(er, ec, ev, si, ci, sj, cj, n, Xq, Zq, 0, 1);
       } else if (jo) {
@@ -188,11 +247,16 @@ __global__ __launch_bounds__(kMultiT) void k_f3_multi(const F3Blk* __restrict__ 
       }
       acc[crow] += val;  // one thread per row in this pass
     };
-    const int ncon = Bc.ncon;
     if (staged) {
       // four rows per thread per round, their records loaded together
-      // (clamped indices stay in the list)
-      for (int k0 = pj + tid; k0 < ncon; k0 += 4 * kMultiT) {
+      // (clamped indices stay in the list); the first round's came with the
+      // staged rows
+#pragma unroll
+      for (int u = 0; u < kR0; ++u)
+        if (pj + tid + kMultiT * u < ncon)
+          add_row(rq0[u].x, rq0[u].y, rq0[u].z, rq0[u].w & 15, (rq0[u].w >> 4) & 3, rq0[u],
+                  vq0[u]);
+      for (int k0 = pj + tid + kR0 * kMultiT; k0 < ncon; k0 += 4 * kMultiT) {
         int4 rq[4];
         double vq[4];
 #pragma unroll
@@ -804,8 +868,9 @@ int make_bmat(Ctx* c, int precision) {
     // stores every entry of the lower triangle: the first writer of B
     hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(kMultiT),
                        sizeof(double) * ((size_t)m + (size_t)kMultiE * 4 * kMultiN), s,
-                       c->d_f3blk, c->d_jptr, c->d_jent, c->d_er, c->d_ec, c->d_ev, c->d_X,
-                       c->d_Zinv, c->d_B, m);
+                       c->d_f3blk, c->d_jptr, c->d_jent, (const int4*)c->d_jab,
+                       (const double4*)c->d_jv, c->d_er, c->d_ec, c->d_ev, c->d_X, c->d_Zinv,
+                       c->d_B, m);
   } else if (!c->bmatStoreFirst) {
     SDPA_HIP_CHECK(hipMemsetAsync(c->d_B, 0, sizeof(double) * (size_t)m * m, s));
   }

@@ -120,6 +120,7 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
                    (void**)&c->d_f3blk, (void**)&c->d_jptr, (void**)&c->d_jent,
+                   (void**)&c->d_jab, (void**)&c->d_jv,
                    (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P, (void**)&c->d_hpart, (void**)&c->d_xz32,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
@@ -638,6 +639,8 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
           for (int k : c->blk[q].h_con) ++cnt[k + 1];
       for (int k = 0; k < m; ++k) cnt[k + 1] += cnt[k];
       std::vector<int> jent(4 * (size_t)cnt[m]), fill(cnt.begin(), cnt.end() - 1);
+      std::vector<int> jab(4 * (size_t)cnt[m], 0);
+      std::vector<double> jv(4 * (size_t)cnt[m], 0.0);
       std::vector<F3Blk> fb(nS);
       for (int q = 0; q < nS; ++q) {
         const BlockTable& T = c->blk[q];
@@ -648,12 +651,19 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
           jent[4 * (size_t)at + 1] = pos;
           jent[4 * (size_t)at + 2] = T.h_estart[pos];
           jent[4 * (size_t)at + 3] = T.h_nzeff[pos];
+          const int st = T.h_estart[pos];
+          for (int e = 0; e < std::min(T.h_ecnt[pos], 4); ++e) {
+            jab[4 * (size_t)at + e] = h_er[st + e] | (h_ec[st + e] << 8);
+            jv[4 * (size_t)at + e] = h_ev[st + e];
+          }
         }
         fb[q] = F3Blk{T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff, T.d_form,
                       (const int4*)T.d_rmeta, T.d_rval, T.off, T.n, T.ncon};
       }
       if (int e = upload(c, &c->d_jptr, cnt)) return e;
       if (int e = upload(c, &c->d_jent, jent)) return e;
+      if (int e = upload(c, &c->d_jab, jab)) return e;
+      if (int e = upload(c, &c->d_jv, jv)) return e;
       if (int e = upload(c, &c->d_f3blk, fb)) return e;
       if (int e = f3multi_prepare(m)) return e;
       c->f3multi = 1;
