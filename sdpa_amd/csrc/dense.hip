@@ -24,6 +24,17 @@ constexpr int TB = 64;    // tile edge
 constexpr int TLD = 65;   // LDS leading dimension (pad)
 constexpr int BK = 16;    // k-depth per LDS stage in the streaming GEMM
 
+// Work-group barrier that orders LDS only.  __syncthreads() also waits for
+// every outstanding global load, which would stall the compute waves on the
+// tile they have just started to prefetch; nothing passes between the waves
+// of these kernels through global memory (the tagged words carry their own
+// validity).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------------------
 // acc += A(64 x kdepth) * B(kdepth x 64) from LDS tiles stored column-major
 // with leading dimension lda_s / ldb_s.  Wave w owns rows [32*(w&1), +32) and
@@ -349,15 +360,23 @@ struct Chol64Lds {
   int failc;
 };
 
+// Hook of the 64x64 factorization: called by all threads with q once the
+// columns 16q..16q+15 of the factor (in L.T) and the inverse of its 16x16
+// diagonal tile (in L.W) are final and visible to the work-group.
+struct NoPub {
+  __device__ __forceinline__ void operator()(int) const {}
+};
+
 // Factor the block in L.T (256 threads; L.failc must be 0 and W zero, both
 // visible to the work-group): 16-column mini-panels on wave 0, trailing
 // 16x16 tiles on MFMA, and the first three diagonal 16x16 inverses and
 // W[1][0] built alongside on waves 1 and 2.  MODE 0: LAPACK (a pivot that
 // is not > 0 fails); MODE 1: SDPA's rule (sdpa_dpotrf.cpp:31-33: < -1e-6
 // fails, < 1e-14 becomes 1e100).  kb: the block's true size (the padding
-// beyond it is the identity).  Returns L.failc.
-template <int MODE>
-__device__ __forceinline__ int chol64_factor(Chol64Lds& L, int kb) {
+// beyond it is the identity).  pub(q) is called for q = 0, 1, 2 (the last
+// one by chol64_inverse_finish).  Returns L.failc.
+template <int MODE, class Pub = NoPub>
+__device__ __forceinline__ int chol64_factor(Chol64Lds& L, int kb, Pub pub = Pub()) {
   double* T = L.T;
   double* W = L.W;
   double* S1 = L.S1;
@@ -419,7 +438,8 @@ __device__ __forceinline__ int chol64_factor(Chol64Lds& L, int kb) {
     } else if (wv == 2 && p == 3) {
       offdiag16_inverse(T, W, S1, 0);  // W[1][0] from D_0, D_1, L[1][0]
     }
-    __syncthreads();
+    lds_barrier();
+    if (p >= 1) pub(p - 1);
     // trailing update of the lower 16x16 tiles (bi, bj), p < bj <= bi < 4:
     // T[bi][bj] -= L[bi][p] L[bj][p]^T   (K = 16)
     const int nt = 3 - p, ntiles = nt * (nt + 1) / 2;
@@ -431,15 +451,17 @@ __device__ __forceinline__ int chol64_factor(Chol64Lds& L, int kb) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) T[(16 * bi + li) + TLD * (16 * bj + 4 * v + lk)] -= acc[v];
     }
-    if (p < 3) __syncthreads();
+    if (p < 3) lds_barrier();
   }
   return failc_s;
 }
 
 // Complete W = inv(L) of a block factored by chol64_factor (256 threads;
 // T is only read, so the factor may be written out concurrently).  Ends
-// with a barrier: W is complete for every thread.
-__device__ __forceinline__ void chol64_inverse_finish(Chol64Lds& L) {
+// with a barrier: W is complete for every thread.  pub(3) is called once the
+// last 16x16 diagonal inverse is in W.
+template <class Pub = NoPub>
+__device__ __forceinline__ void chol64_inverse_finish(Chol64Lds& L, Pub pub = Pub()) {
   double* T = L.T;
   double* W = L.W;
   double* S = L.S;
@@ -461,9 +483,10 @@ __device__ __forceinline__ void chol64_inverse_finish(Chol64Lds& L) {
       for (int v = 0; v < 4; ++v) S[(16 * ti + li) + SLD * (16 * tj + 4 * v + lk)] = u[v];
     }
   }
-  __syncthreads();
+  lds_barrier();
+  pub(3);
   if (wv == 0) offdiag16_inverse(T, W, S2, 2);  // W[3][2] from D_2, D_3, L[3][2]
-  __syncthreads();
+  lds_barrier();
   // W[B][A] = -W[B][B] U, wave = 16x16 output tile (ti, tj)
   {
     const int ti = wv & 1, tj = wv >> 1;
@@ -471,7 +494,7 @@ __device__ __forceinline__ void chol64_inverse_finish(Chol64Lds& L) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) W[(32 + 16 * ti + li) + TLD * (16 * tj + 4 * v + lk)] = -x[v];
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // The diagonal-block work of one item (256 threads).  kp >= 0: first apply
@@ -506,7 +529,7 @@ __device__ __forceinline__ void diag_block(const MatDesc& M, int b, int k0, int 
     for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = a[q];
   }
   if (t == 0) L.failc = 0;
-  __syncthreads();
+  lds_barrier();
   if (kp >= 0) {
     // T[bi][bj] -= X[bi] X[bj]^T on the ten lower 16x16 tiles (K = 64)
     for (int q = wv; q < 10; q += 4) {
@@ -517,11 +540,11 @@ __device__ __forceinline__ void diag_block(const MatDesc& M, int b, int k0, int 
 #pragma unroll
       for (int v = 0; v < 4; ++v) T[(16 * bi + li) + TLD * (16 * bj + 4 * v + lk)] -= acc[v];
     }
-    __syncthreads();
+    lds_barrier();
     const int r = t & 63, cq = t >> 6;
 #pragma unroll
     for (int q = 0; q < 16; ++q) W[r + TLD * (cq + 4 * q)] = 0.0;
-    __syncthreads();
+    lds_barrier();
   }
   const int failc = chol64_factor<MODE>(L, kb);
   if (failc) {
@@ -779,12 +802,18 @@ static void launch_syrk(hipStream_t s, const MatDesc* d_mats, const int* d_info,
 // MFMA operand and result layout, so the solves and updates of the row
 // block never leave the registers.  Step k (64 columns) of the panel:
 //   - work-group k factors its (updated) 64x64 diagonal block in LDS (the
-//     shared chol64 routines, SDPA's pivot rule in MODE 1) and publishes the
-//     block's inverse W_k;
-//   - every work-group b > k solves its block of column block k,
-//     L_bk = A_bk W_k^T (MFMA), stores it, publishes it if b < 4 (the rows
-//     of the panel's later diagonal blocks), and updates its column blocks
-//     k < j <= min(b, 3): A_bj -= L_bk L_jk^T.
+//     shared chol64 routines, SDPA's pivot rule in MODE 1); as each
+//     16-column strip of the factor becomes final it publishes the strip
+//     and the inverse D_q of the strip's 16x16 diagonal tile;
+//   - every work-group b > k solves its block of column block k strip by
+//     strip as the strips arrive (blocked forward substitution on MFMA:
+//     X_q = X_q D_q^T, then X_c -= X_q L_cq^T for the later strips c), so
+//     only the last strip's solve follows the diagonal factorization; it
+//     then stores L_bk, publishes it if b < 4 (the rows of the panel's later
+//     diagonal blocks), and updates its column blocks k < j <= min(b, 3):
+//     A_bj -= L_bk L_jk^T.  Work-group k + 1 (the next diagonal block)
+//     instead applies each solved strip to its own diagonal block at once and
+//     defers the store and publication of L_bk to its factorization.
 // Work-group b only ever waits for work-groups with a smaller index (k < b,
 // j < b), which the dispatcher has started before it, so the launch needs no
 // co-residency.  Values pass between work-groups as epoch-tagged 64-bit
@@ -794,9 +823,13 @@ static void launch_syrk(hipStream_t s, const MatDesc* d_mats, const int* d_info,
 // cache maintenance.  The chain from one diagonal block to the next is then
 // a single work-group's: fetch W_k, solve its own 64 x 64, update its
 // diagonal block, factor it -- without a kernel boundary.
-// tw layout (8192 words per 64x64 block): W_k at block k (k < 4), L_jk at
-// block 4 + 4 j + k; block 20 is the control word (WG 0's go / skip).
-constexpr int kPanelTagBlocks = 21;
+// tw layout (8192-word blocks): block k (k < 4) holds the strips of step k's
+// diagonal factor (strip q: rows 16q..63 of columns 16q..16q+15, element
+// e = row + 64 col at words 2048 q + 2 e); block 21 + k the inverses D_q of
+// its 16x16 diagonal tiles (element e = i + 16 j at words 512 q + 2 e, lower
+// part only); block 4 + 4 j + k holds L_jk; block 20 is the control word
+// (WG 0's go / skip).
+constexpr int kPanelTagBlocks = 25;
 constexpr int kTagSpin = 1 << 22;  // bounded waits: a timed-out wait sets *err
 
 typedef unsigned long long u64;
@@ -810,42 +843,136 @@ __device__ __forceinline__ void tag_store(u64* dst, double x, unsigned epoch) {
 }
 
 // The 64x64 block published at src (column-major element e = row + 64 col
-// in words 2e, 2e + 1) into LDS T (ld TLD); 256 threads, 16 elements each,
-// all loads issued before any wait.
+// in words 2e, 2e + 1) into LDS T (ld TLD); 256 threads, 16 elements each in
+// two rounds of 8.  A round issues its 16 loads at once; while some of its
+// elements are not yet valid it reloads all of those together (one round trip
+// per poll, not one per element).  LOWER: the block is lower triangular and
+// only its lower part is published; the upper part is zero-filled here.
+template <bool LOWER>
 __device__ __forceinline__ void tag_fetch64(const u64* src, unsigned epoch, double* T, int* err) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {  // two rounds of 8 elements: 16 words in flight
+  for (int h = 0; h < 2; ++h) {
     u64 w[16];
+    unsigned pend = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int e = t + 256 * (8 * h + q);
-      w[2 * q] = __hip_atomic_load(src + 2 * e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      w[2 * q + 1] =
-          __hip_atomic_load(src + 2 * e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (!LOWER || (e & 63) >= (e >> 6)) {
+        w[2 * q] = __hip_atomic_load(src + 2 * e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w[2 * q + 1] =
+            __hip_atomic_load(src + 2 * e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        w[2 * q] = w[2 * q + 1] = (u64)epoch << 32;  // a zero, valid
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if ((unsigned)(w[2 * q] >> 32) != epoch || (unsigned)(w[2 * q + 1] >> 32) != epoch)
+        pend |= 1u << q;
+    int spins = 0;
+    while (pend) {
+      // after one timed-out wait anywhere the launch stops waiting (its
+      // results are then discarded: the give-up flag is reported)
+      if (++spins > kTagSpin ||
+          ((spins & 1023) == 0 &&
+           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int e = t + 256 * (8 * h + q);
+        if ((pend >> q) & 1u) {
+          w[2 * q] = __hip_atomic_load(src + 2 * e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          w[2 * q + 1] =
+              __hip_atomic_load(src + 2 * e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if ((unsigned)(w[2 * q] >> 32) == epoch && (unsigned)(w[2 * q + 1] >> 32) == epoch)
+          pend &= ~(1u << q);
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int e = t + 256 * (8 * h + q);
-      int spins = 0;
-      while ((unsigned)(w[2 * q] >> 32) != epoch || (unsigned)(w[2 * q + 1] >> 32) != epoch) {
-        // after one timed-out wait anywhere the launch stops waiting (its
-        // results are then discarded: the give-up flag is reported)
-        if (++spins > kTagSpin ||
-            ((spins & 1023) == 0 &&
-             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        w[2 * q] = __hip_atomic_load(src + 2 * e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        w[2 * q + 1] =
-            __hip_atomic_load(src + 2 * e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
       T[(e & 63) + TLD * (e >> 6)] =
           __hiloint2double((int)(unsigned)w[2 * q + 1], (int)(unsigned)w[2 * q]);
     }
   }
+}
+
+// Strip q of a diagonal factor in T (rows 16q..63 of columns 16q..16q+15)
+// and D_q, the inverse of its 16x16 diagonal tile in W at (16q, 16q) (lower
+// part), published to the strip area sdst and the inverse area ddst.
+__device__ __forceinline__ void strip_publish(int q, const double* T, const double* W, u64* sdst,
+                                              u64* ddst, unsigned epoch) {
+  const int t = threadIdx.x;
+  u64* st = sdst + 2048 * q;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = t + 256 * k, r = e & 63, c = e >> 6;
+    if (r >= 16 * q) tag_store(st + 2 * e, T[r + TLD * (16 * q + c)], epoch);
+  }
+  const int i = t & 15, j = t >> 4;
+  if (i >= j) tag_store(ddst + 512 * q + 2 * t, W[(16 * q + i) + TLD * (16 * q + j)], epoch);
+}
+
+// Strip Q and D_Q (as published by strip_publish) into LDS: the strip at its
+// place in T, D_Q in Dq (16 x 16, ld 17, zero above the diagonal).  Five
+// elements per thread, all loads in flight at once, pending ones re-polled
+// together.
+template <int Q>
+__device__ __forceinline__ void strip_fetch(const u64* ssrc, const u64* dsrc, unsigned epoch,
+                                            double* T, double* Dq, int* err) {
+  const int t = threadIdx.x;
+  u64 w[10];
+  unsigned pend = 0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const bool need = k < 4 ? ((t & 63) >= 16 * Q) : ((t & 15) >= (t >> 4));
+    const u64* a = k < 4 ? ssrc + 2048 * Q + 2 * (t + 256 * k) : dsrc + 512 * Q + 2 * t;
+    if (need) {
+      w[2 * k] = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      w[2 * k + 1] = __hip_atomic_load(a + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((unsigned)(w[2 * k] >> 32) != epoch || (unsigned)(w[2 * k + 1] >> 32) != epoch)
+        pend |= 1u << k;
+    } else {
+      w[2 * k] = w[2 * k + 1] = (u64)epoch << 32;  // a zero, valid
+    }
+  }
+  int spins = 0;
+  while (pend) {
+    if (++spins > kTagSpin ||
+        ((spins & 1023) == 0 &&
+         __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const u64* a = k < 4 ? ssrc + 2048 * Q + 2 * (t + 256 * k) : dsrc + 512 * Q + 2 * t;
+      if ((pend >> k) & 1u) {
+        w[2 * k] = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w[2 * k + 1] = __hip_atomic_load(a + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if ((unsigned)(w[2 * k] >> 32) == epoch && (unsigned)(w[2 * k + 1] >> 32) == epoch)
+        pend &= ~(1u << k);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = t + 256 * k, r = e & 63, c = e >> 6;
+    if (r >= 16 * Q)
+      T[r + TLD * (16 * Q + c)] =
+          __hiloint2double((int)(unsigned)w[2 * k + 1], (int)(unsigned)w[2 * k]);
+  }
+  Dq[(t & 15) + 17 * (t >> 4)] = __hiloint2double((int)(unsigned)w[9], (int)(unsigned)w[8]);
 }
 
 // acc[c] = sum over the 64 columns of register block KB of this wave's rows
@@ -912,33 +1039,117 @@ __device__ __forceinline__ void rows_store(const double (&P)[64], double* A, int
   }
 }
 
-// Step K of the panel for a work-group below the diagonal block (b > K).
-template <int K, int MODE>
-__device__ __forceinline__ void panel_step_rows(double (&P)[64], Chol64Lds& L, int b, int nsteps,
-                                                double* A, int ld, int r0, int m, int K0,
-                                                const u64* tw, u64* twm, unsigned epoch,
-                                                int* err) {
-  // L_bK = A_bK W_K^T
-  tag_fetch64(tw + (size_t)K * 8192, epoch, L.T, err);
-  __syncthreads();
+// The next diagonal work-group's own block, A_bb -= X_q X_q^T, for the solved
+// strip q of block K (all 64 rows of it in U, this wave's 16 in registers).
+template <int K, int q>
+__device__ __forceinline__ void own_strip_update(double (&P)[64], const double* U) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(U[(16 * c + li) + TLD * (16 * q + 4 * s4 + lk)],
+                                                 P[16 * K + 4 * q + s4], acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) P[16 * (K + 1) + 4 * c + v] -= acc[v];
+  }
+}
+
+// Strip Q of step K for a work-group below the diagonal block: fetch the
+// strip and D_Q, apply the previous strip to the work-group's own diagonal
+// block (CHAIN: the next diagonal work-group), solve X_Q = X_Q D_Q^T (X_Q:
+// the wave's 16 rows of columns 16Q..16Q+15 of block K, registers
+// P[16 K + 4 Q + s]), keep it in U (CHAIN) and take it off the later strips:
+// X_c -= X_Q L_cQ^T.
+template <int K, int Q>
+__device__ __forceinline__ void strip_step(double (&P)[64], Chol64Lds& L, double* U, double* Dq,
+                                           bool chain, const u64* ssrc, const u64* dsrc,
+                                           unsigned epoch, int* err) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  strip_fetch<Q>(ssrc, dsrc, epoch, L.T, Dq + 272 * Q, err);
+  lds_barrier();
+  if constexpr (Q > 0) {
+    if (chain) own_strip_update<K, Q - 1>(P, U);
+  }
   {
-    v4d acc[4];
-    rows_times_xt<K>(P, L.T, acc);
+    const double* D = Dq + 272 * Q;
+    v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int s4 = 0; s4 < 4; ++s4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(D[li + 17 * (4 * s4 + lk)], P[16 * K + 4 * Q + s4],
+                                                 acc, 0, 0, 0);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) P[16 * K + 4 * c + v] = acc[c][v];
+    for (int v = 0; v < 4; ++v) P[16 * K + 4 * Q + v] = acc[v];
+  }
+  if (chain) {
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
+      U[(16 * w + li) + TLD * (16 * Q + 4 * s4 + lk)] = P[16 * K + 4 * Q + s4];
+  }
+#pragma unroll
+  for (int c = Q + 1; c < 4; ++c) {
+    v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L.T[(16 * c + li) + TLD * (16 * Q + 4 * s4 + lk)],
+                                                 P[16 * K + 4 * Q + s4], acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) P[16 * K + 4 * c + v] -= acc[v];
+  }
+}
+
+// Step K of the panel for a work-group below the diagonal block (b > K):
+// L_bK = A_bK L_KK^-T strip by strip (strip_step).  The next diagonal
+// work-group (b == K + 1 < nsteps) is the factorization's serial chain: it
+// applies each solved strip to its own diagonal block (its only later column
+// block in the panel) and goes straight on to factor it; its solved block
+// L_bK is left in U, and the matrix store and publication of it are made by
+// waves 1-3 while wave 0 factors (panel_step_diag).  The work-group after it
+// (b == K + 2) leaves the update of its own diagonal block by step K to the
+// start of step K + 1, so that it is ready for step K + 1's first strips as
+// soon as its column block K + 1 is up to date.
+template <int K, int MODE>
+__device__ __forceinline__ void panel_step_rows(double (&P)[64], Chol64Lds& L, double* U,
+                                                double* Dq, int b, int nsteps, double* A, int ld,
+                                                int r0, int m, int K0, const u64* tw, u64* twm,
+                                                unsigned epoch, int* err) {
+  const bool chain = K + 1 < 4 && b == K + 1 && b < nsteps;
+  const bool defer_own = K + 2 < 4 && b == K + 2 && b < nsteps;
+  if constexpr (K >= 1) {
+    if (chain) {
+      // the update of the own diagonal block by step K - 1, deferred from there
+      rows_to_lds<K - 1>(P, U);
+      lds_barrier();
+      v4d acc[4];
+      rows_times_xt<K - 1>(P, U, acc);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) P[16 * (K + 1) + 4 * c + v] -= acc[c][v];
+    }
+  }
+  const u64* ssrc = tw + (size_t)K * 8192;
+  const u64* dsrc = tw + (size_t)(21 + K) * 8192;
+  strip_step<K, 0>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
+  strip_step<K, 1>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
+  strip_step<K, 2>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
+  strip_step<K, 3>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
+  if (chain) {
+    lds_barrier();
+    own_strip_update<K, 3>(P, U);
+    return;
   }
   rows_store<K>(P, A, ld, r0, m, K0);
   if (b < 4) rows_publish<K>(P, twm + (size_t)(4 + 4 * b + K) * 8192, epoch);
   // A_bj -= L_bK L_jK^T for the panel's later column blocks j <= b
 #pragma unroll
   for (int j = K + 1; j < 4; ++j) {
-    if (j < nsteps && j <= b) {
-      __syncthreads();  // the LDS operand of the previous product is consumed
-      if (j < b) tag_fetch64(tw + (size_t)(4 + 4 * j + K) * 8192, epoch, L.T, err);
+    if (j < nsteps && j <= b && !(defer_own && j == b)) {
+      lds_barrier();  // the LDS operand of the previous product is consumed
+      if (j < b) tag_fetch64<false>(tw + (size_t)(4 + 4 * j + K) * 8192, epoch, L.T, err);
       else rows_to_lds<K>(P, L.T);
-      __syncthreads();
+      lds_barrier();
       v4d acc[4];
       rows_times_xt<K>(P, L.T, acc);
 #pragma unroll
@@ -947,28 +1158,50 @@ __device__ __forceinline__ void panel_step_rows(double (&P)[64], Chol64Lds& L, i
         for (int v = 0; v < 4; ++v) P[16 * j + 4 * c + v] -= acc[c][v];
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
-// Step K for work-group K itself: factor the diagonal block of its rows.
+// Step K for work-group K itself: factor the diagonal block of its rows,
+// publishing the factor's strips and their tile inverses as they become
+// final.  For K > 0 the work-group's solved block of the previous step,
+// L_{K,K-1}, waits in U: waves 1-3 store it to the matrix and publish it
+// (block 4 + 4 K + K - 1 of the exchange area) while wave 0 runs the
+// factorization's first column chain.  L.W is zero on entry (cleared at the
+// start of the launch; no other step writes it).
 template <int K, int MODE>
-__device__ __forceinline__ void panel_step_diag(double (&P)[64], Chol64Lds& L, double* A, int ld,
-                                                int r0, int m, int K0, int* info, double* dinv,
-                                                u64* twm, unsigned epoch) {
+__device__ __forceinline__ void panel_step_diag(double (&P)[64], Chol64Lds& L, const double* U,
+                                                double* A, int ld, int r0, int m, int K0,
+                                                int* info, double* dinv, u64* twm,
+                                                unsigned epoch) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
   const int kb = min(TB, m - r0);
-  // the block (identity beyond kb) into T, W cleared
+  // the block (identity beyond kb) into T
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
     const int r = 16 * w + li, c = 4 * s + lk;
     L.T[r + TLD * c] = (r < kb && c < kb) ? P[16 * K + s] : (r == c ? 1.0 : 0.0);
   }
-  for (int e = t; e < TB * TB; e += 256) L.W[(e & 63) + TLD * (e >> 6)] = 0.0;
   if (t == 0) L.failc = 0;
-  __syncthreads();
-  const int failc = chol64_factor<MODE>(L, kb);
+  lds_barrier();
+  if (K > 0 && w > 0) {
+    u64* dst = twm + (size_t)(4 + 4 * K + (K > 0 ? K - 1 : 0)) * 8192;
+    const int col0 = K0 + 64 * (K > 0 ? K - 1 : 0);
+    for (int e = t - 64; e < TB * TB; e += 192) {
+      const int r = e & 63, c = e >> 6;
+      const double x = U[r + TLD * c];
+      if (r0 + r < m && col0 + c < m) A[(r0 + r) + (size_t)ld * (col0 + c)] = x;
+      tag_store(dst + 2 * e, x, epoch);
+    }
+  }
+  u64* sdst = twm + (size_t)K * 8192;
+  u64* ddst = twm + (size_t)(21 + K) * 8192;
+  const double* T = L.T;
+  const double* W = L.W;
+  auto pub = [=](int q) { strip_publish(q, T, W, sdst, ddst, epoch); };
+  const int failc = chol64_factor<MODE>(L, kb, pub);
   if (failc && t == 0) atomicCAS(info, 0, r0 + failc);
-  // the factor (lower part) to the matrix, beside the inverse's last steps
+  chol64_inverse_finish(L, pub);
+  // the factor (lower part) to the matrix, the inverse to dinv
   {
     const int r = t & 63, cq = t >> 6;
 #pragma unroll
@@ -977,14 +1210,8 @@ __device__ __forceinline__ void panel_step_diag(double (&P)[64], Chol64Lds& L, d
       if (r >= c && r < kb && c < kb) A[(r0 + r) + (size_t)ld * (r0 + c)] = L.T[r + TLD * c];
     }
   }
-  chol64_inverse_finish(L);
   double* Dv = dinv + (size_t)(r0 / TB) * TB * TB;
-  u64* dst = twm + (size_t)K * 8192;
-  for (int e = t; e < TB * TB; e += 256) {
-    const double x = L.W[(e & 63) + TLD * (e >> 6)];
-    tag_store(dst + 2 * e, x, epoch);
-    Dv[e] = x;
-  }
+  for (int e = t; e < TB * TB; e += 256) Dv[e] = L.W[(e & 63) + TLD * (e >> 6)];
 }
 
 // The previous panel's update of one work-group's rows (the left-looking
@@ -1035,9 +1262,9 @@ __device__ __forceinline__ void panel_prologue(double (&P)[64], Chol64Lds& L, co
       for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        __syncthreads();  // the previous chunk is consumed
+        lds_barrier();  // the previous chunk is consumed
         chunk_to_lds(pre, L.T);
-        __syncthreads();
+        lds_barrier();
         // the next chunk: (j, q + 1), or (j + 1, 0)
         const int nq = q < 3 ? q + 1 : 0, njj = q < 3 ? j : j + 1;
         if (njj < nj) chunk_load(A, ld, m, K0 + 64 * njj, Kp + 64 * nq, pre);
@@ -1052,7 +1279,7 @@ __device__ __forceinline__ void panel_prologue(double (&P)[64], Chol64Lds& L, co
         for (int v = 0; v < 4; ++v) P[16 * j + 4 * c + v] -= acc[c][v];
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 template <int MODE>
@@ -1062,6 +1289,8 @@ __global__ __launch_bounds__(256) void k_panel_rows(double* __restrict__ A, int 
                                                        u64* __restrict__ tw, unsigned epoch,
                                                        int* __restrict__ err) {
   __shared__ Chol64Lds L;
+  __shared__ double U[TB * TLD];
+  __shared__ double Dq[4 * 272];
   __shared__ int skip_s;
   const int b = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
@@ -1107,17 +1336,19 @@ __global__ __launch_bounds__(256) void k_panel_rows(double* __restrict__ A, int 
       P[s] = (rin && col < ncols) ? src[(size_t)ld * col] : 0.0;
     }
   }
-  __syncthreads();
+  // the diagonal step's inverse buffer starts at zero (only its factorization writes it)
+  for (int e = t; e < TB * TB; e += 256) L.W[(e & 63) + TLD * (e >> 6)] = 0.0;
+  lds_barrier();
   if (skip_s) return;
   if (Kp >= 0) panel_prologue(P, L, A, ld, m, K0, Kp, r0, b, nsteps);
 #define PANEL_STEP(K)                                                                      \
   if (K < nsteps) {                                                                        \
     if (b == K) {                                                                          \
       /* the work-group's last step: its later column blocks are above the diagonal */    \
-      panel_step_diag<K, MODE>(P, L, A, ld, r0, m, K0, info, dinv, tw, epoch);             \
+      panel_step_diag<K, MODE>(P, L, U, A, ld, r0, m, K0, info, dinv, tw, epoch);           \
       return;                                                                              \
     }                                                                                      \
-    panel_step_rows<K, MODE>(P, L, b, nsteps, A, ld, r0, m, K0, tw, tw, epoch, err);       \
+    panel_step_rows<K, MODE>(P, L, U, Dq, b, nsteps, A, ld, r0, m, K0, tw, tw, epoch, err);     \
   }
   PANEL_STEP(0)
   PANEL_STEP(1)
@@ -2124,16 +2355,6 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(TrsvParams P) {
 // each barrier.
 constexpr int kLLThreads = 576;
 
-// Work-group barrier that orders LDS only.  __syncthreads() also waits for
-// every outstanding global load, which would stall the compute waves on the
-// tile they have just started to prefetch; nothing passes between the waves
-// of these kernels through global memory (the tagged words carry their own
-// validity).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 // Raw buffer loads: a 32-bit per-lane byte offset plus a wave-uniform one in a
 // scalar register, so 32 loads in flight cost 32 data registers and one
