@@ -1017,16 +1017,6 @@ __device__ __forceinline__ void rows_to_lds(const double (&P)[64], double* T) {
 }
 
 template <int KB>
-__device__ __forceinline__ void rows_publish(const double (&P)[64], u64* dst, unsigned epoch) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int e = (16 * w + li) + 64 * (4 * s + lk);
-    tag_store(dst + 2 * e, P[16 * KB + s], epoch);
-  }
-}
-
-template <int KB>
 __device__ __forceinline__ void rows_store(const double (&P)[64], double* A, int ld, int r0, int m,
                                            int K0) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
@@ -1065,7 +1055,7 @@ __device__ __forceinline__ void own_strip_update(double (&P)[64], const double* 
 template <int K, int Q>
 __device__ __forceinline__ void strip_step(double (&P)[64], Chol64Lds& L, double* U, double* Dq,
                                            bool chain, const u64* ssrc, const u64* dsrc,
-                                           unsigned epoch, int* err) {
+                                           u64* xdst, unsigned epoch, int* err) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
   strip_fetch<Q>(ssrc, dsrc, epoch, L.T, Dq + 272 * Q, err);
   lds_barrier();
@@ -1087,6 +1077,14 @@ __device__ __forceinline__ void strip_step(double (&P)[64], Chol64Lds& L, double
     for (int s4 = 0; s4 < 4; ++s4)
       U[(16 * w + li) + TLD * (16 * Q + 4 * s4 + lk)] = P[16 * K + 4 * Q + s4];
   }
+  if (xdst) {
+    // the solved strip for the work-groups below that update with it (the
+    // column layout of the whole 64 x 64 block L_bK: element row + 64 col)
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
+      tag_store(xdst + 2 * ((16 * w + li) + 64 * (16 * Q + 4 * s4 + lk)), P[16 * K + 4 * Q + s4],
+                epoch);
+  }
 #pragma unroll
   for (int c = Q + 1; c < 4; ++c) {
     v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
@@ -1104,8 +1102,9 @@ __device__ __forceinline__ void strip_step(double (&P)[64], Chol64Lds& L, double
 // work-group (b == K + 1 < nsteps) is the factorization's serial chain: it
 // applies each solved strip to its own diagonal block (its only later column
 // block in the panel) and goes straight on to factor it; its solved block
-// L_bK is left in U, and the matrix store and publication of it are made by
-// waves 1-3 while wave 0 factors (panel_step_diag).  The work-group after it
+// L_bK is left in U, and the matrix store of it is made by waves 1-3 while
+// wave 0 factors (panel_step_diag).  Work-groups b < 4 publish each solved
+// strip of L_bK at once, for the work-groups below that update with it.  The work-group after it
 // (b == K + 2) leaves the update of its own diagonal block by step K to the
 // start of step K + 1, so that it is ready for step K + 1's first strips as
 // soon as its column block K + 1 is up to date.
@@ -1131,17 +1130,18 @@ __device__ __forceinline__ void panel_step_rows(double (&P)[64], Chol64Lds& L, d
   }
   const u64* ssrc = tw + (size_t)K * 8192;
   const u64* dsrc = tw + (size_t)(21 + K) * 8192;
-  strip_step<K, 0>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
-  strip_step<K, 1>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
-  strip_step<K, 2>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
-  strip_step<K, 3>(P, L, U, Dq, chain, ssrc, dsrc, epoch, err);
+  // the panel's later diagonal blocks' rows are published strip by strip
+  u64* xdst = (b < 4 && b < nsteps) ? twm + (size_t)(4 + 4 * b + K) * 8192 : nullptr;
+  strip_step<K, 0>(P, L, U, Dq, chain, ssrc, dsrc, xdst, epoch, err);
+  strip_step<K, 1>(P, L, U, Dq, chain, ssrc, dsrc, xdst, epoch, err);
+  strip_step<K, 2>(P, L, U, Dq, chain, ssrc, dsrc, xdst, epoch, err);
+  strip_step<K, 3>(P, L, U, Dq, chain, ssrc, dsrc, xdst, epoch, err);
   if (chain) {
     lds_barrier();
     own_strip_update<K, 3>(P, U);
     return;
   }
   rows_store<K>(P, A, ld, r0, m, K0);
-  if (b < 4) rows_publish<K>(P, twm + (size_t)(4 + 4 * b + K) * 8192, epoch);
   // A_bj -= L_bK L_jK^T for the panel's later column blocks j <= b
 #pragma unroll
   for (int j = K + 1; j < 4; ++j) {
@@ -1164,9 +1164,9 @@ __device__ __forceinline__ void panel_step_rows(double (&P)[64], Chol64Lds& L, d
 // Step K for work-group K itself: factor the diagonal block of its rows,
 // publishing the factor's strips and their tile inverses as they become
 // final.  For K > 0 the work-group's solved block of the previous step,
-// L_{K,K-1}, waits in U: waves 1-3 store it to the matrix and publish it
-// (block 4 + 4 K + K - 1 of the exchange area) while wave 0 runs the
-// factorization's first column chain.  L.W is zero on entry (cleared at the
+// L_{K,K-1}, waits in U: waves 1-3 store it to the matrix while wave 0 runs
+// the factorization's first column chain (it was published strip by strip
+// as it was solved).  L.W is zero on entry (cleared at the
 // start of the launch; no other step writes it).
 template <int K, int MODE>
 __device__ __forceinline__ void panel_step_diag(double (&P)[64], Chol64Lds& L, const double* U,
@@ -1184,13 +1184,10 @@ __device__ __forceinline__ void panel_step_diag(double (&P)[64], Chol64Lds& L, c
   if (t == 0) L.failc = 0;
   lds_barrier();
   if (K > 0 && w > 0) {
-    u64* dst = twm + (size_t)(4 + 4 * K + (K > 0 ? K - 1 : 0)) * 8192;
     const int col0 = K0 + 64 * (K > 0 ? K - 1 : 0);
     for (int e = t - 64; e < TB * TB; e += 192) {
       const int r = e & 63, c = e >> 6;
-      const double x = U[r + TLD * c];
-      if (r0 + r < m && col0 + c < m) A[(r0 + r) + (size_t)ld * (col0 + c)] = x;
-      tag_store(dst + 2 * e, x, epoch);
+      if (r0 + r < m && col0 + c < m) A[(r0 + r) + (size_t)ld * (col0 + c)] = U[r + TLD * c];
     }
   }
   u64* sdst = twm + (size_t)K * 8192;
