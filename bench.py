@@ -230,23 +230,23 @@ def batched(steps, warmup, with_cpu):
     (sdpa_batch_solve).  Reports solves/s over the kernel time (and over the
     whole call, which includes the upload of the packed instances and the
     result copy), the phase counts, the iteration histogram and the gap
-    distribution; the launch shape is the faster of 256 and 128 threads per
-    work-group, each timed after one untimed launch (a two-point autotune)."""
+    distribution.  The launch configuration is the autotuner's
+    (sdpa_batch_autotune: work-group size x heavy-pair threshold x queue
+    order, timed on a 512-instance sample); the same batch is also solved with
+    the FP32 Schur factorization and FP64 refinement ("mixed")."""
     from sdpa_amd import batch as B
     from sdpa_amd import problems as PR
     insts = PR.maxcut_batch(4096)
     P = B.Packed(insts)
-    tune = {}
-    for thr in (256, 128):
-        B.solve(insts, packed=P, threads=thr)  # the first launch of a shape also loads code
-        tune[thr] = B.solve(insts, packed=P, threads=thr).kernel_ms
-    thr = min(tune, key=tune.get)
+    t0 = time.perf_counter()
+    best, table = B.autotune(insts, packed=P, sample=512)
+    tune_s = time.perf_counter() - t0
     for _ in range(warmup):
-        B.solve(insts, packed=P, threads=thr)
+        B.solve_cfg(insts, packed=P, cfg=best)
     kms, wall = [], []
     for _ in range(max(1, steps)):
         t0 = time.perf_counter()
-        r = B.solve(insts, packed=P, threads=thr)
+        r = B.solve_cfg(insts, packed=P, cfg=best)
         wall.append(time.perf_counter() - t0)
         kms.append(r.kernel_ms)
     k = float(np.mean(kms)) * 1e-3
@@ -255,9 +255,33 @@ def batched(steps, warmup, with_cpu):
     for ph in r.phase:
         phases[ph] = phases.get(ph, 0) + 1
     gaps = r.relative_gap
+    mcfg = B.config(best.threads, best.heavy, best.order, precision=1)
+    B.solve_cfg(insts, packed=P, cfg=mcfg)
+    mk = []
+    for _ in range(max(1, steps)):
+        rm = B.solve_cfg(insts, packed=P, cfg=mcfg)
+        mk.append(rm.kernel_ms)
+    mkm = float(np.mean(mk)) * 1e-3
+    mph = {}
+    for ph in rm.phase:
+        mph[ph] = mph.get(ph, 0) + 1
+    both = np.array([a == "pdOPT" and b == "pdOPT" for a, b in zip(r.phase, rm.phase)])
+    mixed = {"kernel_ms": mkm * 1e3, "value": len(insts) / mkm, "phases": mph,
+             "fp64_over_mixed": mkm and k / mkm,
+             "max_objective_rel_diff_vs_fp64": float(np.max(
+                 np.abs(rm.primal_obj[both] - r.primal_obj[both]) /
+                 np.maximum(np.abs(r.primal_obj[both]), 1e-8))) if both.any() else None,
+             "schur_factorizations": {"fp32": int(rm.stats[:, 0].sum()),
+                                      "fp64": int(rm.stats[:, 1].sum())},
+             "refinement_steps_per_fp32_solve": float(rm.stats[:, 2].sum() /
+                                                      max(1.0, 2.0 * rm.stats[:, 0].sum())),
+             "refinement_stalls": int(rm.stats[:, 3].sum())}
     out = {"workload": "maxcut_batch_4096_n8-64_p0.2_seed4 (trace row on every fourth)",
            "metric": "solves/s", "value": len(insts) / k, "wall_solves_per_s": len(insts) / w,
-           "kernel_ms": k * 1e3, "threads": thr, "autotune_kernel_ms": tune,
+           "kernel_ms": k * 1e3, "config": best.as_dict(), "autotune_s": tune_s,
+           "autotune_table": [{"threads": int(a), "heavy": int(b), "order": int(c), "ms": float(d)}
+                              for a, b, c, d in table],
+           "mixed": mixed,
            "phases": phases,
            "iterations": {int(i): int(c) for i, c in enumerate(np.bincount(r.iterations)) if c},
            "gap_histogram": {"<=1e-8": int((gaps <= 1e-8).sum()),

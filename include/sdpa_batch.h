@@ -43,6 +43,54 @@ int sdpa_batch_solve(int device, int count, const int* n, const int* m, const lo
                      const double* ev, long nent, const double* lambda, const double* param,
                      int threads, double* out, double* kernel_ms);
 
+/* Launch configuration of the batched solver.
+ *   threads:   work-items per work-group, 128 or 256;
+ *   heavy:     F3 pairs (i, j) with more than heavy = |A_i| |A_j| entry
+ *              products are evaluated by one wave each (lanes split the
+ *              owner's entries), lighter pairs by one thread each (>= 1);
+ *   order:     0 instances taken largest first (m^3/3 + n^3), 1 input order;
+ *   precision: 0 FP64 Schur Cholesky (rATL_dpotrfL's rule); 1 the Schur
+ *              matrix B equilibrated (s_i = B_ii^-1/2) and factored in FP32
+ *              (a pivot that is not > 0 falls back to FP64), each solve
+ *              refined in FP64 with B until LAPACK dsposv's test holds; a
+ *              residual that does not halve switches the instance to the FP64
+ *              factorization for the rest of its run;
+ *   mu_switch: precision 1: FP64 once mu < mu_switch (0: never).
+ * sdpa_batch_solve is sdpa_batch_solve_cfg with {threads, 64, 0, 0, 0}. */
+typedef struct {
+  int threads;
+  int heavy;
+  int order;
+  int precision;
+  double mu_switch;
+} sdpa_batch_config;
+
+/* sdpa_batch_solve with a launch configuration.  stats (may be null): 4
+ * doubles per instance -- Schur factorizations in FP32, in FP64, refinement
+ * steps, refinement stalls.  Returns the codes of sdpa_batch_solve, or -10
+ * for a configuration out of range. */
+int sdpa_batch_solve_cfg(int device, int count, const int* n, const int* m, const long* coff,
+                         const double* C, long nC, const long* boff, const double* b, long nb,
+                         const long* pptr, const int* cptr, long ncptr, const int* er,
+                         const int* ec, const double* ev, long nent, const double* lambda,
+                         const double* param, const sdpa_batch_config* cfg, double* out,
+                         double* stats, double* kernel_ms);
+
+/* Autotuner of the launch configuration: solves a sample of the batch
+ * (every (count / sample)-th instance, at least 64) under every candidate
+ * of threads {256, 128} x heavy {16, 32, 64, 128, 256} x order {0, 1}, at
+ * the given precision and mu switch, twice each, and writes the candidate
+ * with the smallest kernel time to *best.  table (may be null): 4 doubles
+ * per candidate (threads, heavy, order, kernel ms), at most table_cap rows.
+ * Returns the number of candidates timed (> 0) or an error code of
+ * sdpa_batch_solve_cfg. */
+int sdpa_batch_autotune(int device, int count, const int* n, const int* m, const long* coff,
+                        const double* C, long nC, const long* boff, const double* b, long nb,
+                        const long* pptr, const int* cptr, long ncptr, const int* er,
+                        const int* ec, const double* ev, long nent, const double* lambda,
+                        const double* param, int precision, double mu_switch, int sample,
+                        sdpa_batch_config* best, double* table, int table_cap);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): the same solve as sdpa_batch_solve, same arguments and

@@ -41,6 +41,33 @@ def _bind():
     return f
 
 
+class Config(C.Structure):
+    """sdpa_batch_config (include/sdpa_batch.h): launch shape and precision."""
+    _fields_ = [("threads", C.c_int), ("heavy", C.c_int), ("order", C.c_int),
+                ("precision", C.c_int), ("mu_switch", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def config(threads=256, heavy=64, order=0, precision=0, mu_switch=0.0) -> Config:
+    return Config(int(threads), int(heavy), int(order), int(precision), float(mu_switch))
+
+
+def _bind_cfg():
+    L = lib()
+    f = L.sdpa_batch_solve_cfg
+    if f.argtypes is None:
+        f.argtypes = [C.c_int, C.c_int, _ip, _ip, _lp, _dp, C.c_long, _lp, _dp, C.c_long, _lp, _ip,
+                      C.c_long, _ip, _ip, _dp, C.c_long, _dp, _dp, C.POINTER(Config), _dp,
+                      C.c_void_p, C.POINTER(C.c_double)]
+        g = L.sdpa_batch_autotune
+        g.argtypes = [C.c_int, C.c_int, _ip, _ip, _lp, _dp, C.c_long, _lp, _dp, C.c_long, _lp, _ip,
+                      C.c_long, _ip, _ip, _dp, C.c_long, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                      C.POINTER(Config), C.c_void_p, C.c_int]
+    return f
+
+
 class Packed:
     """Instances in the C ABI's concatenated layout (internal convention:
     C = -F0, b = c, A_i = F_i with its upper-triangle entries, 0-based)."""
@@ -100,9 +127,11 @@ class Packed:
 
 
 class BatchResult:
-    """Per-instance results (user convention), arrays of length count."""
+    """Per-instance results (user convention), arrays of length count.
+    stats (solve_cfg only): per instance the Schur factorizations in FP32 and
+    in FP64, the refinement steps and the refinement stalls (count x 4)."""
 
-    def __init__(self, out, kernel_ms):
+    def __init__(self, out, kernel_ms, stats=None):
         o = out.reshape(-1, 8)
         ph = o[:, 0].astype(int)
         self.phase_internal = ph
@@ -120,6 +149,7 @@ class BatchResult:
         gap = np.abs(a - b)
         self.relative_gap = np.where(mean < 1.0, gap, gap / np.maximum(mean, 1.0))
         self.kernel_ms = kernel_ms
+        self.stats = None if stats is None else stats.reshape(-1, 4)
 
 
 def _bind_noalloc():
@@ -183,3 +213,41 @@ def solve(problems, lambdas=None, device: int = 0, threads: int = 256, packed: P
     if rc != 0:
         raise GpuError(f"sdpa_batch_solve failed ({rc})")
     return BatchResult(out, ms.value)
+
+
+def solve_cfg(problems, lambdas=None, device: int = 0, cfg: Config | None = None,
+              packed: Packed | None = None, **params) -> BatchResult:
+    """solve() under a launch configuration (threads, heavy-pair threshold,
+    queue order, Schur precision and mu switch); the result carries the
+    per-instance factorization and refinement counts in .stats."""
+    P, lam, par = _args(problems, lambdas, packed, params)
+    cfg = cfg if cfg is not None else config()
+    out = np.zeros(8 * P.count)
+    st = np.zeros(4 * P.count)
+    ms = C.c_double(0.0)
+    rc = _bind_cfg()(device, P.count, P.n, P.m, P.coff, P.C, len(P.C), P.boff, P.b, len(P.b),
+                     P.pptr, P.cptr, len(P.cptr), P.er, P.ec, P.ev, len(P.ev), lam, par,
+                     C.byref(cfg), out, st.ctypes.data_as(C.c_void_p), C.byref(ms))
+    if rc != 0:
+        raise GpuError(f"sdpa_batch_solve_cfg failed ({rc})")
+    return BatchResult(out, ms.value, st)
+
+
+def autotune(problems, lambdas=None, device: int = 0, precision: int = 0,
+             mu_switch: float = 0.0, sample: int = 512, packed: Packed | None = None,
+             **params):
+    """The launch configuration search of sdpa_batch_autotune on a sample of
+    the batch.  Returns (best Config, table of (threads, heavy, order, ms))."""
+    P, lam, par = _args(problems, lambdas, packed, params)
+    _bind_cfg()
+    best = Config()
+    cap = 64
+    table = np.zeros(4 * cap)
+    rc = lib().sdpa_batch_autotune(device, P.count, P.n, P.m, P.coff, P.C, len(P.C), P.boff, P.b,
+                                   len(P.b), P.pptr, P.cptr, len(P.cptr), P.er, P.ec, P.ev,
+                                   len(P.ev), lam, par, int(precision), float(mu_switch),
+                                   int(sample), C.byref(best), table.ctypes.data_as(C.c_void_p),
+                                   cap)
+    if rc <= 0:
+        raise GpuError(f"sdpa_batch_autotune failed ({rc})")
+    return best, table[: 4 * min(rc, cap)].reshape(-1, 4)

@@ -44,12 +44,20 @@ constexpr int LDB = MMAX + 1;   // of the Schur matrix
 constexpr int kMats = 12;       // n x n work matrices per instance
 constexpr int kLz = 64;         // Lanczos tridiagonal limit (sqrt(64) + 10 steps)
 constexpr int kPadCols = 16;    // scratch columns behind A and B2 (straight-line loops)
+constexpr int kDefaultHeavy = 64;  // F3 pairs with more entry products: one wave each
 
 enum { noINFO, pFEAS, dFEAS, pdFEAS, pdINF, pFEAS_dINF, pINF_dFEAS, pdOPT, pUNBD, dUNBD };
 
 struct BParams {
   int maxIteration;
   double epsilonStar, epsilonDash, betaStar, betaBar, gammaStar, omegaStar, lowerBound, upperBound;
+  // launch configuration (sdpa_batch_config): F3 pairs with more entry
+  // products than `heavy` take one wave each; precision 1 factors the
+  // equilibrated Schur matrix in FP32 and refines the solves in FP64 while
+  // mu >= muSwitch
+  int heavy;
+  int precision;
+  double muSwitch;
 };
 
 struct BData {
@@ -72,6 +80,8 @@ struct BData {
   unsigned* next;     // queue head (zero at launch)
   double* work;
   double* out;
+  double* stats;      // 4 per instance (may be null): FP32 factorizations, FP64
+                      // factorizations, refinement steps, refinement stalls
 };
 
 template <int BT>
@@ -86,6 +96,9 @@ struct Shared {
   double trisc[2 * (BT / 64)];
   double sc[4];
   double pv[2][NMAX];         // pivots of the X and Z factorizations
+  double eqs[MMAX];           // precision 1: equilibration s_i = B_ii^-1/2
+  double xr[MMAX], rr[MMAX];  // precision 1: refined solution and residual
+  float fd[MMAX];             // precision 1: pivots of the FP32 factor
   unsigned short perm[kEnt];  // entries in position-major order
   unsigned short cp[MMAX + 1];
   unsigned char er[kEnt], ec[kEnt], con[kEnt], head[kEnt];
@@ -432,6 +445,138 @@ __device__ __forceinline__ void schur_solve(int m, double* x, Shared<BT>& sh) {
   __syncthreads();
 }
 
+// Precision 1: the Schur matrix B stays in the lower triangle of sh.S (FP64,
+// for the refinement residuals); the FP32 Cholesky factor of the
+// equilibrated matrix s_i B_ij s_j (s_i = B_ii^-1/2, unit diagonal) takes the
+// strictly upper slots -- L_ij (i > j) as a float in the first half of the
+// double slot S[j + LDB i] -- and its pivots sh.fd.  Right-looking with one
+// barrier per column, LAPACK's rule (a pivot that is not > 0 fails: the
+// caller then factors B in FP64 with SDPA's rule).  Returns 0 or k + 1.
+template <int BT>
+__device__ __forceinline__ float& f32_slot(Shared<BT>& sh, int i, int j) {
+  return *reinterpret_cast<float*>(&sh.S[j + LDB * i]);  // i > j
+}
+
+template <int BT>
+__device__ __forceinline__ int chol_schur32(int m, Shared<BT>& sh) {
+  double* S = sh.S;
+  for (int i = threadIdx.x; i < m; i += BT) {
+    const double d = S[i + LDB * i];
+    sh.eqs[i] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m * m; e += BT) {
+    const int i = e % m, j = e / m;
+    if (i > j) f32_slot(sh, i, j) = (float)(sh.eqs[i] * S[i + LDB * j] * sh.eqs[j]);
+    else if (i == j) sh.fd[i] = (float)(sh.eqs[i] * S[i + LDB * i] * sh.eqs[i]);
+  }
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    const float d = sh.fd[k];
+    if (!(d > 0.0f)) return k + 1;
+    const float rs = 1.0f / sqrtf(d);
+    const int rem = m - k - 1, tot = rem * rem;
+    // the lower triangle (i >= j > k) of the trailing matrix
+    for (int e = threadIdx.x; e < tot; e += BT) {
+      const int ii = k + 1 + e % rem, jj = k + 1 + e / rem;
+      if (ii < jj) continue;
+      const float lik = f32_slot(sh, ii, k) * rs, ljk = f32_slot(sh, jj, k) * rs;
+      if (ii == jj) sh.fd[ii] -= lik * ljk;
+      else f32_slot(sh, ii, jj) -= lik * ljk;
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < m; k += BT) {
+    const float s = sqrtf(sh.fd[k]), rs = 1.0f / s;
+    for (int i = k + 1; i < m; ++i) f32_slot(sh, i, k) *= rs;
+    sh.fd[k] = s;
+  }
+  __syncthreads();
+  return 0;
+}
+
+// x := (L L^T)^-1 x with the FP32 factor (one wave, FP32 arithmetic; lane l
+// keeps x_l and x_{l+64})
+template <int BT>
+__device__ __forceinline__ void schur_solve32(int m, double* x, Shared<BT>& sh) {
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
+    float x0 = l < m ? (float)x[l] : 0.0f, x1 = l + 64 < m ? (float)x[l + 64] : 0.0f;
+    const float p0 = l < m ? 1.0f / sh.fd[l] : 0.0f, p1 = l + 64 < m ? 1.0f / sh.fd[l + 64] : 0.0f;
+    for (int k = 0; k < m; ++k) {
+      const float t0 = l > k && l < m ? f32_slot(sh, l, k) : 0.0f;
+      const float t1 = l + 64 > k && l + 64 < m ? f32_slot(sh, l + 64, k) : 0.0f;
+      const float xk = k < 64 ? __shfl(x0, k) * __shfl(p0, k) : __shfl(x1, k - 64) * __shfl(p1, k - 64);
+      x0 = (l == k) ? xk : x0 - t0 * xk;
+      x1 = (l + 64 == k) ? xk : x1 - t1 * xk;
+    }
+    for (int k = m - 1; k >= 0; --k) {
+      const float t0 = l < k ? f32_slot(sh, k, l) : 0.0f;
+      const float t1 = l + 64 < k ? f32_slot(sh, k, l + 64) : 0.0f;
+      const float xk = k < 64 ? __shfl(x0, k) * __shfl(p0, k) : __shfl(x1, k - 64) * __shfl(p1, k - 64);
+      x0 = (l == k) ? xk : x0 - t0 * xk;
+      x1 = (l + 64 == k) ? xk : x1 - t1 * xk;
+    }
+    if (l < m) x[l] = x0;
+    if (l + 64 < m) x[l + 64] = x1;
+  }
+  __syncthreads();
+}
+
+// x := B^-1 x by the FP32 factor of the equilibrated matrix and FP64
+// iterative refinement: r = x0 - B x in FP64 (B's lower triangle), until
+// LAPACK dsposv's test on the equilibrated system holds,
+// |S r|_inf <= sqrt(m) eps |S B S|_inf |S^-1 x|_inf.  A residual that does
+// not halve is a stall: returns 1 (x unchanged), else 0.  *steps: the
+// refinement steps taken.
+template <int BT>
+__device__ __forceinline__ int schur_refine(int m, double* x, Shared<BT>& sh, int* steps) {
+  double* S = sh.S;
+  // |S B S|_inf (every thread gets it)
+  double bn = 0.0;
+  for (int i = threadIdx.x; i < m; i += BT) {
+    double a = 0.0;
+    for (int j = 0; j < m; ++j) a += fabs(S[max(i, j) + LDB * min(i, j)]) * sh.eqs[j];
+    bn = fmax(bn, a * sh.eqs[i]);
+  }
+  {
+    double o = 0.0;
+    bmax2(bn, o, sh);
+  }
+  for (int i = threadIdx.x; i < m; i += BT) {
+    sh.rr[i] = x[i];
+    sh.xr[i] = 0.0;
+  }
+  __syncthreads();
+  double prev = -1.0;
+  for (int it = 0; it < 30; ++it) {
+    for (int i = threadIdx.x; i < m; i += BT) sh.rr[i] *= sh.eqs[i];
+    __syncthreads();
+    schur_solve32(m, sh.rr, sh);
+    for (int i = threadIdx.x; i < m; i += BT) sh.xr[i] += sh.eqs[i] * sh.rr[i];
+    __syncthreads();
+    // r = x0 - B xr, and the norms of the test
+    double rn = 0.0, xn = 0.0;
+    for (int i = threadIdx.x; i < m; i += BT) {
+      double a = x[i];
+      for (int j = 0; j < m; ++j) a -= S[max(i, j) + LDB * min(i, j)] * sh.xr[j];
+      sh.rr[i] = a;
+      rn = fmax(rn, fabs(a) * sh.eqs[i]);
+      xn = fmax(xn, fabs(sh.xr[i]) / sh.eqs[i]);
+    }
+    bmax2(rn, xn, sh);
+    ++*steps;
+    if (rn <= sqrt((double)m) * 1.1102230246251565e-16 * bn * xn) {
+      for (int i = threadIdx.x; i < m; i += BT) x[i] = sh.xr[i];
+      __syncthreads();
+      return 0;
+    }
+    if (prev >= 0.0 && rn > 0.5 * prev) return 1;
+    prev = rn;
+  }
+  return 1;
+}
+
 // M (LDS, ld LDA, symmetric n x n, lower part read) := L^-1 (lower) where
 // M = L L^T, with LAPACK's rule (a pivot that is not > 0 fails), by a group
 // of NS waves (ws = this wave's index in the group).  Every wave of the
@@ -532,7 +677,6 @@ __device__ __forceinline__ double calF3(const Shared<BT>& sh, int i, int j, int 
   return ret;
 }
 
-constexpr int kHeavy = 64;  // pairs with more entry products than this: one wave each
 
 // the owner of pair (i, j): the larger NonZeroEffect (ties: the smaller index)
 template <int BT>
@@ -546,7 +690,7 @@ __device__ __forceinline__ bool j_owns(const Shared<BT>& sh, int i, int j) {
 // split the owner's entries)
 template <int BT>
 __device__ __forceinline__ void make_bmat(int n, int m, const double* X, const double* Zi,
-                                          Shared<BT>& sh) {
+                                          int kHeavy, Shared<BT>& sh) {
   for (int e = threadIdx.x; e < n * n; e += BT) {
     const int i = e % n, j = e / n;
     sh.A[i + LDA * j] = X[e];
@@ -787,6 +931,10 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
     }
     int phase = noINFO, failure = 0;
     double alphaP = 0.0, alphaD = 0.0;
+    // precision 1: FP32 factor + FP64 refinement until mu < muSwitch or the
+    // first refinement stall (then FP64 for the rest of the instance's run)
+    bool fp64_only = Pm.precision != 1;
+    int nf32 = 0, nf64 = 0, nref = 0, nstall = 0;
     // Phase::updateCheck (sdpa_parts.cpp:1150-1242)
     auto update_phase = [&]() -> bool {
       const double NONZERO = 1.0e-6, acc = Pm.epsilonDash;
@@ -895,14 +1043,37 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
       const int infeasible = (phase == pFEAS || phase == noINFO) ? 1 : 0;
       // Newton::Mehrotra(PREDICTOR)
       make_gvec(0, betaVal * muCur, infeasible);
-      make_bmat(n, m, X, Zi, sh);
-      if (chol_schur(m, sh)) {
+      make_bmat(n, m, X, Zi, Pm.heavy, sh);
+      bool use32 = !fp64_only && muCur >= Pm.muSwitch;
+      if (use32 && chol_schur32(m, sh)) use32 = false;  // B is intact: factor it in FP64
+      if (use32) {
+        ++nf32;
+      } else {
+        ++nf64;
+        if (chol_schur(m, sh)) {
+          failure = 1;
+          break;
+        }
+      }
+      // dy := B^-1 g; a refinement stall falls back to the FP64 factorization
+      auto schur_dy = [&]() -> bool {
+        for (int i = tid; i < m; i += BT) sh.dy[i] = sh.g[i];
+        __syncthreads();
+        if (use32) {
+          if (!schur_refine(m, sh.dy, sh, &nref)) return true;
+          ++nstall;
+          ++nf64;
+          use32 = false;
+          fp64_only = true;
+          if (chol_schur(m, sh)) return false;
+        }
+        schur_solve(m, sh.dy, sh);
+        return true;
+      };
+      if (!schur_dy()) {
         failure = 1;
         break;
       }
-      for (int i = tid; i < m; i += BT) sh.dy[i] = sh.g[i];
-      __syncthreads();
-      schur_solve(m, sh.dy, sh);
       make_direction(infeasible);
       double q[7];
       direction_dots(q, false);
@@ -933,9 +1104,10 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
       }
       // Newton::Mehrotra(CORRECTOR)
       make_gvec(1, betaVal * muCur, infeasible);
-      for (int i = tid; i < m; i += BT) sh.dy[i] = sh.g[i];
-      __syncthreads();
-      schur_solve(m, sh.dy, sh);
+      if (!schur_dy()) {
+        failure = 1;
+        break;
+      }
       make_direction(infeasible);
       // StepLength::MehrotraCorrector
       const double lamX = min_eig(n, LXi, Dx, T, sh);
@@ -1043,6 +1215,13 @@ __global__ __launch_bounds__(BT) void k_batch_solve(BData Dt, BParams Pm) {
       o[5] = normD;
       o[6] = muCur;
       o[7] = failure;
+      if (Dt.stats) {
+        double* st = Dt.stats + 4 * (size_t)k;
+        st[0] = nf32;
+        st[1] = nf64;
+        st[2] = nref;
+        st[3] = nstall;
+      }
     }
     __syncthreads();
   }
@@ -1059,7 +1238,8 @@ using namespace sdpa_amd;
 static int check_batch(int count, const int* n, const int* m, const long* coff, long nC,
                        const long* boff, long nb, const long* pptr, const int* cptr, long ncptr,
                        const int* er, const int* ec, long nent, const double* lambda,
-                       std::vector<int>& order, std::vector<long>* woff, long* wtot) {
+                       std::vector<int>& order, std::vector<long>* woff, long* wtot,
+                       int input_order = 0) {
   long tot = 0;
   if (woff) woff->assign(count, 0);
   for (int k = 0; k < count; ++k) {
@@ -1083,6 +1263,10 @@ static int check_batch(int count, const int* n, const int* m, const long* coff, 
   // queue then ends on small ones and the work-groups finish together
   order.resize(count);
   std::iota(order.begin(), order.end(), 0);
+  if (input_order) {
+    if (wtot) *wtot = tot;
+    return 0;
+  }
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
     const double wa = (double)m[a] * m[a] * m[a] / 3.0 + (double)n[a] * n[a] * n[a];
     const double wc = (double)m[c] * m[c] * m[c] / 3.0 + (double)n[c] * n[c] * n[c];
@@ -1092,21 +1276,26 @@ static int check_batch(int count, const int* n, const int* m, const long* coff, 
   return 0;
 }
 
-extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* m,
-                                const long* coff, const double* C, long nC, const long* boff,
-                                const double* b, long nb, const long* pptr, const int* cptr,
-                                long ncptr, const int* er, const int* ec, const double* ev,
-                                long nent, const double* lambda, const double* param,
-                                int threads, double* out, double* kernel_ms) {
+extern "C" int sdpa_batch_solve_cfg(int device, int count, const int* n, const int* m,
+                                    const long* coff, const double* C, long nC, const long* boff,
+                                    const double* b, long nb, const long* pptr, const int* cptr,
+                                    long ncptr, const int* er, const int* ec, const double* ev,
+                                    long nent, const double* lambda, const double* param,
+                                    const sdpa_batch_config* cfg, double* out, double* stats,
+                                    double* kernel_ms) {
   if (count <= 0 || !n || !m || !coff || !C || !boff || !b || !pptr || !cptr || !er || !ec ||
-      !ev || !lambda || !param || !out)
+      !ev || !lambda || !param || !out || !cfg)
     return -1;
+  const int threads = cfg->threads;
   if (threads != 128 && threads != 256) return -2;
+  if (cfg->heavy < 1 || (cfg->order != 0 && cfg->order != 1) ||
+      (cfg->precision != 0 && cfg->precision != 1) || !(cfg->mu_switch >= 0.0))
+    return -10;
   std::vector<long> woff;
   std::vector<int> order;
   long wtot = 0;
   if (int e = check_batch(count, n, m, coff, nC, boff, nb, pptr, cptr, ncptr, er, ec, nent,
-                          lambda, order, &woff, &wtot))
+                          lambda, order, &woff, &wtot, cfg->order))
     return e;
   if (hipSetDevice(device) != hipSuccess) return -100;
   hipStream_t s;
@@ -1116,6 +1305,7 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
   unsigned* dnext = nullptr;
   long *dco = nullptr, *dbo = nullptr, *dpp = nullptr, *dwo = nullptr;
   double *dC = nullptr, *db = nullptr, *dev = nullptr, *dl = nullptr, *dw = nullptr, *dout = nullptr;
+  double* dst = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
   auto up = [&](void** dst, const void* src, size_t bytes) -> int {
@@ -1152,10 +1342,14 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
       rc = -100;
       break;
     }
+    if (stats && hipMalloc(&dst, sizeof(double) * 4 * count) != hipSuccess) {
+      rc = -100;
+      break;
+    }
     BData D{count, dn, dm, dco, dC, dbo, db, dpp, dcp, der, dec, dev, dl, dwo, dord, dnext, dw,
-            dout};
+            dout, dst};
     BParams P{(int)param[0], param[1], param[2], param[3], param[4], param[5], param[6], param[7],
-              param[8]};
+              param[8], cfg->heavy, cfg->precision, cfg->mu_switch};
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
       rc = -100;
@@ -1183,15 +1377,108 @@ extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* 
     if (kernel_ms) *kernel_ms = ms;
     if (hipMemcpy(out, dout, sizeof(double) * 8 * count, hipMemcpyDeviceToHost) != hipSuccess)
       rc = -100;
+    if (stats && hipMemcpy(stats, dst, sizeof(double) * 4 * count, hipMemcpyDeviceToHost) !=
+                     hipSuccess)
+      rc = -100;
   } while (0);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   for (void* p : {(void*)dn, (void*)dm, (void*)dcp, (void*)der, (void*)dec, (void*)dco, (void*)dbo,
                   (void*)dpp, (void*)dwo, (void*)dC, (void*)db, (void*)dev, (void*)dl, (void*)dw,
-                  (void*)dout, (void*)dord, (void*)dnext})
+                  (void*)dout, (void*)dord, (void*)dnext, (void*)dst})
     if (p) (void)hipFree(p);
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+extern "C" int sdpa_batch_solve(int device, int count, const int* n, const int* m,
+                                const long* coff, const double* C, long nC, const long* boff,
+                                const double* b, long nb, const long* pptr, const int* cptr,
+                                long ncptr, const int* er, const int* ec, const double* ev,
+                                long nent, const double* lambda, const double* param,
+                                int threads, double* out, double* kernel_ms) {
+  const sdpa_batch_config cfg{threads, kDefaultHeavy, 0, 0, 0.0};
+  return sdpa_batch_solve_cfg(device, count, n, m, coff, C, nC, boff, b, nb, pptr, cptr, ncptr,
+                              er, ec, ev, nent, lambda, param, &cfg, out, nullptr, kernel_ms);
+}
+
+// Launch-configuration search: every candidate of the grid (work-group
+// size x heavy-pair threshold x queue order) runs the whole solve of a
+// sample of the batch (every stride-th instance, at least 64 and at most
+// `sample` of them) twice, the faster kernel time of the two counting;
+// the fastest candidate is returned.  The precision and mu switch are the
+// caller's and are not searched (they change the results, the launch shape
+// does not change what converges).
+extern "C" int sdpa_batch_autotune(int device, int count, const int* n, const int* m,
+                                   const long* coff, const double* C, long nC, const long* boff,
+                                   const double* b, long nb, const long* pptr, const int* cptr,
+                                   long ncptr, const int* er, const int* ec, const double* ev,
+                                   long nent, const double* lambda, const double* param,
+                                   int precision, double mu_switch, int sample,
+                                   sdpa_batch_config* best, double* table, int table_cap) {
+  if (count <= 0 || !best || sample < 1) return -1;
+  // the sample: instances k = stride * i, renumbered with their own offsets
+  const int ns = std::min(count, std::max(64, sample));
+  const int stride = std::max(1, count / ns);
+  std::vector<int> sn, sm, scp, ser, sec;
+  std::vector<long> sco, sbo, spp;
+  std::vector<double> sC, sb, sev, slam;
+  for (int i = 0; i < ns && (long)i * stride < count; ++i) {
+    const int k = i * stride;
+    if (n[k] < 1 || n[k] > NMAX || m[k] < 1 || m[k] > MMAX || coff[k] < 0 ||
+        coff[k] + (long)n[k] * n[k] > nC || boff[k] < 0 || boff[k] + m[k] > nb || pptr[k] < 0 ||
+        pptr[k] + m[k] + 1 > ncptr)
+      return -4;
+    sn.push_back(n[k]);
+    sm.push_back(m[k]);
+    sco.push_back((long)sC.size());
+    sC.insert(sC.end(), C + coff[k], C + coff[k] + (long)n[k] * n[k]);
+    sbo.push_back((long)sb.size());
+    sb.insert(sb.end(), b + boff[k], b + boff[k] + m[k]);
+    spp.push_back((long)scp.size());
+    const int e0 = cptr[pptr[k]], e1 = cptr[pptr[k] + m[k]];
+    if (e0 < 0 || e1 < e0 || e1 > nent) return -5;
+    for (int q = 0; q <= m[k]; ++q) scp.push_back(cptr[pptr[k] + q] - e0 + (int)sev.size());
+    ser.insert(ser.end(), er + e0, er + e1);
+    sec.insert(sec.end(), ec + e0, ec + e1);
+    sev.insert(sev.end(), ev + e0, ev + e1);
+    slam.push_back(lambda[k]);
+  }
+  const int cnt = (int)sn.size();
+  std::vector<double> out(8 * (size_t)cnt);
+  const int thr[2] = {256, 128};
+  const int hv[5] = {16, 32, 64, 128, 256};
+  double bestms = -1.0;
+  int row = 0;
+  for (int a = 0; a < 2; ++a)
+    for (int h = 0; h < 5; ++h)
+      for (int o = 0; o < 2; ++o) {
+        const sdpa_batch_config cfg{thr[a], hv[h], o, precision, mu_switch};
+        double ms = -1.0;
+        for (int rep = 0; rep < 2; ++rep) {
+          double t = 0.0;
+          const int rc = sdpa_batch_solve_cfg(
+              device, cnt, sn.data(), sm.data(), sco.data(), sC.data(), (long)sC.size(),
+              sbo.data(), sb.data(), (long)sb.size(), spp.data(), scp.data(), (long)scp.size(),
+              ser.data(), sec.data(), sev.data(), (long)sev.size(), slam.data(), param, &cfg,
+              out.data(), nullptr, &t);
+          if (rc) return rc;
+          ms = (ms < 0.0 || t < ms) ? t : ms;
+        }
+        if (table && row < table_cap) {
+          double* r = table + 4 * (size_t)row;
+          r[0] = cfg.threads;
+          r[1] = cfg.heavy;
+          r[2] = cfg.order;
+          r[3] = ms;
+        }
+        ++row;
+        if (bestms < 0.0 || ms < bestms) {
+          bestms = ms;
+          *best = cfg;
+        }
+      }
+  return row;
 }
 
 // ---------------------------------------------------------------------------
@@ -1306,9 +1593,10 @@ extern "C" int sdpa_batch_solve_noalloc(int device, int count, const int* n, con
           reinterpret_cast<const int*>(B0 + o_ord),
           dnext,
           reinterpret_cast<double*>(B0 + o_w),
-          reinterpret_cast<double*>(B0 + o_out)};
+          reinterpret_cast<double*>(B0 + o_out),
+          nullptr};
   BParams P{(int)param[0], param[1], param[2], param[3], param[4], param[5], param[6], param[7],
-            param[8]};
+            param[8], kDefaultHeavy, 0, 0.0};
   if (threads == 256)
     hipLaunchKernelGGL(k_batch_solve<256>, dim3(grid), dim3(256), 0, s, D, P);
   else

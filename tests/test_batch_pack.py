@@ -57,3 +57,14 @@ def test_pack_rejects_out_of_range():
         B.Packed([PR.maxcut(65, 0.1, 1)])
     with pytest.raises(ValueError):
         B.Packed([PR.multi_block(2, 5, 8, 3, 4, seed=1)])
+
+
+def test_config_struct_layout():
+    """sdpa_batch_config: four ints then a double (24 bytes, the C layout)."""
+    import ctypes
+    from sdpa_amd import batch as B
+    assert ctypes.sizeof(B.Config) == 24
+    assert B.Config.mu_switch.offset == 16
+    c = B.config(threads=128, heavy=32, order=1, precision=1, mu_switch=1e-3)
+    assert c.as_dict() == {"threads": 128, "heavy": 32, "order": 1, "precision": 1,
+                           "mu_switch": 1e-3}

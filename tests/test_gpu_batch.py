@@ -153,3 +153,106 @@ def test_batch4096_matches_oracle_fixture():
           f"differ: {differ}")
     assert len(differ) <= 4
     assert all({a, b} == {"pdOPT", "pdFEAS"} for _, a, b in differ)
+
+
+def test_batch_mixed_precision_matches_oracle():
+    """precision 1 (FP32 Schur factor of the equilibrated matrix, FP64
+    refinement to dsposv's test): the oracle's FP64 results to the same
+    tolerances as the FP64 launch, with the FP32 factorization taken on most
+    iterations."""
+    insts = instances()
+    r = B.solve_cfg(insts, cfg=B.config(precision=1))
+    for t, d in enumerate(insts):
+        o = O.Problem(d).solve()
+        print(d.name, r.phase[t], o.phase, r.iterations[t], o.iterations, r.primal_obj[t],
+              o.primal_obj, r.stats[t])
+        check(d, o, r, t)
+    st = r.stats
+    assert st[:, 0].sum() > 0.5 * (st[:, 0] + st[:, 1]).sum()  # mostly FP32 factorizations
+    assert (st[:, 2] >= st[:, 0]).all()  # at least one refinement step per FP32 solve pair
+
+
+def test_batch_mu_switch_and_fp64_identity():
+    """precision 1 with a switch above every mu never factors in FP32 and
+    gives the FP64 launch's results bit for bit; a switch inside the run
+    factors in FP32 first and in FP64 after."""
+    insts = PR.maxcut_batch(24, nmin=8, nmax=48, seed=21)
+    r64 = B.solve_cfg(insts, cfg=B.config(precision=0))
+    rhi = B.solve_cfg(insts, cfg=B.config(precision=1, mu_switch=1e30))
+    assert (rhi.stats[:, 0] == 0).all()
+    assert np.array_equal(rhi.primal_obj, r64.primal_obj)
+    assert np.array_equal(rhi.iterations, r64.iterations)
+    rmid = B.solve_cfg(insts, cfg=B.config(precision=1, mu_switch=1e-2))
+    assert (rmid.stats[:, 0] > 0).all() and (rmid.stats[:, 1] > 0).all()
+    for t, d in enumerate(insts):
+        o = O.Problem(d).solve()
+        check(d, o, rmid, t)
+
+
+def test_batch_launch_configurations():
+    """Every heavy-pair threshold and queue order of the autotuner's grid
+    solves to the oracle's results; the queue order does not change any
+    instance's arithmetic (bitwise equal)."""
+    insts = instances()
+    ref = [O.Problem(d).solve() for d in insts]
+    for heavy in (16, 256):
+        r0 = B.solve_cfg(insts, cfg=B.config(threads=128, heavy=heavy, order=0))
+        r1 = B.solve_cfg(insts, cfg=B.config(threads=128, heavy=heavy, order=1))
+        assert np.array_equal(r0.primal_obj, r1.primal_obj)
+        assert np.array_equal(r0.iterations, r1.iterations)
+        for t, d in enumerate(insts):
+            check(d, ref[t], r0, t)
+
+
+def test_batch_autotune():
+    """The autotuner times the 20 candidates of its grid on a sample and
+    returns the fastest; the batch solved under it matches the oracle
+    fixture's first 512 instances."""
+    import json
+    import os
+    insts = PR.maxcut_batch(4096)[:512]
+    best, table = B.autotune(insts, sample=128)
+    print("autotune best", best.as_dict())
+    print(table)
+    assert table.shape == (20, 4) and (table[:, 3] > 0).all()
+    assert best.threads in (128, 256) and best.heavy in (16, 32, 64, 128, 256)
+    assert table[:, 3].min() == pytest.approx(
+        table[(table[:, 0] == best.threads) & (table[:, 1] == best.heavy) &
+              (table[:, 2] == best.order), 3][0])
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "oracle_batch4096.json")))
+    r = B.solve_cfg(insts, cfg=best)
+    same = [t for t in range(len(insts)) if r.phase[t] == gold["phase"][t]]
+    assert len(same) >= len(insts) - 1
+    for t in same:
+        assert abs(int(r.iterations[t]) - gold["iterations"][t]) <= 1, t
+        assert r.primal_obj[t] == pytest.approx(gold["primal_obj"][t], rel=1e-6, abs=1e-8), t
+
+
+def test_batch4096_mixed_precision_matches_oracle_fixture():
+    """The 4096-instance workload with the FP32 Schur factorization and FP64
+    refinement against the oracle's FP64 solves (same criteria as the FP64
+    launch's fixture test)."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "oracle_batch4096.json")))
+    insts = PR.maxcut_batch(4096)
+    r = B.solve_cfg(insts, cfg=B.config(precision=1))
+    differ = []
+    for t in range(len(insts)):
+        if r.phase[t] != gold["phase"][t]:
+            differ.append((t, r.phase[t], gold["phase"][t]))
+            continue
+        assert abs(int(r.iterations[t]) - gold["iterations"][t]) <= 1, t
+        assert r.primal_obj[t] == pytest.approx(gold["primal_obj"][t], rel=1e-6, abs=1e-8), t
+        assert r.dual_obj[t] == pytest.approx(gold["dual_obj"][t], rel=1e-6, abs=1e-8), t
+        if r.phase[t] == "pdOPT":
+            assert r.relative_gap[t] <= 1e-7, t
+    st = r.stats
+    print(f"batch4096 mixed: {len(insts) - len(differ)} in the oracle's phase; differ {differ}; "
+          f"FP32 factorizations {st[:, 0].sum():.0f}, FP64 {st[:, 1].sum():.0f}, "
+          f"refinement steps {st[:, 2].sum():.0f}, stalls {st[:, 3].sum():.0f}; "
+          f"kernel {r.kernel_ms:.1f} ms")
+    assert len(differ) <= 4
+    assert all({a, b} == {"pdOPT", "pdFEAS"} for _, a, b in differ)
