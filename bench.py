@@ -24,7 +24,9 @@ median over the timed steps.  The JSON line also carries:
     oracle (OpenMP over the host threads), same stopping rule;
   - "mixed_vs_fp64": the same workload with the mixed-precision Schur path;
   - "configs": time-to-1e-7 of the other single-GPU workloads (MaxCut
-    n = 100, dense-data n = 500 / m = 2000, the 200-block problem);
+    n = 100, dense-data n = 500 / m = 2000, the 200-block problem, and a
+    problem whose Schur matrix takes the sparse branch: 100 blocks of 64 local
+    constraints plus 8 coupling ones, m = 6408);
   - "batched": 4096 small MaxCut SDPs in one persistent launch (solves/s,
     phase counts, gap histogram, its own CPU-oracle sample);
   - "lambda_star": the 200-block multi-block SDP (m = 5000) solved from the
@@ -74,6 +76,10 @@ CONFIGS = {
     "maxcut": ("maxcut_G100_p0.1_seed1", lambda P: P.maxcut(100, 0.1, 1)),
     "dense": ("dense_random_n500_m2000_seed3", lambda P: P.dense_random(500, 2000, 3)),
     "multiblock": ("multiblock_200x[5,50]_lp1000_m5000_seed5", lambda P: P.multi_block()),
+    # the sparse-Schur branch: 100 blocks of 64 local constraints and 8
+    # constraints touching every block (an arrow-shaped Schur pattern)
+    "sparse": ("block_arrow_100x12_local64_couple8_m6408_seed31",
+               lambda P: P.block_separated(n_blocks=100, per_block=64, couple=8, seed=31)),
 }
 
 
@@ -157,6 +163,16 @@ def other_config(key, steps):
                  "phase_ms_per_iteration": {k: v / mits for k, v in mpt.items()}}
     s.close()
     its = max(1, rt.iterations)
+    sparse = None
+    if key == "sparse":
+        g = sdpa_amd.GpuContext(0).load(d)
+        info = g.info
+        g.close()
+        sparse = {"schur_dense": bool(info[9]), "tile_factorization": bool(info[12]),
+                  "stored_tiles": int(info[13]), "tiles_per_side": int(info[14]),
+                  "levels": int(info[15]),
+                  "stored_fraction": float(info[13]) / max(1.0, info[14] * (info[14] + 1) / 2.0),
+                  "dense_factorization_flops": d.m ** 3 / 3.0}
     out = {"workload": name, "m": d.m, "time_to_1e7_s": val, "phase": r.phase,
            "iterations": r.iterations, "relative_gap": r.relative_gap,
            "primal_error": r.primal_error, "dual_error": r.dual_error,
@@ -188,6 +204,8 @@ def other_config(key, steps):
                 "frac_peak": fl / (ams * 1e-3) / 1e12 / peak, "peak": peak}
         g.close()
         out["bmat_f1_by_precision"] = asm
+    if sparse is not None:
+        out["sparse_branch"] = sparse
     del d
     return out
 
@@ -471,7 +489,7 @@ def main():
     configs = None
     if not args.no_configs:
         configs = {}
-        for key in ("maxcut", "dense", "multiblock"):
+        for key in ("maxcut", "dense", "multiblock", "sparse"):
             if key == args.config:
                 continue
             log(f"bench: config {key}")

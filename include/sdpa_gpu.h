@@ -45,9 +45,13 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* ctx, int m, int nBlock, const int* block
                           const double* c, long nnz, const int* k, const int* l, const int* i,
                           const int* j, const double* v, double kappa);
 
-/* out[0..11] = m, nBlock, nSDP, nLP, nDim, packed length, #F1, #F2, #F3,
+/* out[0..15] = m, nBlock, nSDP, nLP, nDim, packed length, #F1, #F2, #F3,
  * schur_dense (1 if the reference would choose the dense Schur path),
- * bmat_store_first, number of SDP blocks.  Returns the count written. */
+ * bmat_store_first, number of SDP blocks, schur_sparse (1 when the sparse
+ * branch is factored by tiles of the ordered matrix: off the dense path and
+ * cheaper than the dense factorization), and for it the stored 64 x 64
+ * tiles of the factor, the tiles per side and the levels of the tile
+ * elimination tree.  Returns the count written. */
 int sdpa_gpu_problem_info(sdpa_gpu_ctx* ctx, long* out, int cap);
 /* Size of SDP block q (0-based) or -1. */
 int sdpa_gpu_block_size(sdpa_gpu_ctx* ctx, int q);
@@ -81,15 +85,21 @@ int sdpa_gpu_make_bmat(sdpa_gpu_ctx* ctx, int precision);
  * Where Chordal::ordering_bMat would leave the dense path (problem_info
  * schur_dense = 0), bMat + 1e-10 I is factored instead, the matrix the
  * reference's sparse branch factors (Chordal::factorizeSchur,
- * sdpa_chordal.cpp:526-534); the shift is added to the stored bMat once per
- * assembled (or set) matrix.
+ * sdpa_chordal.cpp:526-534).  When that branch's analysis finds the factor of
+ * the ordered matrix cheaper than the dense one (problem_info schur_sparse =
+ * 1) the shifted matrix is factored by tiles of the ordered matrix (FP64 at
+ * either precision, LAPACK's pivot rule, bMat itself left as assembled;
+ * returns a 1-based failing column of the ordered matrix); otherwise the
+ * shift is added to the stored bMat once per assembled (or set) matrix and
+ * the dense factorization runs.
  * precision 0: FP64, in place.  precision 1 (mixed; no counterpart in the
  * reference): FP32 factor of the column-equilibrated bMat on FP32 MFMA, bMat
  * kept in FP64 for refinement; where that factorization fails or m <= 512 it
  * falls back to FP64 by itself.  Returns 0 or the 1-based failing column. */
 int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* ctx, int precision);
 /* Lal::solveSystems (sdpa_linear.cpp:843-862): dy = bMat^-1 gVec with the
- * factor left by sdpa_gpu_factor_bmat.  After a mixed factorization the
+ * factor left by sdpa_gpu_factor_bmat (the tile factor of the sparse branch:
+ * Chordal::solveSchur, sdpa_chordal.cpp:573-579).  After a mixed factorization the
  * solution is refined in FP64 until the residual gVec - bMat dy is at FP64
  * accuracy; returns 2 if the refinement stalls (dy is then not set: factor
  * again with precision 0 and solve again).  The FP64 solve is asynchronous:

@@ -174,8 +174,8 @@ class GpuContext:
         bs, c, k, l, i, j, v = self._keep
         self._chk(lib().sdpa_gpu_load_problem(self.h, d.m, len(bs), bs, c, len(k), k, l, i, j, v,
                                               kappa), "load_problem")
-        info = np.zeros(12, np.int64)
-        lib().sdpa_gpu_problem_info(self.h, info, 12)
+        info = np.zeros(16, np.int64)
+        lib().sdpa_gpu_problem_info(self.h, info, 16)
         self.info = info
         self.m = int(info[0])
         self.total = int(info[5])

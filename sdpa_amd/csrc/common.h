@@ -66,6 +66,59 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
 int potrf_mixed(hipStream_t s, DenseWork* w, const double* B, int ld, int m, float* A32,
                 double* scale, double* P64, double* D64, double* dinv, int* d_info, int* d_bad,
                 unsigned long long* d_bmax);
+// Sparse-Schur branch: a fill-reducing ordering of the Schur matrix and the
+// factorization of the permuted matrix by 64 x 64 tiles, only the tiles of
+// the factor's (filled) pattern stored and computed (sparse_schur.cpp for the
+// analysis, dense.hip for the kernels).
+struct TileSparse {
+  int m = 0, T = 0, ntiles = 0, nlev = 0;
+  std::vector<int> perm, iperm;   // position in the ordering -> constraint, inverse
+  // tiles of L column by column, the diagonal tile first: tile p is (tI[p], tJ[p])
+  std::vector<int> colptr, tI, tJ, tile_of;  // tile_of[I * T + J]: tile id or -1
+  // schedule: the tile columns by height in the tile elimination tree (a
+  // level's columns are independent); per level the off-diagonal tiles to
+  // solve and the update tasks, one per target tile with its source pairs
+  // (tile ids of L(I, J), L(K, J)) in column order: target t's pairs are
+  // upd_src[2 q], upd_src[2 q + 1] for upd_src_ptr[t] <= q < upd_src_ptr[t + 1]
+  std::vector<int> lvl_ptr, lvl_cols, trsm_ptr, trsm_tiles;
+  std::vector<int> upd_ptr, upd_target, upd_src_ptr, upd_src;
+  // a target with many sources: its chunks write partial sums to scratch
+  // slots (upd_slot >= 0, else -1: subtracted at once), then a reduction
+  // task per level subtracts slots red_slot0 .. + red_nslot in order
+  std::vector<int> upd_slot, red_ptr, red_target, red_slot0, red_nslot;
+  int max_slots = 0;
+  // forward solve: per level the tile rows I updated by its columns, each
+  // with its tiles (x_I -= sum L(I, J) x_J over fwd_src[fwd_src_ptr[r] ..])
+  std::vector<int> fwd_ptr, fwd_row, fwd_src_ptr, fwd_src;
+  double flops = 0.0, ext_nnz = 0.0;  // tile-level estimates
+  double* d_tiles = nullptr;  // ntiles x 4096 doubles (column-major tiles)
+  double* d_winv = nullptr;   // T x 4096: inverses of the diagonal tiles of L
+  double* d_x = nullptr;      // 64 T: solve workspace
+  double* d_u = nullptr;      // 64 ntiles: products L(I, J) x_J of the forward solve
+  int* d_info = nullptr;      // T: failing column within each diagonal tile (1-based) or 0
+  int *d_perm = nullptr, *d_tI = nullptr, *d_tJ = nullptr, *d_colptr = nullptr;
+  int *d_lvl_cols = nullptr, *d_trsm_tiles = nullptr, *d_upd_target = nullptr;
+  int *d_upd_src_ptr = nullptr, *d_upd_src = nullptr;
+  int *d_fwd_row = nullptr, *d_fwd_src_ptr = nullptr, *d_fwd_src = nullptr;
+  int *d_upd_slot = nullptr, *d_red_target = nullptr, *d_red_slot0 = nullptr,
+      *d_red_nslot = nullptr;
+  double* d_part = nullptr;  // max_slots x 4096: partial update sums
+};
+// Ordering, symbolic factorization and schedule from the cliques of the
+// pattern (constraint lists of the SDP blocks and LP coordinates).  Returns
+// 0, 1 when the analysis would exceed max_work word operations, 2 on an
+// inconsistent symbolic factorization.
+int sparse_schur_analyse(int m, const std::vector<std::vector<int>>& cliques, double max_work,
+                         TileSparse* ts);
+int sparse_schur_upload(TileSparse* ts);  // device copies of the tables and workspaces
+void sparse_schur_free(TileSparse* ts);
+// Factor P (B + shift I) P^T (B: dense m x m, lower triangle, ld m) by tiles
+// with LAPACK's pivot rule (a pivot that is not > 0 fails).  Returns 0, a
+// 1-based failing column of the permuted matrix, or < 0 on a device error.
+int sparse_schur_factor(hipStream_t s, TileSparse* ts, const double* B, double shift);
+// dy := (P^T L L^T P)^-1 g with the tile factor, level by level.
+int sparse_schur_solve(hipStream_t s, TileSparse* ts, const double* g, double* dy);
+
 // W = L^-1 for a batch of lower factors (upper parts of W zeroed).
 // dinv_pre (optional): the 64x64 diagonal inverses potrf_batched produced for
 // the same batch (slot (item, k0/64)); otherwise they are computed here.

@@ -71,6 +71,10 @@ struct Ctx {
   int nF[3] = {0, 0, 0};
   int schurDense = 1;
   bool bshifted = false;  // the sparse branch's diagonal shift is in d_B
+  // sparse branch factored by tiles of the ordered matrix (when its analysis
+  // finds the factor cheaper than the dense one by the reference's cost rule)
+  int schurSparse = 0;
+  TileSparse ts;
   bool bmatStoreFirst = false;  // single block covering all constraints, all F3
   // F3 of the SDP blocks of n <= kMultiN by one launch (several blocks,
   // m <= kF3MultiMaxM; larger blocks keep their per-block kernels): per

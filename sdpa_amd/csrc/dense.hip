@@ -2849,4 +2849,323 @@ int chol_solve_f32(hipStream_t s, DenseWork* w, const float* L, int ld, int m, c
   return chol_solve_ll<float>(s, w, L, ld, m, Winv, x);
 }
 
+// ---------------------------------------------------------------------------
+// Tile-sparse factorization of the sparse-Schur branch (the tables come from
+// sparse_schur_analyse).  The permuted, shifted Schur matrix is gathered into
+// the stored tiles of its factor's pattern (64 x 64, column-major; rows and
+// columns beyond m are the identity); then, level by level of the tile
+// elimination tree: every diagonal tile of the level is factored with its
+// inverse (the 64 x 64 routines of the dense factorization, LAPACK's pivot
+// rule), every off-diagonal tile of the level's columns solved,
+// L(I, J) = A(I, J) inv(L_JJ)^T, and every target tile of the level's
+// updates gets A(I, K) -= sum_J L(I, J) L(K, J)^T from one work-group, its
+// sources in column order.
+
+__global__ __launch_bounds__(256) void k_ts_gather(const double* __restrict__ B, int m,
+                                                   const int* __restrict__ perm,
+                                                   const int* __restrict__ tI,
+                                                   const int* __restrict__ tJ,
+                                                   double* __restrict__ tiles, double shift) {
+  const int p = blockIdx.x;
+  const int I = tI[p], J = tJ[p];
+  double* D = tiles + (size_t)p * TB * TB;
+  for (int e = threadIdx.x; e < TB * TB; e += 256) {
+    const int r = 64 * I + (e & 63), c = 64 * J + (e >> 6);
+    double v;
+    if (r < m && c < m) {
+      const int pr = perm[r], pc = perm[c];
+      v = B[(size_t)max(pr, pc) + (size_t)m * min(pr, pc)] + (r == c ? shift : 0.0);
+    } else {
+      v = (r == c) ? 1.0 : 0.0;
+    }
+    D[e] = v;
+  }
+}
+
+// the diagonal tiles of one level: factor (LAPACK's rule) and invert
+__global__ __launch_bounds__(256) void k_ts_potrf(double* __restrict__ tiles,
+                                                  const int* __restrict__ colptr,
+                                                  const int* __restrict__ cols,
+                                                  double* __restrict__ winv, int* __restrict__ info) {
+  const int J = cols[blockIdx.x];
+  const MatDesc M{tiles + (size_t)colptr[J] * TB * TB, TB, TB};
+  diag_block<0>(M, 0, 0, -1, info + J, winv + (size_t)J * TB * TB, 1);
+}
+
+// the off-diagonal tiles of one level's columns: L(I, J) = A(I, J) W_J^T, in
+// place; wave w takes rows 16 w .. 16 w + 15 (each wave reads only the rows
+// it writes)
+__global__ __launch_bounds__(256) void k_ts_trsm(double* __restrict__ tiles,
+                                                 const int* __restrict__ list,
+                                                 const int* __restrict__ tJ,
+                                                 const double* __restrict__ winv) {
+  const int p = list[blockIdx.x];
+  double* X = tiles + (size_t)p * TB * TB;
+  const double* Wt = winv + (size_t)tJ[p] * TB * TB;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int row = 16 * w + li;
+  double a[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) a[s] = X[row + TB * (4 * s + lk)];
+  v4d acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 16; ++s)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(Wt[(16 * c + li) + TB * (4 * s + lk)], a[s],
+                                                    acc[c], 0, 0, 0);
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) X[row + TB * (16 * c + 4 * v + lk)] = acc[c][v];
+}
+
+// the update tasks of one level: target tile -= sum over its source pairs of
+// L(I, J) L(K, J)^T, or (slot >= 0) that sum stored to the partial slot
+__global__ __launch_bounds__(256) void k_ts_update(double* __restrict__ tiles,
+                                                   const int* __restrict__ target,
+                                                   const int* __restrict__ src_ptr,
+                                                   const int* __restrict__ src,
+                                                   const int* __restrict__ slot,
+                                                   double* __restrict__ part) {
+  const int t = blockIdx.x;
+  double* Tg = tiles + (size_t)target[t] * TB * TB;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int row = 16 * w + li;
+  v4d acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int q = src_ptr[t]; q < src_ptr[t + 1]; ++q) {
+    const double* Li = tiles + (size_t)src[2 * q] * TB * TB;
+    const double* Lk = tiles + (size_t)src[2 * q + 1] * TB * TB;
+    double a[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) a[s] = Li[row + TB * (4 * s + lk)];
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lk[(16 * c + li) + TB * (4 * s + lk)], a[s],
+                                                      acc[c], 0, 0, 0);
+  }
+  if (slot[t] >= 0) {
+    double* Pt = part + (size_t)slot[t] * TB * TB;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Pt[row + TB * (16 * c + 4 * v + lk)] = acc[c][v];
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Tg[row + TB * (16 * c + 4 * v + lk)] -= acc[c][v];
+}
+
+// the targets split into chunks: target -= the chunks' partial sums, in order
+__global__ __launch_bounds__(256) void k_ts_reduce(double* __restrict__ tiles,
+                                                   const int* __restrict__ target,
+                                                   const int* __restrict__ slot0,
+                                                   const int* __restrict__ nslot,
+                                                   const double* __restrict__ part) {
+  const int r = blockIdx.x;
+  double* Tg = tiles + (size_t)target[r] * TB * TB;
+  const double* P0 = part + (size_t)slot0[r] * TB * TB;
+  for (int e = threadIdx.x; e < TB * TB; e += 256) {
+    double acc = 0.0;
+    for (int c = 0; c < nslot[r]; ++c) acc += P0[(size_t)c * TB * TB + e];
+    Tg[e] -= acc;
+  }
+}
+
+// dy := P^T (L L^T)^-1 P g by the tile factor, level by level of the tile
+// elimination tree.  Forward: the columns J of a level (one work-group each)
+// take x_J := W_J x_J (x_J has every update from the lower levels) and form
+// u_p = L_p x_J for their off-diagonal tiles p; then each tile row I the
+// level touches subtracts its u_p in column order.  Backward, levels in
+// reverse: each column J of a level takes x_J := W_J^T (x_J - sum_p L_p^T
+// x_{I(p)}) (the x_I of higher levels are final).  The 64 x 64 products: thread
+// t takes row t % 64 over a quarter of the columns, the quarters summed
+// through LDS in a fixed order.
+__device__ __forceinline__ double ts_matvec(const double* Mt, int trans, const double* xv,
+                                            double (*part)[TB]) {
+  const int t = threadIdx.x, i = t & 63, q = t >> 6;
+  double s = 0.0;
+#pragma unroll 4
+  for (int k = 16 * q; k < 16 * q + 16; ++k) s += (trans ? Mt[k + TB * i] : Mt[i + TB * k]) * xv[k];
+  part[q][i] = s;
+  __syncthreads();
+  const double v = part[0][i] + part[1][i] + part[2][i] + part[3][i];
+  __syncthreads();
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_ts_perm(const double* __restrict__ src,
+                                                 const int* __restrict__ perm, int m, int n,
+                                                 double* __restrict__ dst, int inverse) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  if (!inverse) dst[r] = r < m ? src[perm[r]] : 0.0;
+  else if (r < m) dst[perm[r]] = src[r];
+}
+
+__global__ __launch_bounds__(256) void k_ts_fwd_cols(const double* __restrict__ tiles,
+                                                     const double* __restrict__ winv,
+                                                     const int* __restrict__ colptr,
+                                                     const int* __restrict__ cols,
+                                                     double* __restrict__ x,
+                                                     double* __restrict__ u) {
+  __shared__ double part[4][TB];
+  __shared__ double xv[TB];
+  const int J = cols[blockIdx.x], t = threadIdx.x;
+  if (t < TB) xv[t] = x[64 * J + t];
+  __syncthreads();
+  const double y = ts_matvec(winv + (size_t)J * TB * TB, 0, xv, part);
+  if (t < TB) {
+    x[64 * J + t] = y;
+    xv[t] = y;
+  }
+  __syncthreads();
+  for (int p = colptr[J] + 1; p < colptr[J + 1]; ++p) {
+    const double v = ts_matvec(tiles + (size_t)p * TB * TB, 0, xv, part);
+    if (t < TB) u[(size_t)p * TB + t] = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_ts_fwd_rows(const int* __restrict__ rows,
+                                                    const int* __restrict__ src_ptr,
+                                                    const int* __restrict__ src,
+                                                    const double* __restrict__ u,
+                                                    double* __restrict__ x) {
+  const int r = blockIdx.x, t = threadIdx.x;
+  double acc = 0.0;
+  for (int q = src_ptr[r]; q < src_ptr[r + 1]; ++q) acc += u[(size_t)src[q] * TB + t];
+  x[64 * rows[r] + t] -= acc;
+}
+
+__global__ __launch_bounds__(256) void k_ts_bwd_cols(const double* __restrict__ tiles,
+                                                     const double* __restrict__ winv,
+                                                     const int* __restrict__ colptr,
+                                                     const int* __restrict__ tI,
+                                                     const int* __restrict__ cols,
+                                                     double* __restrict__ x) {
+  __shared__ double part[4][TB];
+  __shared__ double xv[TB];
+  const int J = cols[blockIdx.x], t = threadIdx.x;
+  double acc = 0.0;
+  for (int p = colptr[J] + 1; p < colptr[J + 1]; ++p) {
+    if (t < TB) xv[t] = x[64 * tI[p] + t];
+    __syncthreads();
+    acc += ts_matvec(tiles + (size_t)p * TB * TB, 1, xv, part);
+  }
+  if (t < TB) xv[t] = x[64 * J + t] - acc;
+  __syncthreads();
+  const double y = ts_matvec(winv + (size_t)J * TB * TB, 1, xv, part);
+  if (t < TB) x[64 * J + t] = y;
+}
+
+int sparse_schur_upload(TileSparse* ts) {
+  auto up = [](int** d, const std::vector<int>& h) -> int {
+    SDPA_HIP_CHECK(hipMalloc(d, sizeof(int) * (h.empty() ? 1 : h.size())));
+    if (!h.empty())
+      SDPA_HIP_CHECK(hipMemcpy(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (int e = up(&ts->d_perm, ts->perm)) return e;
+  if (int e = up(&ts->d_tI, ts->tI)) return e;
+  if (int e = up(&ts->d_tJ, ts->tJ)) return e;
+  if (int e = up(&ts->d_colptr, ts->colptr)) return e;
+  if (int e = up(&ts->d_lvl_cols, ts->lvl_cols)) return e;
+  if (int e = up(&ts->d_trsm_tiles, ts->trsm_tiles)) return e;
+  if (int e = up(&ts->d_upd_target, ts->upd_target)) return e;
+  if (int e = up(&ts->d_upd_src_ptr, ts->upd_src_ptr)) return e;
+  if (int e = up(&ts->d_upd_src, ts->upd_src)) return e;
+  if (int e = up(&ts->d_fwd_row, ts->fwd_row)) return e;
+  if (int e = up(&ts->d_upd_slot, ts->upd_slot)) return e;
+  if (int e = up(&ts->d_red_target, ts->red_target)) return e;
+  if (int e = up(&ts->d_red_slot0, ts->red_slot0)) return e;
+  if (int e = up(&ts->d_red_nslot, ts->red_nslot)) return e;
+  SDPA_HIP_CHECK(hipMalloc(&ts->d_part, sizeof(double) * TB * TB * (size_t)std::max(1, ts->max_slots)));
+  if (int e = up(&ts->d_fwd_src_ptr, ts->fwd_src_ptr)) return e;
+  if (int e = up(&ts->d_fwd_src, ts->fwd_src)) return e;
+  SDPA_HIP_CHECK(hipMalloc(&ts->d_u, sizeof(double) * TB * (size_t)ts->ntiles));
+  SDPA_HIP_CHECK(hipMalloc(&ts->d_tiles, sizeof(double) * TB * TB * (size_t)ts->ntiles));
+  SDPA_HIP_CHECK(hipMalloc(&ts->d_winv, sizeof(double) * TB * TB * (size_t)ts->T));
+  SDPA_HIP_CHECK(hipMalloc(&ts->d_x, sizeof(double) * TB * (size_t)ts->T));
+  SDPA_HIP_CHECK(hipMalloc(&ts->d_info, sizeof(int) * (size_t)ts->T));
+  return 0;
+}
+
+void sparse_schur_free(TileSparse* ts) {
+  for (void* p : {(void*)ts->d_perm, (void*)ts->d_tI, (void*)ts->d_tJ, (void*)ts->d_colptr,
+                  (void*)ts->d_lvl_cols, (void*)ts->d_trsm_tiles, (void*)ts->d_upd_target,
+                  (void*)ts->d_upd_src_ptr, (void*)ts->d_upd_src, (void*)ts->d_tiles,
+                  (void*)ts->d_winv, (void*)ts->d_x, (void*)ts->d_info, (void*)ts->d_u,
+                  (void*)ts->d_fwd_row, (void*)ts->d_fwd_src_ptr, (void*)ts->d_fwd_src,
+                  (void*)ts->d_upd_slot, (void*)ts->d_red_target, (void*)ts->d_red_slot0,
+                  (void*)ts->d_red_nslot, (void*)ts->d_part})
+    if (p) (void)hipFree(p);
+  *ts = TileSparse{};
+}
+
+int sparse_schur_factor(hipStream_t s, TileSparse* ts, const double* B, double shift) {
+  if (ts->ntiles <= 0) return 0;
+  SDPA_HIP_CHECK(hipMemsetAsync(ts->d_info, 0, sizeof(int) * (size_t)ts->T, s));
+  hipLaunchKernelGGL(k_ts_gather, dim3(ts->ntiles), dim3(256), 0, s, B, ts->m, ts->d_perm,
+                     ts->d_tI, ts->d_tJ, ts->d_tiles, shift);
+  for (int l = 0; l < ts->nlev; ++l) {
+    const int c0 = ts->lvl_ptr[l], nc = ts->lvl_ptr[l + 1] - c0;
+    hipLaunchKernelGGL(k_ts_potrf, dim3(nc), dim3(256), 0, s, ts->d_tiles, ts->d_colptr,
+                       ts->d_lvl_cols + c0, ts->d_winv, ts->d_info);
+    const int r0 = ts->trsm_ptr[l], nr = ts->trsm_ptr[l + 1] - r0;
+    if (nr > 0)
+      hipLaunchKernelGGL(k_ts_trsm, dim3(nr), dim3(256), 0, s, ts->d_tiles, ts->d_trsm_tiles + r0,
+                         ts->d_tJ, ts->d_winv);
+    const int u0 = ts->upd_ptr[l], nu = ts->upd_ptr[l + 1] - u0;
+    if (nu > 0)
+      hipLaunchKernelGGL(k_ts_update, dim3(nu), dim3(256), 0, s, ts->d_tiles,
+                         ts->d_upd_target + u0, ts->d_upd_src_ptr + u0, ts->d_upd_src,
+                         ts->d_upd_slot + u0, ts->d_part);
+    const int q0 = ts->red_ptr[l], nq = ts->red_ptr[l + 1] - q0;
+    if (nq > 0)
+      hipLaunchKernelGGL(k_ts_reduce, dim3(nq), dim3(256), 0, s, ts->d_tiles,
+                         ts->d_red_target + q0, ts->d_red_slot0 + q0, ts->d_red_nslot + q0,
+                         ts->d_part);
+  }
+  SDPA_HIP_CHECK(hipGetLastError());
+  std::vector<int> info(ts->T);
+  SDPA_HIP_CHECK(hipMemcpyAsync(info.data(), ts->d_info, sizeof(int) * (size_t)ts->T,
+                                hipMemcpyDeviceToHost, s));
+  SDPA_HIP_CHECK(hipStreamSynchronize(s));
+  for (int J = 0; J < ts->T; ++J)
+    if (info[J]) return 64 * J + info[J];
+  return 0;
+}
+
+int sparse_schur_solve(hipStream_t s, TileSparse* ts, const double* g, double* dy) {
+  const int n = 64 * ts->T;
+  hipLaunchKernelGGL(k_ts_perm, dim3((n + 255) / 256), dim3(256), 0, s, g, ts->d_perm, ts->m, n,
+                     ts->d_x, 0);
+  for (int l = 0; l < ts->nlev; ++l) {
+    const int c0 = ts->lvl_ptr[l], nc = ts->lvl_ptr[l + 1] - c0;
+    hipLaunchKernelGGL(k_ts_fwd_cols, dim3(nc), dim3(256), 0, s, ts->d_tiles, ts->d_winv,
+                       ts->d_colptr, ts->d_lvl_cols + c0, ts->d_x, ts->d_u);
+    const int r0 = ts->fwd_ptr[l], nr = ts->fwd_ptr[l + 1] - r0;
+    if (nr > 0)
+      hipLaunchKernelGGL(k_ts_fwd_rows, dim3(nr), dim3(64), 0, s, ts->d_fwd_row + r0,
+                         ts->d_fwd_src_ptr + r0, ts->d_fwd_src, ts->d_u, ts->d_x);
+  }
+  for (int l = ts->nlev - 1; l >= 0; --l) {
+    const int c0 = ts->lvl_ptr[l], nc = ts->lvl_ptr[l + 1] - c0;
+    hipLaunchKernelGGL(k_ts_bwd_cols, dim3(nc), dim3(256), 0, s, ts->d_tiles, ts->d_winv,
+                       ts->d_colptr, ts->d_tI, ts->d_lvl_cols + c0, ts->d_x);
+  }
+  hipLaunchKernelGGL(k_ts_perm, dim3((n + 255) / 256), dim3(256), 0, s, ts->d_x, ts->d_perm,
+                     ts->m, n, dy, 1);
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 }  // namespace sdpa_amd

@@ -103,6 +103,8 @@ struct Arena {
 };
 
 void free_problem(Ctx* c) {
+  sparse_schur_free(&c->ts);
+  c->schurSparse = 0;
   for (auto& T : c->blk) {
     if (T.d_dense) (void)hipFree(T.d_dense);
     if (T.d_dense32) (void)hipFree(T.d_dense32);
@@ -623,6 +625,28 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       }
       if ((double)(2 * lower - m) > 0.70 * (double)m * (double)m) c->schurDense = 1;
     }
+    if (!c->schurDense) {
+      // the sparse branch: ordering and tile analysis (sparse_schur.cpp);
+      // the tile factorization is taken when its flops (x 1.15) stay below
+      // 0.85 of the dense m^3 / 3, the reference's choice between its sparse
+      // solver and the dense one (sdpa_chordal.cpp:497-514); otherwise the
+      // dense factorization runs on the same shifted matrix
+      std::vector<std::vector<int>> cliques;
+      for (int q = 0; q < nS; ++q) cliques.push_back(c->blk[q].h_con);
+      for (int t = 0; t < c->nLP; ++t) {
+        std::vector<int> g;
+        for (auto& pr : byc[t]) g.push_back(pr.first);
+        cliques.push_back(std::move(g));
+      }
+      const int ar = sparse_schur_analyse(m, cliques, 4.0e9, &c->ts);
+      const double dense_cost = (double)m * m * m / 3.0;
+      if (ar == 0 && c->ts.flops * 1.15 <= 0.85 * dense_cost) {
+        if (int e = sparse_schur_upload(&c->ts)) return e;
+        c->schurSparse = 1;
+      } else {
+        sparse_schur_free(&c->ts);
+      }
+    }
   }
   c->bmatStoreFirst = (nS == 1 && c->nLP == 0 && c->blk[0].ncon == m && c->blk[0].nF1 == 0);
   c->f3multi = 0;
@@ -703,9 +727,11 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
 int sdpa_gpu_problem_info(sdpa_gpu_ctx* h, long* out, int cap) {
   if (!h || !h->c.loaded) return -1;
   Ctx* c = &h->c;
-  long v[12] = {c->m, c->nBlock, c->nSDP, c->nLP, c->nDim, c->total, c->nF[0], c->nF[1],
-                c->nF[2], c->schurDense, c->bmatStoreFirst ? 1 : 0, c->nSDP};
-  int n = cap < 12 ? cap : 12;
+  long v[16] = {c->m, c->nBlock, c->nSDP, c->nLP, c->nDim, c->total, c->nF[0], c->nF[1],
+                c->nF[2], c->schurDense, c->bmatStoreFirst ? 1 : 0, c->nSDP, c->schurSparse,
+                c->schurSparse ? c->ts.ntiles : 0, c->schurSparse ? c->ts.T : 0,
+                c->schurSparse ? c->ts.nlev : 0};
+  int n = cap < 16 ? cap : 16;
   for (int t = 0; t < n; ++t) out[t] = v[t];
   return n;
 }
@@ -855,6 +881,15 @@ int sdpa_gpu_factor_bmat(sdpa_gpu_ctx* h, int precision) {
   if (precision != 0 && precision != 1) return fail(c, -30, "precision must be 0 (FP64) or 1 (mixed)");
   tic(c);
   c->factor_mixed = 0;
+  if (c->schurSparse) {
+    // Chordal::factorizeSchur (sdpa_chordal.cpp:526-571): bMat + 1e-10 I
+    // factored by the sparse branch (FP64 at either precision); the shift is
+    // added as the tiles are gathered, d_B keeps the assembled matrix
+    const int r = sparse_schur_factor(c->stream, &c->ts, c->d_B, 1.0e-10);
+    c->t_chol += toc(c);
+    c->n_chol++;
+    return r;
+  }
   if (!c->schurDense && !c->bshifted) {
     // Chordal::factorizeSchur (sdpa_chordal.cpp:526-534): where the reference
     // leaves the dense path it factors bMat + 1e-10 I (PLUS_ADJUST_DIAGONAL)
@@ -897,6 +932,13 @@ int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* h, int refine) {
   tic(c);
   if (c->factor_mixed) {
     const int r = solve_mixed(c);
+    c->t_solve += toc(c);
+    c->n_solve++;
+    return r;
+  }
+  if (c->schurSparse) {
+    // Chordal::solveSchur (sdpa_chordal.cpp:573-579)
+    const int r = sparse_schur_solve(c->stream, &c->ts, c->d_g, c->d_dy);
     c->t_solve += toc(c);
     c->n_solve++;
     return r;

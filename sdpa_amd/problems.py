@@ -208,20 +208,22 @@ def multi_block(n_sdp_blocks: int = 200, nmin: int = 5, nmax: int = 50, lp: int 
 
 
 def block_separated(n_blocks: int = 2, n: int = 12, per_block: int = 60, nnz: int = 3,
-                    seed: int = 14) -> SdpData:
+                    seed: int = 14, couple: int = 0) -> SdpData:
     """SDP blocks whose constraints never share a block: constraint k has nnz
     N(0,1) entries in block k // per_block only, so the aggregate Schur
     pattern is block diagonal (the shape for which SDPA's ordering_bMat
-    leaves the dense path once m > 100).  Feasible like dense_random."""
+    leaves the dense path once m > 100).  couple > 0 appends that many
+    constraints with one diagonal entry in every block (an arrow pattern).
+    Feasible like dense_random."""
     rng = np.random.default_rng(seed)
-    m = n_blocks * per_block
+    m = n_blocks * per_block + couple
     X0 = [(_q := _rand_orth(n, rng)) * rng.uniform(1, 10, n) @ _q.T for _ in range(n_blocks)]
     S0 = [(_q := _rand_orth(n, rng)) * rng.uniform(1, 10, n) @ _q.T for _ in range(n_blocks)]
     y0 = rng.standard_normal(m)
     b = np.zeros(m)
     C = [S.copy() for S in S0]
     K, L, I, J, V = [], [], [], [], []
-    for k in range(m):
+    for k in range(n_blocks * per_block):
         bl = k // per_block
         ent = {}
         while len(ent) < nnz:
@@ -234,12 +236,20 @@ def block_separated(n_blocks: int = 2, n: int = 12, per_block: int = 60, nnz: in
             if a != c:
                 C[bl][c, a] += y0[k] * v
             K.append(k + 1); L.append(bl + 1); I.append(a + 1); J.append(c + 1); V.append(v)
+    for t in range(couple):
+        k = n_blocks * per_block + t
+        for bl in range(n_blocks):
+            a = int(rng.integers(0, n))
+            v = 1.0 + rng.uniform()
+            b[k] += v * X0[bl][a, a]
+            C[bl][a, a] += y0[k] * v
+            K.append(k + 1); L.append(bl + 1); I.append(a + 1); J.append(a + 1); V.append(v)
     for bl in range(n_blocks):
         iu, ju = np.triu_indices(n)
         for a, c in zip(iu, ju):
             K.append(0); L.append(bl + 1); I.append(a + 1); J.append(c + 1); V.append(-C[bl][a, c])
     return _make(m, [n] * n_blocks, b, K, L, I, J, V,
-                 f"block_separated_{n_blocks}x{n}_m{m}_s{seed}")
+                 f"block_separated_{n_blocks}x{n}_m{m}_s{seed}" + (f"_c{couple}" if couple else ""))
 
 
 def maxcut_batch(count: int = 4096, nmin: int = 8, nmax: int = 64, p: float = 0.2, seed: int = 4):

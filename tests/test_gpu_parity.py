@@ -350,3 +350,57 @@ def test_sparse_schur_decision_shift(per_block, dense):
     assert abs(r.iterations - o.iterations) <= 1
     assert r.primal_obj == pytest.approx(o.primal_obj, rel=1e-6, abs=1e-8)
     assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)
+
+
+@pytest.mark.parametrize("per_block,couple", [(64, 0), (64, 3), (40, 2)])
+def test_sparse_branch_tile_factorization(per_block, couple):
+    """The sparse-Schur branch factored by tiles of the ordered matrix: 20
+    blocks of 64 local constraints (block-diagonal Schur pattern) or 30 of 40
+    (tiles straddle blocks: fill, several levels), plus `couple` constraints
+    touching every block (an arrow).  The loader leaves
+    the dense path like the oracle (the aggregate-pattern rule) and takes the
+    tile factorization (its flops far below the dense m^3 / 3).  The factor
+    and solve of bMat + 1e-10 I at two iterates: dy against numpy's solve of
+    the same shifted matrix (B from the device, its lower triangle), relative
+    error <= 1e-9; an indefinite matrix of the same pattern fails the
+    factorization; the whole solve against the oracle's (which factors the
+    same shifted matrix densely)."""
+    d = PR.block_separated(n_blocks=20 if per_block == 64 else 30, per_block=per_block,
+                           couple=couple, seed=31)
+    g = ctx_for(d)
+    m = d.m
+    assert int(g.info[9]) == 0 and not O.Problem(d).schur_dense
+    print(d.name, "tile factorization", int(g.info[12]), "tiles", int(g.info[13]), "per side",
+          int(g.info[14]), "levels", int(g.info[15]))
+    assert int(g.info[12]) == 1
+    assert int(g.info[13]) < int(g.info[14]) * (int(g.info[14]) + 1) // 2  # not all tiles
+    rng = np.random.default_rng(couple)
+    for lam in (10.0, 0.3):
+        g.identity_start(lam)
+        g.make_bmat()
+        B = g.bmat()
+        Bs = np.tril(B) + np.tril(B, -1).T
+        x = rng.standard_normal(m)
+        g.set("g", (Bs + 1e-10 * np.eye(m)) @ x)
+        assert g.factor_bmat() == 0
+        g.solve_bmat()
+        dy = g.get("dy")
+        want = np.linalg.solve(Bs + 1e-10 * np.eye(m), (Bs + 1e-10 * np.eye(m)) @ x)
+        rel = np.linalg.norm(dy - want) / np.linalg.norm(want)
+        print(f"lambda={lam}: solve relative error {rel:.2e}")
+        assert rel <= 1e-9
+        assert np.array_equal(g.bmat(), B)  # the assembled matrix is left as it was
+    # an indefinite matrix with the same pattern
+    Bn = Bs.copy()
+    Bn[np.diag_indices(m)] *= -1.0
+    g.set("B", Bn.reshape(-1, order="F"))
+    assert g.factor_bmat() > 0
+    g.close()
+    r = sa.Solver(0).load(d).solve()
+    o = O.Problem(d).solve()
+    print(d.name, r.phase, r.iterations, o.iterations, r.primal_obj, o.primal_obj)
+    assert r.phase == o.phase == "pdOPT"
+    assert abs(r.iterations - o.iterations) <= 1
+    assert r.primal_obj == pytest.approx(o.primal_obj, rel=1e-6, abs=1e-8)
+    assert r.dual_obj == pytest.approx(o.dual_obj, rel=1e-6, abs=1e-8)
+    assert r.relative_gap <= 1e-7
