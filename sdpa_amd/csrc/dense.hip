@@ -1591,8 +1591,8 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
 // triangle) is left intact for the refinement residuals.  A32 = S B S in
 // FP32, S = diag(B_ii^-1/2) (column equilibration: unit diagonal), is
 // factored in place by 256-column panels: each panel is widened to FP64 and
-// factored by the FP64 panel kernels (diagonal blocks, pivot rule, 64x64
-// inverses), written back as FP32, and applied to the trailing matrix by an
+// factored by the single-launch FP64 panel kernel (diagonal blocks, pivot
+// rule, 64x64 inverses), written back as FP32, and applied to the trailing matrix by an
 // FP32-MFMA update.  The 256x256 diagonal blocks of the FP64 panel factors
 // are kept (D64) for the block inverses of the solves.
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -1806,15 +1806,9 @@ int potrf_mixed(hipStream_t s, DenseWork* w, const double* B, int ld, int m, flo
   hipLaunchKernelGGL(k_eq_scale, dim3((m + 255) / 256), dim3(256), 0, s, B, ld, m, scale, d_bad,
                      d_bmax);
   hipLaunchKernelGGL(k_to_f32, dim3(nt, nt), dim3(256), 0, s, B, ld, m, scale, A32, m);
-  // one descriptor per panel: the FP64 panel buffer P64 (m x PW, ld m) seen
-  // as columns [K0, K0 + PW) of an m x m matrix, so the panel kernels index it
-  // with the global row and column numbers
-  if (int e = w->ensure_mats(np)) return e;
-  std::vector<MatDesc> pd(np);
-  for (int p = 0; p < np; ++p)
-    pd[p] = MatDesc{P64 - (ptrdiff_t)m * (p * PW), m, m};
-  SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats, pd.data(), sizeof(MatDesc) * np, hipMemcpyHostToDevice, s));
-  const int nslot = (m + TB - 1) / TB;
+  // the FP64 panel buffer P64 (m x PW, ld m) is seen as columns [K0, K0 + PW)
+  // of an m x m matrix (base P64 - m K0), so the panel kernel indexes it with
+  // the global row and column numbers
   // Look-ahead as in potrf_batched: the panel chain (widen, factor, narrow)
   // and the FP32 update of the next panel's columns on `s`, the rest of each
   // trailing update on the CU-masked side stream.  The next panel's update
@@ -1829,7 +1823,10 @@ int potrf_mixed(hipStream_t s, DenseWork* w, const double* B, int ld, int m, flo
     const int K0 = p * PW, pw = std::min(PW, m - K0);
     const dim3 grid((m - K0 + 63) / 64, (pw + 63) / 64);
     if (p == 0) hipLaunchKernelGGL(k_panel_to64, grid, dim3(256), 0, s, A32, m, m, K0, pw, P64, m);
-    factor_panel(s, w->d_mats + p, 1, m, K0, PW, 1, d_info, dinv, nslot);
+    // the widened panel by the single-launch panel kernel (its columns are
+    // already updated: no prologue)
+    if (int e = factor_panel_rows(s, w, P64 - (ptrdiff_t)m * K0, m, m, K0, -1, 1, d_info, dinv))
+      return e;
     hipLaunchKernelGGL(k_panel_to32, grid, dim3(256), 0, s, P64, m, m, K0, pw, A32, m,
                        D64 + (size_t)p * PW * PW);
     if (K0 + PW >= m) break;
