@@ -42,17 +42,12 @@ struct BlockTable {
   std::vector<int> f1rlo;
 };
 
-// one SDP block's F3 lists, for the kernel that takes all blocks at once
-struct F3Blk {
-  const int* con;
-  const int* estart;
-  const int* ecnt;
-  const int* nzeff;
-  const int* form;
-  const int4* rmeta;   // BlockTable::d_rmeta (n <= 256), else null
-  const double* rval;
+// one SDP block's entry records for the kernel that takes the light F3
+// pairs of all blocks of n <= kMultiN at once (64 per chunk)
+struct F3MBlk {
+  const int4* rec;
   long off;  // packed offset of the block in X, Z^-1
-  int n, ncon;
+  int n, nchunk;
 };
 
 struct Ctx {
@@ -78,18 +73,20 @@ struct Ctx {
   bool bmatStoreFirst = false;  // single block covering all constraints, all F3
   // F3 of the SDP blocks of n <= kMultiN by one launch (several blocks,
   // m <= kF3MultiMaxM; larger blocks keep their per-block kernels): per
-  // constraint j the (block, position in the block's list) pairs, in block
-  // order
+  // constraint j its visits, one per block where A_j has at most kLight
+  // entries, in block order
   int f3multi = 0;
-  F3Blk* d_f3blk = nullptr;
+  int f3m_pmax = 0, f3m_nmax = 0;  // largest n (n + 1) / 2 and n of those blocks
+  F3MBlk* d_f3blk = nullptr;
   int* d_jptr = nullptr;  // m + 1
-  // per jent record: the first four entries of A_j in the block, a | b << 8
-  // each (int4), and their values (double4)
+  // per visit: q | min(ecnt, 15) << 20 | (form == 3) << 24, the position of
+  // j's first entry record in the block's records, nzeff, estart (int4);
+  // the first four entries of A_j in the block, a | b << 8 each (int4), and
+  // their values (double4)
+  int* d_jent = nullptr;
   int* d_jab = nullptr;
   double* d_jv = nullptr;
-  // per (block q, constraint j) one record: q | min(ecnt, 15) << 20 | form << 24,
-  // j's position in the block's list, estart, nzeff
-  int* d_jent = nullptr;
+  int* d_mrec = nullptr;  // every block's entry records (int4 each)
   std::vector<BlockTable> blk;
   // one allocation holding every block's small tables (d_con ... d_dlist,
   // d_rmeta, d_rval: the BlockTable pointers point into it; d_dense does not)
@@ -175,7 +172,8 @@ struct Ctx {
 constexpr int kF3MultiMaxM = 8192;  // the all-blocks F3 kernel keeps a Schur column in LDS
 constexpr int kMultiN = 64;         // ... and takes the blocks up to this size
 int make_bmat(Ctx* c, int precision);  // 1: F1/F2 products in FP32
-int f3multi_prepare(int m);  // LDS size of that kernel for m constraints
+int f3multi_prepare(int m, int pmax, int nmax);  // LDS size of that kernel
+size_t f3multi_lds(int m, int pmax, int nmax, int* soff);  // its LDS in doubles
 // ops.hip
 enum DotCode {
   DOT_CX = 0, DOT_BY, DOT_CDX, DOT_BDY, DOT_XDZ, DOT_DXZ, DOT_DXDZ, DOT_XZ,

@@ -64,227 +64,200 @@ __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int*
   return ret;
 }
 
-// k_f3_multi stages the rows of A_j's entries in LDS (blocks of n <= kMultiN)
-// when A_j has at most kMultiE entries in the block
-constexpr int kMultiE = 4;
-constexpr int kMultiT = 512;  // threads per work-group (one Schur column)
+// k_f3_multi: the light F3 pairs of every SDP block of n <= kMultiN in one
+// launch, one work-group per Schur column j, the column kept in LDS (acc).
+//
+// With c = v (a != b) or v / 2 (a == b) for an entry (a, b) of value v, the
+// value calF3 (sdpa_newton.cpp:1319-1403) gives the entry pair (a, b), (g, d)
+// is c c' (X[a][g] Z[b][d] + X[a][d] Z[b][g] + X[b][g] Z[a][d] + X[b][d] Z[a][g])
+// with c' the partner's (f3_pair's four terms, its two-term forms being the
+// four-term one halved for a == b and for g == d; Z here is Z^-1).  It is
+// symmetric in (a, b) and in (g, d).  A visit of column j to a block
+// therefore first forms, for every position p = (lo, hi), lo <= hi, of the
+// block's upper triangle,
+//   R[p] = s_p sum_e (U1_e[lo] W1_e[hi] + U1_e[hi] W1_e[lo] + U2_e[lo] W2_e[hi] + U2_e[hi] W2_e[lo])
+// over A_j's entries e = (a, b) in the block, U1 = c X[a][.], W1 = Z[b][.],
+// U2 = c X[b][.], W2 = Z[a][.] (rows staged in LDS), s_p = 1 (lo != hi) or
+// 1/2; then each partner i >= j of the block's list adds sum over its
+// entries of v' R[p'] to acc[i]: one LDS read per entry pair.  The pair is
+// evaluated here when its owner (more NonZeroEffect, ties to the smaller
+// index, sdpa_newton.cpp:2079-2081) takes F3 and both lists are light; heavy
+// lists are neither visited nor listed (k_f3_heavy*).
+//
+// Entry records (16 bytes): meta = i | p << 13 | (entries of i after this
+// one) << 25 | head << 28 | (i takes F3) << 29, i's NonZeroEffect, v'.  A
+// block's records come in chunks of 64, one per lane of a wave, and no
+// constraint's entries straddle two chunks: a wave adds a constraint's
+// entries by a segmented sum over at most kLight lanes and the head lane
+// updates acc, so each row takes one update per visit, in block order.
+//
+// Visits are software-pipelined with one barrier each: while the pairs of
+// visit u are summed against R[u & 1], R of visit u + 1 is formed from the
+// rows staged in S[(u + 1) & 1], the rows of visit u + 2 are loaded and then
+// stored to S[u & 1], and the first record chunks of visit u + 1 and the
+// descriptor of visit u + 3 are loaded.
+constexpr int kMultiE = 4;   // entries of A_j staged (more: read from X, Z^-1 directly)
+constexpr int kMultiT = 512; // threads per work-group (one Schur column)
+constexpr int kMultiR = 4;   // record chunks per wave loaded one visit ahead
 
-// One term of calF3 (f3_pair's plu) for the owner entry (a, b) and the
-// partner entry (g, d), given the eight values it reads: xag = X[a][g],
-// xad = X[a][d], xbg = X[b][g], xbd = X[b][d], and the same of Z^-1.
-__device__ __forceinline__ double f3_plu(bool ab, bool gd, double xag, double xad, double xbg,
-                                         double xbd, double zag, double zad, double zbg,
-                                         double zbd) {
-  if (ab) return gd ? zbd * xag + zbg * xad + zad * xbg + zag * xbd : zbd * xag + zad * xbg;
-  return gd ? zbd * xag + zbg * xad : zbd * xag;
+struct MVisit {
+  int4 hd;    // q | cj << 20 | (j takes F3) << 24, j's first record, nzeff, estart
+  int4 ab;    // a | b << 8 of A_j's first kMultiE entries in the block
+  double4 v;  // their values
+  F3MBlk blk;
+};
+
+// row of the upper-triangle position p = hi (hi + 1) / 2 + lo
+__device__ __forceinline__ int tri_hi(int p) {
+  int h = (int)((sqrtf((float)(8 * p + 1)) - 1.0f) * 0.5f);
+  if (h * (h + 1) / 2 > p) --h;
+  if ((h + 1) * (h + 2) / 2 <= p) ++h;
+  return h;
 }
 
-// F3 of several SDP blocks in one launch: one work-group per Schur column j
-// (constraint j), the column kept in LDS (acc).  For each block that holds
-// data of A_j, in block order: the rows a, b of X and Z^-1 of each entry
-// (a, b) of A_j in the block are staged in LDS (blocks of n <= kMultiN), then
-// the threads walk the block's constraint list from j's position on (rows
-// i >= j: the lists are in increasing constraint order), one 16-byte record
-// per row, and add each light pair's value to acc[i].  The value follows
-// f3_pair: the same owner rule, loops and order of terms; every X and Z^-1
-// entry it reads comes from a staged row -- directly when j owns the pair,
-// through the symmetry X[u][v] = X[v][u] when i does.  Then the column's
-// lower part is stored once.  Against one launch per block this writes each
-// entry of B once, needs no zero-fill, and reads X and Z^-1 from LDS.  F1/F2
-// and heavy-pair terms are added afterwards.
-// The next block's record, descriptor and A_j's entries there (jab: a | b << 8
-// of the first kMultiE, jv their values) are loaded one pass ahead; a pass
-// issues its first round of row records and the loads of its staged rows
-// together, before its first barrier, so that it waits for one round trip.
-// Dynamic LDS: m doubles (acc) + kMultiE * 4 * kMultiN doubles (rows).
-__global__ __launch_bounds__(kMultiT) void k_f3_multi(const F3Blk* __restrict__ blks,
-                                                  const int* __restrict__ jptr,
-                                                  const int* __restrict__ jent,
-                                                  const int4* __restrict__ jab,
-                                                  const double4* __restrict__ jv,
-                                                  const int* __restrict__ er,
-                                                  const int* __restrict__ ec,
-                                                  const double* __restrict__ ev,
-                                                  const double* __restrict__ X,
-                                                  const double* __restrict__ Zi,
-                                                  double* __restrict__ B, int m) {
+__device__ __forceinline__ int sel4(const int4& q, int e) {
+  return e == 0 ? q.x : e == 1 ? q.y : e == 2 ? q.z : q.w;
+}
+__device__ __forceinline__ double sel4(const double4& q, int e) {
+  return e == 0 ? q.x : e == 1 ? q.y : e == 2 ? q.z : q.w;
+}
+
+__global__ __launch_bounds__(kMultiT) void k_f3_multi(
+    const F3MBlk* __restrict__ blks, const int* __restrict__ jptr, const int4* __restrict__ jent,
+    const int4* __restrict__ jab, const double4* __restrict__ jv, const int* __restrict__ er,
+    const int* __restrict__ ec, const double* __restrict__ ev, const double* __restrict__ X,
+    const double* __restrict__ Zi, double* __restrict__ B, int m, int pmax, int nmax,
+    int soff) {
   extern __shared__ double acc[];
-  double* sR = acc + m;  // [e][4][kMultiN]: X row a_e, X row b_e, Z row a_e, Z row b_e
-  __shared__ int sja[kMultiE], sjb[kMultiE];
-  __shared__ double sjv[kMultiE];
-  const int j = blockIdx.x, tid = threadIdx.x;
+  double* R = acc + m;     // [2][pmax]
+  double* S = acc + soff;  // [2][kMultiE][nmax] x {U1, W1, U2, W2}
+  const int sstride = kMultiE * nmax * 4;
+  const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int r = j + tid; r < m; r += kMultiT) acc[r] = 0.0;
-  const int t0 = jptr[j], t1 = jptr[j + 1];
-  int4 jr = t0 < t1 ? reinterpret_cast<const int4*>(jent)[t0] : make_int4(0, 0, 0, 0);
-  int4 jq = t0 < t1 ? jab[t0] : make_int4(0, 0, 0, 0);
-  double4 jw = t0 < t1 ? jv[t0] : make_double4(0.0, 0.0, 0.0, 0.0);
-  F3Blk Bq = blks[jr.x & 0xfffff];
-  // a thread's share of the staged rows: at most kMultiE * 2 * kMultiN
-  // elements of X and of Z^-1 over the work-group
-  constexpr int kStg = (kMultiE * 2 * kMultiN + kMultiT - 1) / kMultiT;
-  for (int t = t0; t < t1; ++t) {
-    const int pj = jr.y, sj = jr.z, nzj = jr.w, cj = (jr.x >> 20) & 15, fj = (jr.x >> 24) & 3;
-    // entry e of A_j in the block (selects: no local arrays)
-    const int4 abq = jq;
-    const double4 jwq = jw;
-    auto ab_of = [&](int e) { return e == 0 ? abq.x : e == 1 ? abq.y : e == 2 ? abq.z : abq.w; };
-    auto v_of = [&](int e) { return e == 0 ? jwq.x : e == 1 ? jwq.y : e == 2 ? jwq.z : jwq.w; };
-    const F3Blk Bc = Bq;
-    if (t + 1 < t1) {
-      jr = reinterpret_cast<const int4*>(jent)[t + 1];
-      jq = jab[t + 1];
-      jw = jv[t + 1];
-      Bq = blks[jr.x & 0xfffff];
-    }
-    if (cj > kLight) continue;  // every pair of this block is heavy (k_f3_heavy)
-    const int n = Bc.n;
-    const double* Xq = X + Bc.off;
-    const double* Zq = Zi + Bc.off;
-    const bool staged = n <= kMultiN && cj <= kMultiE && Bc.rmeta != nullptr;
-    const int ncon = Bc.ncon;
-    // issued together: the first round of row records (one per thread) and
-    // the staged rows
-    constexpr int kR0 = 1;
-    int4 rq0[kR0];
-    double vq0[kR0];
-    double sx[kStg], sz[kStg];
-    if (staged) {
+  const int t0 = jptr[j], nv = jptr[j + 1] - t0;
+  auto load_visit = [&](int u, MVisit& V) {
+    V.hd = jent[t0 + u];
+    V.ab = jab[t0 + u];
+    V.v = jv[t0 + u];
+    V.blk = blks[V.hd.x & 0xfffff];
+  };
+  MVisit D0, D1, D2, D3;
+  int4 rc0[kMultiR], rc1[kMultiR];
 #pragma unroll
-      for (int u = 0; u < kR0; ++u) {
-        const int kl = min(pj + tid + kMultiT * u, ncon - 1);
-        rq0[u] = Bc.rmeta[kl];
-        vq0[u] = Bc.rval[kl];
-      }
-#pragma unroll
-      for (int q = 0; q < kStg; ++q) {
-        const int x = tid + kMultiT * q;
-        const int e = x / (2 * n), h = (x / n) & 1, g = x % n;  // h: row a or row b
-        const int row = e < cj ? (ab_of(e) >> (8 * h)) & 255 : 0;
-        sx[q] = e < cj ? Xq[(size_t)n * row + g] : 0.0;
-        sz[q] = e < cj ? Zq[(size_t)n * row + g] : 0.0;
+  for (int r = 0; r < kMultiR; ++r) rc0[r] = rc1[r] = make_int4(0, 0, 0, 0);
+  if (nv > 0) load_visit(0, D2);
+  for (int u = -2; u < nv; ++u) {
+    // (a) loads: the descriptor of visit u + 3, the staged rows of u + 2, the
+    // first record chunks of u + 1
+    if (u + 3 < nv) load_visit(u + 3, D3);
+    int se = -1, sx = 0;
+    double sv[4] = {0.0, 0.0, 0.0, 0.0}, sc = 0.0;
+    if (u + 2 < nv) {
+      const int n2 = D2.blk.n, ne = min((D2.hd.x >> 20) & 15, kMultiE);
+      if (tid < ne * n2) {
+        se = tid / n2;
+        sx = tid - se * n2;
+        const int ab = sel4(D2.ab, se), a = ab & 255, b = (ab >> 8) & 255;
+        const double v = sel4(D2.v, se);
+        sc = a != b ? v : 0.5 * v;
+        const double* Xq = X + D2.blk.off;
+        const double* Zq = Zi + D2.blk.off;
+        sv[0] = Xq[(size_t)n2 * a + sx];
+        sv[1] = Zq[(size_t)n2 * b + sx];
+        sv[2] = Xq[(size_t)n2 * b + sx];
+        sv[3] = Zq[(size_t)n2 * a + sx];
       }
     }
-    __syncthreads();  // the previous pass is done with acc and the staged rows
-    if (staged) {
-      if (tid < cj) {
-        sja[tid] = ab_of(tid) & 255;
-        sjb[tid] = (ab_of(tid) >> 8) & 255;
-        sjv[tid] = v_of(tid);
-      }
+    if (u + 1 >= 0 && u + 1 < nv) {
+      const int c0 = D1.hd.y >> 6;
 #pragma unroll
-      for (int q = 0; q < kStg; ++q) {
-        const int x = tid + kMultiT * q;
-        const int e = x / (2 * n), h = (x / n) & 1, g = x % n;
-        if (e < cj) {
-          sR[(e * 4 + h) * kMultiN + g] = sx[q];
-          sR[(e * 4 + 2 + h) * kMultiN + g] = sz[q];
+      for (int r = 0; r < kMultiR; ++r) {
+        const int c = c0 + w + 8 * r;
+        rc1[r] = c < D1.blk.nchunk ? D1.blk.rec[(size_t)c * 64 + lane] : make_int4(0, 0, 0, 0);
+      }
+    }
+    // (b) R of visit u + 1 from S[(u + 1) & 1]
+    if (u + 1 >= 0 && u + 1 < nv) {
+      const int n1 = D1.blk.n, P1 = n1 * (n1 + 1) / 2, cj = (D1.hd.x >> 20) & 15;
+      const int ne = min(cj, kMultiE);
+      const double2* S2 = reinterpret_cast<const double2*>(S + ((u + 1) & 1) * sstride);
+      double* Rw = R + ((u + 1) & 1) * pmax;
+      for (int p = tid; p < P1; p += kMultiT) {
+        const int hi = tri_hi(p), lo = p - hi * (hi + 1) / 2;
+        double r = 0.0;
+        for (int e = 0; e < ne; ++e) {
+          const double2 l0 = S2[2 * (e * nmax + lo)], l1 = S2[2 * (e * nmax + lo) + 1];
+          const double2 h0 = S2[2 * (e * nmax + hi)], h1 = S2[2 * (e * nmax + hi) + 1];
+          r += l0.x * h0.y + h0.x * l0.y + l1.x * h1.y + h1.x * l1.y;
         }
+        if (cj > kMultiE) {
+          // entries beyond the staged ones, read from X and Z^-1 directly
+          const double* Xq = X + D1.blk.off;
+          const double* Zq = Zi + D1.blk.off;
+          for (int e = kMultiE; e < cj; ++e) {
+            const int a = er[D1.hd.w + e], b = ec[D1.hd.w + e];
+            const double v = ev[D1.hd.w + e], c = a != b ? v : 0.5 * v;
+            r += c * (Xq[(size_t)n1 * a + lo] * Zq[(size_t)n1 * b + hi] +
+                      Xq[(size_t)n1 * a + hi] * Zq[(size_t)n1 * b + lo] +
+                      Xq[(size_t)n1 * b + lo] * Zq[(size_t)n1 * a + hi] +
+                      Xq[(size_t)n1 * b + hi] * Zq[(size_t)n1 * a + lo]);
+          }
+        }
+        Rw[p] = lo != hi ? r : 0.5 * r;
       }
+    }
+    // (c) the pairs of visit u against R[u & 1]
+    if (u >= 0) {
+      const double* Rr = R + (u & 1) * pmax;
+      const int nzj = D0.hd.z, fj3 = (D0.hd.x >> 24) & 1, c0 = D0.hd.y >> 6;
+      auto chunk = [&](const int4 rec) {
+        const int meta = rec.x;
+        const int crow = meta & 8191, p = (meta >> 13) & 4095, rem = (meta >> 25) & 7;
+        double x = __hiloint2double(rec.w, rec.z) * Rr[p];
+#pragma unroll
+        for (int s = 1; s < kLight; s <<= 1) {
+          const double y = __shfl_down(x, s, 64);
+          if (s <= rem) x += y;
+        }
+        if (((meta >> 28) & 1) && crow >= j) {
+          const bool jo = !(rec.y > nzj);  // owner: tie -> the smaller index (j)
+          if (jo ? fj3 : (meta >> 29) & 1) acc[crow] += x;
+        }
+      };
+#pragma unroll
+      for (int r = 0; r < kMultiR; ++r)
+        if (c0 + w + 8 * r < D0.blk.nchunk) chunk(rc0[r]);
+      for (int c = c0 + w + 8 * kMultiR; c < D0.blk.nchunk; c += 8)
+        chunk(D0.blk.rec[(size_t)c * 64 + lane]);
+    }
+    // (d) the staged rows of visit u + 2 into S[u & 1]
+    if (se >= 0) {
+      double2* S2 = reinterpret_cast<double2*>(S + ((u + 2) & 1) * sstride);
+      S2[2 * (se * nmax + sx)] = make_double2(sc * sv[0], sv[1]);
+      S2[2 * (se * nmax + sx) + 1] = make_double2(sc * sv[2], sv[3]);
     }
     __syncthreads();
-    // one row of the block's list: its pair with j, added to acc
-    auto add_row = [&](int crow, int si, int nzi, int ci, int fi, int4 rec, double vi) {
-      const bool jo = !(nzi > nzj);  // owner: tie -> the smaller index (j)
-      const int co = jo ? cj : ci, cp = jo ? ci : cj, fo = jo ? fj : fi;
-      if (!(fo == 3 && co <= kLight && cp <= kLight)) return;
-      double val;
-      if (staged && cj == 1 && ci == 1) {
-        // one entry each, the common case: (a, b) of A_j staged, (g, d) of
-        // A_i in the row's record; the general loops' terms and order
-        const int g = (rec.w >> 8) & 255, d = (rec.w >> 16) & 255;
-        const double* R0 = sR;  // X row a, X row b, Z row a, Z row b of A_j's entry
-        if (jo) {
-          val = sjv[0] * (vi * f3_plu(sja[0] != sjb[0], g != d, R0[g], R0[d], R0[kMultiN + g],
-                                      R0[kMultiN + d], R0[2 * kMultiN + g],
-                                      R0[2 * kMultiN + d], R0[3 * kMultiN + g],
-                                      R0[3 * kMultiN + d]));
-        } else {
-          val = vi * (sjv[0] * f3_plu(g != d, sja[0] != sjb[0], R0[g], R0[kMultiN + g], R0[d],
-                                      R0[kMultiN + d], R0[2 * kMultiN + g],
-                                      R0[3 * kMultiN + g], R0[2 * kMultiN + d],
-                                      R0[3 * kMultiN + d]));
-        }
-      } else if (!staged) {
-        val = jo ? f3_pair(er, ec, ev, sj, cj, si, ci, n, Xq, Zq, 0, 1)
-                 : f3_pair(er, ec, ev, si, ci, sj, cj, n, Xq, Zq, 0, 1);
-      } else if (jo) {
-        // owner j: outer over A_j's entries (staged rows a, b), inner over A_i's
-        val = 0.0;
-        for (int e1 = 0; e1 < cj; ++e1) {
-          const double* Xa = sR + (e1 * 4 + 0) * kMultiN;
-          const double* Xb = sR + (e1 * 4 + 1) * kMultiN;
-          const double* Za = sR + (e1 * 4 + 2) * kMultiN;
-          const double* Zb = sR + (e1 * 4 + 3) * kMultiN;
-          const bool ab = sja[e1] != sjb[e1];
-          double sum = 0.0;
-          for (int e2 = 0; e2 < ci; ++e2) {
-            const int g = ci == 1 ? (rec.w >> 8) & 255 : er[si + e2];
-            const int d = ci == 1 ? (rec.w >> 16) & 255 : ec[si + e2];
-            const double v2 = ci == 1 ? vi : ev[si + e2];
-            sum += v2 * f3_plu(ab, g != d, Xa[g], Xa[d], Xb[g], Xb[d], Za[g], Za[d], Zb[g], Zb[d]);
-          }
-          val += sjv[e1] * sum;
-        }
-      } else {
-        // owner i: outer over A_i's entries (a, b), inner over A_j's (g, d),
-        // X[a][g] read as X[g][a] from the staged row g, and so on
-        val = 0.0;
-        for (int e1 = 0; e1 < ci; ++e1) {
-          const int a = ci == 1 ? (rec.w >> 8) & 255 : er[si + e1];
-          const int b = ci == 1 ? (rec.w >> 16) & 255 : ec[si + e1];
-          const double v1 = ci == 1 ? vi : ev[si + e1];
-          double sum = 0.0;
-          for (int e2 = 0; e2 < cj; ++e2) {
-            const double* Xg = sR + (e2 * 4 + 0) * kMultiN;
-            const double* Xd = sR + (e2 * 4 + 1) * kMultiN;
-            const double* Zg = sR + (e2 * 4 + 2) * kMultiN;
-            const double* Zd = sR + (e2 * 4 + 3) * kMultiN;
-            sum += sjv[e2] * f3_plu(a != b, sja[e2] != sjb[e2], Xg[a], Xd[a], Xg[b], Xd[b], Zg[a],
-                                    Zd[a], Zg[b], Zd[b]);
-          }
-          val += v1 * sum;
-        }
-      }
-      acc[crow] += val;  // one thread per row in this pass
-    };
-    if (staged) {
-      // four rows per thread per round, their records loaded together
-      // (clamped indices stay in the list); the first round's came with the
-      // staged rows
+    D0 = D1;
+    D1 = D2;
+    D2 = D3;
 #pragma unroll
-      for (int u = 0; u < kR0; ++u)
-        if (pj + tid + kMultiT * u < ncon)
-          add_row(rq0[u].x, rq0[u].y, rq0[u].z, rq0[u].w & 15, (rq0[u].w >> 4) & 3, rq0[u],
-                  vq0[u]);
-      for (int k0 = pj + tid + kR0 * kMultiT; k0 < ncon; k0 += 4 * kMultiT) {
-        int4 rq[4];
-        double vq[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int kl = min(k0 + kMultiT * u, ncon - 1);
-          rq[u] = Bc.rmeta[kl];
-          vq[u] = Bc.rval[kl];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (k0 + kMultiT * u < ncon)
-            add_row(rq[u].x, rq[u].y, rq[u].z, rq[u].w & 15, (rq[u].w >> 4) & 3, rq[u], vq[u]);
-      }
-    } else {
-      for (int ki = pj + tid; ki < ncon; ki += kMultiT)
-        add_row(Bc.con[ki], Bc.estart[ki], Bc.nzeff[ki], Bc.ecnt[ki], Bc.form[ki],
-                make_int4(0, 0, 0, 0), 0.0);
-    }
+    for (int r = 0; r < kMultiR; ++r) rc0[r] = rc1[r];
   }
-  __syncthreads();
   for (int r = j + tid; r < m; r += kMultiT) B[(size_t)r + (size_t)m * j] = acc[r];
 }
 
-int f3multi_prepare(int m) {
-  if (m > kF3MultiMaxM) return -1;
+// dynamic LDS of k_f3_multi (doubles): acc, R and S, S on a 32-byte boundary
+size_t f3multi_lds(int m, int pmax, int nmax, int* soff) {
+  const int so = (m + 2 * pmax + 3) & ~3;
+  if (soff) *soff = so;
+  return (size_t)so + (size_t)2 * kMultiE * nmax * 4;
+}
+
+int f3multi_prepare(int m, int pmax, int nmax) {
+  if (m > kF3MultiMaxM || nmax > kMultiN) return -1;
   SDPA_HIP_CHECK(hipFuncSetAttribute(
       (const void*)k_f3_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
-      (int)(sizeof(double) * ((size_t)kF3MultiMaxM + (size_t)kMultiE * 4 * kMultiN))));
+      (int)(sizeof(double) * f3multi_lds(m, pmax, nmax, nullptr))));
   return 0;
 }
 
@@ -866,11 +839,12 @@ int make_bmat(Ctx* c, int precision) {
   const int m = c->m;
   if (c->f3multi) {
     // stores every entry of the lower triangle: the first writer of B
-    hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(kMultiT),
-                       sizeof(double) * ((size_t)m + (size_t)kMultiE * 4 * kMultiN), s,
-                       c->d_f3blk, c->d_jptr, c->d_jent, (const int4*)c->d_jab,
+    int soff = 0;
+    const size_t lds = f3multi_lds(m, c->f3m_pmax, c->f3m_nmax, &soff);
+    hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(kMultiT), sizeof(double) * lds, s, c->d_f3blk,
+                       c->d_jptr, (const int4*)c->d_jent, (const int4*)c->d_jab,
                        (const double4*)c->d_jv, c->d_er, c->d_ec, c->d_ev, c->d_X, c->d_Zinv,
-                       c->d_B, m);
+                       c->d_B, m, c->f3m_pmax, c->f3m_nmax, soff);
   } else if (!c->bmatStoreFirst) {
     SDPA_HIP_CHECK(hipMemsetAsync(c->d_B, 0, sizeof(double) * (size_t)m * m, s));
   }

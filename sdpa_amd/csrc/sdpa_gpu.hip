@@ -122,7 +122,7 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
                    (void**)&c->d_f3blk, (void**)&c->d_jptr, (void**)&c->d_jent,
-                   (void**)&c->d_jab, (void**)&c->d_jv,
+                   (void**)&c->d_jab, (void**)&c->d_jv, (void**)&c->d_mrec,
                    (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P, (void**)&c->d_hpart, (void**)&c->d_xz32,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
@@ -657,39 +657,87 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
     for (int q = 0; q < nS; ++q)
       if (c->blk[q].n <= kMultiN) nf3 += c->blk[q].nF3;
     if (nS > 1 && nS < (1 << 20) && nf3 > 0 && m <= kF3MultiMaxM) {
+      // per block its entry records in chunks of 64, no constraint's entries
+      // straddling two chunks; constraints with more than kLightHost entries
+      // (heavy) are not listed
+      std::vector<int> mrec;  // int4 each
+      std::vector<long> rec0(nS, 0);
+      std::vector<int> nchunk(nS, 0);
+      std::vector<std::vector<int>> first(nS);
+      int pmax = 0, nmax = 0;
+      auto pad_to = [&](long n4) {
+        while ((long)mrec.size() < 4 * n4) mrec.push_back(0);
+      };
+      for (int q = 0; q < nS; ++q) {
+        const BlockTable& T = c->blk[q];
+        if (T.n > kMultiN) continue;
+        pmax = std::max(pmax, T.n * (T.n + 1) / 2);
+        nmax = std::max(nmax, T.n);
+        rec0[q] = (long)mrec.size() / 4;
+        long fill = 0;
+        first[q].assign(T.ncon, -1);
+        for (int pos = 0; pos < T.ncon; ++pos) {
+          const int ci = T.h_ecnt[pos];
+          if (ci < 1 || ci > kLightHost) continue;
+          if ((fill & 63) + ci > 64) fill = (fill | 63) + 1;
+          pad_to(rec0[q] + fill);
+          first[q][pos] = (int)fill;
+          const int st = T.h_estart[pos];
+          for (int e = 0; e < ci; ++e) {
+            const int lo = std::min(h_er[st + e], h_ec[st + e]);
+            const int hi = std::max(h_er[st + e], h_ec[st + e]);
+            const int meta = T.h_con[pos] | ((hi * (hi + 1) / 2 + lo) << 13) |
+                             ((ci - 1 - e) << 25) | ((e == 0 ? 1 : 0) << 28) |
+                             ((T.h_form[pos] == 3 ? 1 : 0) << 29);
+            int vw[2];
+            std::memcpy(vw, &h_ev[st + e], sizeof(vw));
+            mrec.insert(mrec.end(), {meta, T.h_nzeff[pos], vw[0], vw[1]});
+          }
+          fill += ci;
+        }
+        fill = (fill + 63) & ~63L;
+        pad_to(rec0[q] + fill);
+        nchunk[q] = (int)(fill / 64);
+      }
       std::vector<int> cnt(m + 1, 0);
       for (int q = 0; q < nS; ++q)
-        if (c->blk[q].n <= kMultiN)
-          for (int k : c->blk[q].h_con) ++cnt[k + 1];
+        for (int pos = 0; pos < (int)first[q].size(); ++pos)
+          if (first[q][pos] >= 0) ++cnt[c->blk[q].h_con[pos] + 1];
       for (int k = 0; k < m; ++k) cnt[k + 1] += cnt[k];
       std::vector<int> jent(4 * (size_t)cnt[m]), fill(cnt.begin(), cnt.end() - 1);
       std::vector<int> jab(4 * (size_t)cnt[m], 0);
       std::vector<double> jv(4 * (size_t)cnt[m], 0.0);
-      std::vector<F3Blk> fb(nS);
       for (int q = 0; q < nS; ++q) {
         const BlockTable& T = c->blk[q];
-        if (T.n > kMultiN) continue;
-        for (int pos = 0; pos < T.ncon; ++pos) {
+        for (int pos = 0; pos < (int)first[q].size(); ++pos) {
+          if (first[q][pos] < 0) continue;
           const int at = fill[T.h_con[pos]]++;
-          jent[4 * (size_t)at] = q | (std::min(T.h_ecnt[pos], 15) << 20) | (T.h_form[pos] << 24);
-          jent[4 * (size_t)at + 1] = pos;
-          jent[4 * (size_t)at + 2] = T.h_estart[pos];
-          jent[4 * (size_t)at + 3] = T.h_nzeff[pos];
+          jent[4 * (size_t)at] = q | (std::min(T.h_ecnt[pos], 15) << 20) |
+                                 ((T.h_form[pos] == 3 ? 1 : 0) << 24);
+          jent[4 * (size_t)at + 1] = first[q][pos];
+          jent[4 * (size_t)at + 2] = T.h_nzeff[pos];
+          jent[4 * (size_t)at + 3] = T.h_estart[pos];
           const int st = T.h_estart[pos];
           for (int e = 0; e < std::min(T.h_ecnt[pos], 4); ++e) {
             jab[4 * (size_t)at + e] = h_er[st + e] | (h_ec[st + e] << 8);
             jv[4 * (size_t)at + e] = h_ev[st + e];
           }
         }
-        fb[q] = F3Blk{T.d_con, T.d_estart, T.d_ecnt, T.d_nzeff, T.d_form,
-                      (const int4*)T.d_rmeta, T.d_rval, T.off, T.n, T.ncon};
       }
+      if (int e = upload(c, &c->d_mrec, mrec)) return e;
+      std::vector<F3MBlk> fb(nS, F3MBlk{nullptr, 0, 0, 0});
+      for (int q = 0; q < nS; ++q)
+        if (c->blk[q].n <= kMultiN)
+          fb[q] = F3MBlk{reinterpret_cast<const int4*>(c->d_mrec) + rec0[q], c->blk[q].off,
+                         c->blk[q].n, nchunk[q]};
       if (int e = upload(c, &c->d_jptr, cnt)) return e;
       if (int e = upload(c, &c->d_jent, jent)) return e;
       if (int e = upload(c, &c->d_jab, jab)) return e;
       if (int e = upload(c, &c->d_jv, jv)) return e;
       if (int e = upload(c, &c->d_f3blk, fb)) return e;
-      if (int e = f3multi_prepare(m)) return e;
+      if (int e = f3multi_prepare(m, pmax, nmax)) return e;
+      c->f3m_pmax = pmax;
+      c->f3m_nmax = nmax;
       c->f3multi = 1;
     }
   }
