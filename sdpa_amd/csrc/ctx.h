@@ -42,13 +42,8 @@ struct BlockTable {
   std::vector<int> f1rlo;
 };
 
-// one SDP block's entry records for the kernel that takes the light F3
-// pairs of all blocks of n <= kMultiN at once (64 per chunk)
-struct F3MBlk {
-  const int4* rec;
-  long off;  // packed offset of the block in X, Z^-1
-  int n, nchunk;
-};
+// int4 words per visit record of the all-blocks F3 kernel
+constexpr int kVisInt4 = 5;
 
 struct Ctx {
   int device = 0;
@@ -77,15 +72,12 @@ struct Ctx {
   // entries, in block order
   int f3multi = 0;
   int f3m_pmax = 0, f3m_nmax = 0;  // largest n (n + 1) / 2 and n of those blocks
-  F3MBlk* d_f3blk = nullptr;
   int* d_jptr = nullptr;  // m + 1
-  // per visit: q | min(ecnt, 15) << 20 | (form == 3) << 24, the position of
-  // j's first entry record in the block's records, nzeff, estart (int4);
-  // the first four entries of A_j in the block, a | b << 8 each (int4), and
-  // their values (double4)
+  // per visit kVisInt4 int4: the block's first entry record and chunks, n,
+  // min(ecnt, 15) and (form == 3) of A_j, j's chunk; nzeff, estart, the
+  // block's packed offset; a | b << 8 of A_j's first four entries (16 bits
+  // each) and their values
   int* d_jent = nullptr;
-  int* d_jab = nullptr;
-  double* d_jv = nullptr;
   int* d_mrec = nullptr;  // every block's entry records (int4 each)
   std::vector<BlockTable> blk;
   // one allocation holding every block's small tables (d_con ... d_dlist,

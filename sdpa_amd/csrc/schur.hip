@@ -99,13 +99,7 @@ __device__ __forceinline__ double f3_pair(const int* __restrict__ er, const int*
 constexpr int kMultiE = 4;   // entries of A_j staged (more: read from X, Z^-1 directly)
 constexpr int kMultiT = 512; // threads per work-group (one Schur column)
 constexpr int kMultiR = 4;   // record chunks per wave loaded one visit ahead
-
-struct MVisit {
-  int4 hd;    // q | cj << 20 | (j takes F3) << 24, j's first record, nzeff, estart
-  int4 ab;    // a | b << 8 of A_j's first kMultiE entries in the block
-  double4 v;  // their values
-  F3MBlk blk;
-};
+constexpr int kMultiP = (kMultiN * (kMultiN + 1) / 2 + kMultiT - 1) / kMultiT;  // positions per thread
 
 // row of the upper-triangle position p = hi (hi + 1) / 2 + lo
 __device__ __forceinline__ int tri_hi(int p) {
@@ -115,53 +109,66 @@ __device__ __forceinline__ int tri_hi(int p) {
   return h;
 }
 
-__device__ __forceinline__ int sel4(const int4& q, int e) {
-  return e == 0 ? q.x : e == 1 ? q.y : e == 2 ? q.z : q.w;
-}
-__device__ __forceinline__ double sel4(const double4& q, int e) {
-  return e == 0 ? q.x : e == 1 ? q.y : e == 2 ? q.z : q.w;
-}
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-__global__ __launch_bounds__(kMultiT) void k_f3_multi(
-    const F3MBlk* __restrict__ blks, const int* __restrict__ jptr, const int4* __restrict__ jent,
-    const int4* __restrict__ jab, const double4* __restrict__ jv, const int* __restrict__ er,
-    const int* __restrict__ ec, const double* __restrict__ ev, const double* __restrict__ X,
-    const double* __restrict__ Zi, double* __restrict__ B, int m, int pmax, int nmax,
-    int soff) {
+// Visit records (kVisInt4 int4 each, in column order): [0] first record of
+// the block, chunks, n | cj << 8 | (j takes F3) << 16 (cj = A_j's entries in
+// the block), j's chunk; [1] j's NonZeroEffect, estart of A_j's entries, the
+// block's packed offset (two words); [2] a | b << 8 of A_j's first two
+// entries, of the next two (16 bits each); [3], [4] their values.  A ring of
+// four in LDS holds the visits u .. u + 3 (written one visit ahead by five
+// lanes' vector loads, read back as broadcasts).
+__global__ __launch_bounds__(kMultiT) __attribute__((amdgpu_waves_per_eu(4))) void k_f3_multi(
+    const int4* __restrict__ vis, const int* __restrict__ jptr, const int4* __restrict__ mrec,
+    const int* __restrict__ er, const int* __restrict__ ec, const double* __restrict__ ev,
+    const double* __restrict__ X, const double* __restrict__ Zi, double* __restrict__ B, int m,
+    int pmax, int nmax, int soff) {
   extern __shared__ double acc[];
+  __shared__ int4 ring[4][kVisInt4];
   double* R = acc + m;     // [2][pmax]
   double* S = acc + soff;  // [2][kMultiE][nmax] x {U1, W1, U2, W2}
   const int sstride = kMultiE * nmax * 4;
   const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int r = j + tid; r < m; r += kMultiT) acc[r] = 0.0;
   const int t0 = jptr[j], nv = jptr[j + 1] - t0;
-  auto load_visit = [&](int u, MVisit& V) {
-    V.hd = jent[t0 + u];
-    V.ab = jab[t0 + u];
-    V.v = jv[t0 + u];
-    V.blk = blks[V.hd.x & 0xfffff];
-  };
-  MVisit D0, D1, D2, D3;
+  // this thread's positions p = tid + kMultiT k of R (the same for every
+  // block): lo | hi << 16
+  int plh[kMultiP];
+#pragma unroll
+  for (int k = 0; k < kMultiP; ++k) {
+    const int p = tid + kMultiT * k, hi = tri_hi(p);
+    plh[k] = (p - hi * (hi + 1) / 2) | (hi << 16);
+  }
+  // byte addresses of lanes lane + 1, + 2, + 4 (mod 64) for ds_bpermute
+  const int bp1 = ((lane + 1) & 63) << 2, bp2 = ((lane + 2) & 63) << 2,
+            bp4 = ((lane + 4) & 63) << 2;
+  if (tid < kVisInt4 && nv > 0) ring[0][tid] = vis[(size_t)t0 * kVisInt4 + tid];
   int4 rc0[kMultiR], rc1[kMultiR];
 #pragma unroll
   for (int r = 0; r < kMultiR; ++r) rc0[r] = rc1[r] = make_int4(0, 0, 0, 0);
-  if (nv > 0) load_visit(0, D2);
+  __syncthreads();
   for (int u = -2; u < nv; ++u) {
-    // (a) loads: the descriptor of visit u + 3, the staged rows of u + 2, the
+    // (a) loads: the record of visit u + 3, the staged rows of u + 2, the
     // first record chunks of u + 1
-    if (u + 3 < nv) load_visit(u + 3, D3);
+    int4 vq = make_int4(0, 0, 0, 0);
+    if (u + 3 < nv && tid < kVisInt4) vq = vis[(size_t)(t0 + u + 3) * kVisInt4 + tid];
     int se = -1, sx = 0;
     double sv[4] = {0.0, 0.0, 0.0, 0.0}, sc = 0.0;
     if (u + 2 < nv) {
-      const int n2 = D2.blk.n, ne = min((D2.hd.x >> 20) & 15, kMultiE);
+      const int4* V = ring[(u + 2) & 3];
+      const int4 q0 = V[0];
+      const int n2 = wave_uniform(q0.z & 255), ne = min(wave_uniform((q0.z >> 8) & 15), kMultiE);
       if (tid < ne * n2) {
         se = tid / n2;
         sx = tid - se * n2;
-        const int ab = sel4(D2.ab, se), a = ab & 255, b = (ab >> 8) & 255;
-        const double v = sel4(D2.v, se);
+        const int4 q1 = V[1], q2 = V[2], qv = V[3 + (se >> 1)];
+        const int ab = ((se < 2 ? q2.x : q2.y) >> (16 * (se & 1))) & 0xffff;
+        const int a = ab & 255, b = ab >> 8;
+        const double v = (se & 1) ? __hiloint2double(qv.w, qv.z) : __hiloint2double(qv.y, qv.x);
         sc = a != b ? v : 0.5 * v;
-        const double* Xq = X + D2.blk.off;
-        const double* Zq = Zi + D2.blk.off;
+        const long off = (long)(((unsigned long)(unsigned)q1.w << 32) | (unsigned)q1.z);
+        const double* Xq = X + off;
+        const double* Zq = Zi + off;
         sv[0] = Xq[(size_t)n2 * a + sx];
         sv[1] = Zq[(size_t)n2 * b + sx];
         sv[2] = Xq[(size_t)n2 * b + sx];
@@ -169,21 +176,26 @@ __global__ __launch_bounds__(kMultiT) void k_f3_multi(
       }
     }
     if (u + 1 >= 0 && u + 1 < nv) {
-      const int c0 = D1.hd.y >> 6;
+      const int4 q0 = ring[(u + 1) & 3][0];
+      const int rec0 = wave_uniform(q0.x), nch = wave_uniform(q0.y), c0 = wave_uniform(q0.w);
 #pragma unroll
       for (int r = 0; r < kMultiR; ++r) {
         const int c = c0 + w + 8 * r;
-        rc1[r] = c < D1.blk.nchunk ? D1.blk.rec[(size_t)c * 64 + lane] : make_int4(0, 0, 0, 0);
+        rc1[r] = c < nch ? mrec[(size_t)rec0 + (size_t)c * 64 + lane] : make_int4(0, 0, 0, 0);
       }
     }
     // (b) R of visit u + 1 from S[(u + 1) & 1]
     if (u + 1 >= 0 && u + 1 < nv) {
-      const int n1 = D1.blk.n, P1 = n1 * (n1 + 1) / 2, cj = (D1.hd.x >> 20) & 15;
-      const int ne = min(cj, kMultiE);
+      const int4 q0 = ring[(u + 1) & 3][0];
+      const int n1 = wave_uniform(q0.z & 255), cj = wave_uniform((q0.z >> 8) & 15);
+      const int P1 = n1 * (n1 + 1) / 2, ne = min(cj, kMultiE);
       const double2* S2 = reinterpret_cast<const double2*>(S + ((u + 1) & 1) * sstride);
       double* Rw = R + ((u + 1) & 1) * pmax;
-      for (int p = tid; p < P1; p += kMultiT) {
-        const int hi = tri_hi(p), lo = p - hi * (hi + 1) / 2;
+#pragma unroll
+      for (int k = 0; k < kMultiP; ++k) {
+        const int p = tid + kMultiT * k;
+        if (p >= P1) break;
+        const int lo = plh[k] & 0xffff, hi = plh[k] >> 16;
         double r = 0.0;
         for (int e = 0; e < ne; ++e) {
           const double2 l0 = S2[2 * (e * nmax + lo)], l1 = S2[2 * (e * nmax + lo) + 1];
@@ -192,11 +204,13 @@ __global__ __launch_bounds__(kMultiT) void k_f3_multi(
         }
         if (cj > kMultiE) {
           // entries beyond the staged ones, read from X and Z^-1 directly
-          const double* Xq = X + D1.blk.off;
-          const double* Zq = Zi + D1.blk.off;
+          const int4 q1 = ring[(u + 1) & 3][1];
+          const long off = (long)(((unsigned long)(unsigned)q1.w << 32) | (unsigned)q1.z);
+          const double* Xq = X + off;
+          const double* Zq = Zi + off;
           for (int e = kMultiE; e < cj; ++e) {
-            const int a = er[D1.hd.w + e], b = ec[D1.hd.w + e];
-            const double v = ev[D1.hd.w + e], c = a != b ? v : 0.5 * v;
+            const int a = er[q1.y + e], b = ec[q1.y + e];
+            const double v = ev[q1.y + e], c = a != b ? v : 0.5 * v;
             r += c * (Xq[(size_t)n1 * a + lo] * Zq[(size_t)n1 * b + hi] +
                       Xq[(size_t)n1 * a + hi] * Zq[(size_t)n1 * b + lo] +
                       Xq[(size_t)n1 * b + lo] * Zq[(size_t)n1 * a + hi] +
@@ -209,37 +223,58 @@ __global__ __launch_bounds__(kMultiT) void k_f3_multi(
     // (c) the pairs of visit u against R[u & 1]
     if (u >= 0) {
       const double* Rr = R + (u & 1) * pmax;
-      const int nzj = D0.hd.z, fj3 = (D0.hd.x >> 24) & 1, c0 = D0.hd.y >> 6;
-      auto chunk = [&](const int4 rec) {
-        const int meta = rec.x;
-        const int crow = meta & 8191, p = (meta >> 13) & 4095, rem = (meta >> 25) & 7;
-        double x = __hiloint2double(rec.w, rec.z) * Rr[p];
+      const int4 q0 = ring[u & 3][0], q1 = ring[u & 3][1];
+      const int rec0 = wave_uniform(q0.x), nch = wave_uniform(q0.y), c0 = wave_uniform(q0.w);
+      const int fj3 = wave_uniform((q0.z >> 16) & 1), nzj = wave_uniform(q1.x);
+      // kMultiR chunks of one wave at once (records past the list are
+      // zeros: no head), so that their LDS reads overlap
+      auto chunks = [&](const int4* rec) {
+        double x[kMultiR];
 #pragma unroll
-        for (int s = 1; s < kLight; s <<= 1) {
-          const double y = __shfl_down(x, s, 64);
-          if (s <= rem) x += y;
+        for (int r = 0; r < kMultiR; ++r)
+          x[r] = __hiloint2double(rec[r].w, rec[r].z) * Rr[(rec[r].x >> 13) & 4095];
+        // segmented sums over the lanes of a constraint (rem after this one)
+        const int bps[3] = {bp1, bp2, bp4};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          int lo32[kMultiR], hi32[kMultiR];
+#pragma unroll
+          for (int r = 0; r < kMultiR; ++r) {
+            lo32[r] = __builtin_amdgcn_ds_bpermute(bps[k], __double2loint(x[r]));
+            hi32[r] = __builtin_amdgcn_ds_bpermute(bps[k], __double2hiint(x[r]));
+          }
+#pragma unroll
+          for (int r = 0; r < kMultiR; ++r)
+            if ((1 << k) <= ((rec[r].x >> 25) & 7)) x[r] += __hiloint2double(hi32[r], lo32[r]);
         }
-        if (((meta >> 28) & 1) && crow >= j) {
-          const bool jo = !(rec.y > nzj);  // owner: tie -> the smaller index (j)
-          if (jo ? fj3 : (meta >> 29) & 1) acc[crow] += x;
+#pragma unroll
+        for (int r = 0; r < kMultiR; ++r) {
+          const int meta = rec[r].x, crow = meta & 8191;
+          const bool jo = !(rec[r].y > nzj);  // owner: tie -> the smaller index (j)
+          if (((meta >> 28) & 1) && crow >= j && (jo ? fj3 : (meta >> 29) & 1))
+            acc[crow] += x[r];
         }
       };
+      chunks(rc0);
+      for (int c1 = c0 + w + 8 * kMultiR; c1 < nch; c1 += 8 * kMultiR) {
+        int4 rq[kMultiR];
 #pragma unroll
-      for (int r = 0; r < kMultiR; ++r)
-        if (c0 + w + 8 * r < D0.blk.nchunk) chunk(rc0[r]);
-      for (int c = c0 + w + 8 * kMultiR; c < D0.blk.nchunk; c += 8)
-        chunk(D0.blk.rec[(size_t)c * 64 + lane]);
+        for (int r = 0; r < kMultiR; ++r) {
+          const int c = c1 + 8 * r;
+          rq[r] = c < nch ? mrec[(size_t)rec0 + (size_t)c * 64 + lane] : make_int4(0, 0, 0, 0);
+        }
+        chunks(rq);
+      }
     }
-    // (d) the staged rows of visit u + 2 into S[u & 1]
+    // (d) the staged rows of visit u + 2 into S[u & 1], the record of visit
+    // u + 3 into the ring (the slot of visit u - 1, done)
     if (se >= 0) {
       double2* S2 = reinterpret_cast<double2*>(S + ((u + 2) & 1) * sstride);
       S2[2 * (se * nmax + sx)] = make_double2(sc * sv[0], sv[1]);
       S2[2 * (se * nmax + sx) + 1] = make_double2(sc * sv[2], sv[3]);
     }
+    if (u + 3 < nv && tid < kVisInt4) ring[(u + 3) & 3][tid] = vq;
     __syncthreads();
-    D0 = D1;
-    D1 = D2;
-    D2 = D3;
 #pragma unroll
     for (int r = 0; r < kMultiR; ++r) rc0[r] = rc1[r];
   }
@@ -841,10 +876,10 @@ int make_bmat(Ctx* c, int precision) {
     // stores every entry of the lower triangle: the first writer of B
     int soff = 0;
     const size_t lds = f3multi_lds(m, c->f3m_pmax, c->f3m_nmax, &soff);
-    hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(kMultiT), sizeof(double) * lds, s, c->d_f3blk,
-                       c->d_jptr, (const int4*)c->d_jent, (const int4*)c->d_jab,
-                       (const double4*)c->d_jv, c->d_er, c->d_ec, c->d_ev, c->d_X, c->d_Zinv,
-                       c->d_B, m, c->f3m_pmax, c->f3m_nmax, soff);
+    hipLaunchKernelGGL(k_f3_multi, dim3(m), dim3(kMultiT), sizeof(double) * lds, s,
+                       (const int4*)c->d_jent, c->d_jptr, (const int4*)c->d_mrec, c->d_er,
+                       c->d_ec, c->d_ev, c->d_X, c->d_Zinv, c->d_B, m, c->f3m_pmax,
+                       c->f3m_nmax, soff);
   } else if (!c->bmatStoreFirst) {
     SDPA_HIP_CHECK(hipMemsetAsync(c->d_B, 0, sizeof(double) * (size_t)m * m, s));
   }

@@ -121,8 +121,7 @@ void free_problem(Ctx* c) {
                    (void**)&c->d_Dx, (void**)&c->d_Dz, (void**)&c->d_dy, (void**)&c->d_rzinv,
                    (void**)&c->d_xrdzinv, (void**)&c->d_g, (void**)&c->d_w1, (void**)&c->d_w2,
                    (void**)&c->d_w3, (void**)&c->d_pvec, (void**)&c->d_dmat, (void**)&c->d_X0,
-                   (void**)&c->d_f3blk, (void**)&c->d_jptr, (void**)&c->d_jent,
-                   (void**)&c->d_jab, (void**)&c->d_jv, (void**)&c->d_mrec,
+                   (void**)&c->d_jptr, (void**)&c->d_jent, (void**)&c->d_mrec,
                    (void**)&c->d_Z0, (void**)&c->d_B, (void**)&c->d_Binv, (void**)&c->d_Winv, (void**)&c->d_G, (void**)&c->d_F1C, (void**)&c->d_F1P, (void**)&c->d_hpart, (void**)&c->d_xz32,
                    (void**)&c->d_info, (void**)&c->d_red, (void**)&c->d_eig,
                    (void**)&c->d_sdpN, (void**)&c->d_sdpOff};
@@ -704,37 +703,31 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
         for (int pos = 0; pos < (int)first[q].size(); ++pos)
           if (first[q][pos] >= 0) ++cnt[c->blk[q].h_con[pos] + 1];
       for (int k = 0; k < m; ++k) cnt[k + 1] += cnt[k];
-      std::vector<int> jent(4 * (size_t)cnt[m]), fill(cnt.begin(), cnt.end() - 1);
-      std::vector<int> jab(4 * (size_t)cnt[m], 0);
-      std::vector<double> jv(4 * (size_t)cnt[m], 0.0);
+      constexpr int kW = 4 * kVisInt4;  // ints per visit record
+      std::vector<int> jent(kW * (size_t)cnt[m], 0), fill(cnt.begin(), cnt.end() - 1);
       for (int q = 0; q < nS; ++q) {
         const BlockTable& T = c->blk[q];
         for (int pos = 0; pos < (int)first[q].size(); ++pos) {
           if (first[q][pos] < 0) continue;
-          const int at = fill[T.h_con[pos]]++;
-          jent[4 * (size_t)at] = q | (std::min(T.h_ecnt[pos], 15) << 20) |
-                                 ((T.h_form[pos] == 3 ? 1 : 0) << 24);
-          jent[4 * (size_t)at + 1] = first[q][pos];
-          jent[4 * (size_t)at + 2] = T.h_nzeff[pos];
-          jent[4 * (size_t)at + 3] = T.h_estart[pos];
+          int* r = &jent[kW * (size_t)fill[T.h_con[pos]]++];
           const int st = T.h_estart[pos];
+          r[0] = (int)rec0[q];
+          r[1] = nchunk[q];
+          r[2] = T.n | (std::min(T.h_ecnt[pos], 15) << 8) | ((T.h_form[pos] == 3 ? 1 : 0) << 16);
+          r[3] = first[q][pos] >> 6;
+          r[4] = T.h_nzeff[pos];
+          r[5] = st;
+          std::memcpy(&r[6], &T.off, sizeof(long));
           for (int e = 0; e < std::min(T.h_ecnt[pos], 4); ++e) {
-            jab[4 * (size_t)at + e] = h_er[st + e] | (h_ec[st + e] << 8);
-            jv[4 * (size_t)at + e] = h_ev[st + e];
+            r[8 + e / 2] |= (int)((unsigned)(h_er[st + e] | (h_ec[st + e] << 8)) << (16 * (e & 1)));
+            std::memcpy(&r[12 + 2 * e], &h_ev[st + e], sizeof(double));
           }
         }
       }
+      if ((long)mrec.size() / 4 > (long)INT_MAX) return -1;
       if (int e = upload(c, &c->d_mrec, mrec)) return e;
-      std::vector<F3MBlk> fb(nS, F3MBlk{nullptr, 0, 0, 0});
-      for (int q = 0; q < nS; ++q)
-        if (c->blk[q].n <= kMultiN)
-          fb[q] = F3MBlk{reinterpret_cast<const int4*>(c->d_mrec) + rec0[q], c->blk[q].off,
-                         c->blk[q].n, nchunk[q]};
       if (int e = upload(c, &c->d_jptr, cnt)) return e;
       if (int e = upload(c, &c->d_jent, jent)) return e;
-      if (int e = upload(c, &c->d_jab, jab)) return e;
-      if (int e = upload(c, &c->d_jv, jv)) return e;
-      if (int e = upload(c, &c->d_f3blk, fb)) return e;
       if (int e = f3multi_prepare(m, pmax, nmax)) return e;
       c->f3m_pmax = pmax;
       c->f3m_nmax = nmax;
