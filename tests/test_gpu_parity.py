@@ -82,6 +82,11 @@ CASES = {
     # blocks above 64 (per-block kernels adding to B) beside small ones
     "multiblock_sizes": lambda: PR.multi_block(n_sdp_blocks=6, nmin=10, nmax=120, lp=10, m=150,
                                                density=0.01, seed=13),
+    # small blocks (n 8..12) whose constraints have 1..10 entries each: the
+    # all-blocks F3 kernel's entry chunks padded at constraint boundaries, its
+    # column entries beyond the four staged ones, and heavy lists beside them
+    "multiblock_entries": lambda: PR.multi_block(n_sdp_blocks=10, nmin=8, nmax=12, lp=0, m=400,
+                                                 density=0.05, seed=21),
     # the benchmark workloads at full size: Lovasz theta G(200, 0.3), m = 6011
     # (two 4096-row segments of the column-block F3 kernel plus the trace
     # row), and the 200-block problem, m = 5000 (the all-blocks F3 kernel with
