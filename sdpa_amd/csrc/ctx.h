@@ -130,7 +130,7 @@ struct Ctx {
   float* d_xz32 = nullptr;  // FP32 assembly: one block's X and Z^-1 in FP32
   size_t capXZ32 = 0;
   int* d_info = nullptr;   // factorization info (one per batch item)
-  double* d_red = nullptr;  // reduction partials
+  double* d_red = nullptr;  // reduction partials (kMaxRed x 4096), then the results (64)
   double* h_red = nullptr;  // pinned host mirror of reduction results
   int* h_info = nullptr;    // pinned
   double* d_eig = nullptr;  // per-item eigen results
@@ -160,6 +160,8 @@ struct Ctx {
   std::string last_error;
 };
 
+// ops.hip: reductions per batch (dots, norms)
+constexpr int kMaxRed = 16;
 // schur.hip
 constexpr int kF3MultiMaxM = 8192;  // the all-blocks F3 kernel keeps a Schur column in LDS
 constexpr int kMultiN = 64;         // ... and takes the blocks up to this size

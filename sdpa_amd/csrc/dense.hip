@@ -1898,10 +1898,19 @@ __global__ __launch_bounds__(256) void k_trtri_row(const MatDesc* __restrict__ L
   acc_zero(acc);
   for (int K = J; K < I; ++K) {
     const int k0 = K * TB;
-    for (int e = threadIdx.x; e < TB * TB; e += 256) {
-      int r = e & 63, c = e >> 6;
-      As[r + TLD * c] = (r < ib) ? Lm.ptr[i0 + r + (size_t)Lm.ld * (k0 + c)] : 0.0;
-      Bs[r + TLD * c] = Wm.ptr[k0 + r + (size_t)Wm.ld * (J * TB + c)];
+    // a thread's 32 loads in flight at once, then the LDS stores
+    double la[16], wb[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = threadIdx.x + 256 * q, r = e & 63, c = e >> 6;
+      la[q] = (r < ib) ? Lm.ptr[i0 + r + (size_t)Lm.ld * (k0 + c)] : 0.0;
+      wb[q] = Wm.ptr[k0 + r + (size_t)Wm.ld * (J * TB + c)];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = threadIdx.x + 256 * q, r = e & 63, c = e >> 6;
+      As[r + TLD * c] = la[q];
+      Bs[r + TLD * c] = wb[q];
     }
     __syncthreads();
     mma_tile(acc, As, TLD, Bs, TLD, TB);
@@ -1919,9 +1928,15 @@ __global__ __launch_bounds__(256) void k_trtri_row(const MatDesc* __restrict__ L
         Bs[r + TLD * c] = acc[ti][tj][v];
       }
   const double* Dv = dinv + ((size_t)b * nblk_max + I) * TB * TB;
-  for (int e = threadIdx.x; e < TB * TB; e += 256) {
-    int r = e & 63, c = e >> 6;
-    As[r + TLD * c] = Dv[r + TB * c];
+  {
+    double dv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dv[q] = Dv[threadIdx.x + 256 * q];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      As[(e & 63) + TLD * (e >> 6)] = dv[q];
+    }
   }
   __syncthreads();
   acc_zero(acc);
@@ -1952,9 +1967,13 @@ __global__ __launch_bounds__(256) void k_diag_place(const MatDesc* __restrict__ 
   const int kb = min(TB, Lm.n - k0);
   const double* Dv = dinv + ((size_t)b * nblk_max + I) * TB * TB;
   double* Wd = Wm.ptr + k0 + (size_t)Wm.ld * k0;
-  for (int e = threadIdx.x; e < TB * TB; e += 256) {
-    const int r = e & 63, c = e >> 6;
-    if (r < kb && c < kb) Wd[r + (size_t)Wm.ld * c] = Dv[r + TB * c];
+  double dv[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) dv[q] = Dv[threadIdx.x + 256 * q];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q, r = e & 63, c = e >> 6;
+    if (r < kb && c < kb) Wd[r + (size_t)Wm.ld * c] = dv[q];
   }
   const int ncol = Wm.n - (k0 + kb);
   for (int e = threadIdx.x; e < kb * ncol; e += 256) {

@@ -16,7 +16,7 @@
 namespace sdpa_amd {
 
 constexpr int kRedChunk = 4096;   // elements per partial sum
-constexpr int kMaxChunks = 4096;  // partial slots per dot
+constexpr int kMaxChunks = 4096;  // partial slots per reduction
 
 // out_i = sum over the sparse terms of constraint i of v*(D[p1] + D[p2]) (p2 != p1)
 __global__ __launch_bounds__(256) void k_adot_sparse(const int* __restrict__ cptr,
@@ -103,13 +103,32 @@ __global__ void k_symmetrize(double* __restrict__ X, const long* __restrict__ of
   a[j + (size_t)n * i] = s;
 }
 
-// partial sums of x.y (op 0), max|x| (op 1), min x*y*y (op 2, LP step check)
-__global__ __launch_bounds__(256) void k_reduce_partial(const double* __restrict__ x,
-                                                        const double* __restrict__ y, long len,
-                                                        int op, double* __restrict__ part) {
+// A batch of reductions in two launches: x.y (op 0), max|x| (op 1), min x*y*y
+// (op 2, LP step check) of each job, its partial sums over chunks of
+// kRedChunk elements (a fixed tree per chunk), then the partials added in
+// chunk order: every result is reproducible and independent of the batch.
+struct RedJob {
+  const double* x;
+  const double* y;
+  long len;
+  int op;
+  int out;  // result slot
+};
+struct RedBatch {
+  RedJob j[kMaxRed];
+  int n;
+};
+
+// grid = (chunks of the longest job, jobs)
+__global__ __launch_bounds__(256) void k_reduce_partial(RedBatch B, double* __restrict__ part) {
+  const RedJob J = B.j[blockIdx.y];
   const long base = (long)blockIdx.x * kRedChunk;
+  if (blockIdx.x > 0 && base >= J.len) return;
+  const int op = J.op;
+  const double* __restrict__ x = J.x;
+  const double* __restrict__ y = J.y;
   double s = (op == 2) ? 1.0e50 : 0.0;
-  for (long t = base + threadIdx.x; t < base + kRedChunk && t < len; t += 256) {
+  for (long t = base + threadIdx.x; t < base + kRedChunk && t < J.len; t += 256) {
     if (op == 0) s += x[t] * y[t];
     else if (op == 1) s = fmax(s, fabs(x[t]));
     else s = fmin(s, x[t] * y[t] * y[t]);
@@ -126,30 +145,59 @@ __global__ __launch_bounds__(256) void k_reduce_partial(const double* __restrict
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * kMaxChunks + blockIdx.x] = red[0];
 }
 
-__global__ void k_reduce_final(const double* __restrict__ part, int nparts, int op,
+// thread k: job k's partials in chunk order
+__global__ void k_reduce_final(RedBatch B, const double* __restrict__ part,
                                double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double s = (op == 2) ? 1.0e50 : 0.0;
-  for (int i = 0; i < nparts; ++i) {
-    if (op == 0) s += part[i];
-    else if (op == 1) s = fmax(s, part[i]);
-    else s = fmin(s, part[i]);
+  const int k = threadIdx.x;
+  if (k >= B.n) return;
+  const RedJob J = B.j[k];
+  const int np = (int)((J.len + kRedChunk - 1) / kRedChunk);
+  double s = (J.op == 2) ? 1.0e50 : 0.0;
+  for (int i = 0; i < (np > 0 ? np : 1); ++i) {
+    const double v = part[(size_t)k * kMaxChunks + i];
+    if (J.op == 0) s += v;
+    else if (J.op == 1) s = fmax(s, v);
+    else s = fmin(s, v);
   }
-  *out = s;
+  out[J.out] = s;
+}
+
+// the two launches of a batch (results to out[job.out])
+static int reduce_batch(Ctx* c, const RedBatch& B, double* out) {
+  if (B.n <= 0) return 0;
+  int npmax = 1;
+  for (int k = 0; k < B.n; ++k) {
+    const long np = (B.j[k].len + kRedChunk - 1) / kRedChunk;
+    if (np > kMaxChunks) return -4;
+    npmax = np > npmax ? (int)np : npmax;
+  }
+  hipLaunchKernelGGL(k_reduce_partial, dim3(npmax, B.n), dim3(256), 0, c->stream, B, c->d_red);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, B, c->d_red, out);
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 // sum of the diagonals of the SDP blocks plus the LP entries
+// (few blocks: the work-group runs down each diagonal; many: a thread per block)
 __global__ void k_trace(const double* __restrict__ X, const long* __restrict__ off,
                         const int* __restrict__ nn, int nblk, long lpOff, int nLP,
                         double* __restrict__ out) {
   __shared__ double red[256];
   double s = 0.0;
-  for (int q = 0; q < nblk; ++q) {
-    const int n = nn[q];
-    for (int k = threadIdx.x; k < n; k += 256) s += X[off[q] + (long)k * n + k];
+  if (nblk <= 32) {
+    for (int q = 0; q < nblk; ++q) {
+      const int n = nn[q];
+      for (int k = threadIdx.x; k < n; k += 256) s += X[off[q] + (long)k * n + k];
+    }
+  } else {
+    for (int q = threadIdx.x; q < nblk; q += 256) {
+      const int n = nn[q];
+      const double* Xq = X + off[q];
+      for (int k = 0; k < n; ++k) s += Xq[(long)k * n + k];
+    }
   }
   for (int t = threadIdx.x; t < nLP; t += 256) s += X[lpOff + t];
   red[threadIdx.x] = s;
@@ -277,21 +325,14 @@ int residuals(Ctx* c, double* normP, double* normD) {
   if (int e = axpbypcz(c, c->total, -1.0, c->d_dmat, 1.0, c->d_C, -1.0, c->d_Z, c->d_dmat))
     return e;
   // max norms
-  const int codes_dummy = 0;
-  (void)codes_dummy;
-  double* part = c->d_red;
-  int np1 = (int)((c->m + kRedChunk - 1) / kRedChunk);
-  hipLaunchKernelGGL(k_reduce_partial, dim3(np1), dim3(256), 0, c->stream, c->d_pvec,
-                     (const double*)nullptr, (long)c->m, 1, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, part, np1, 1,
-                     part + 2 * kMaxChunks);
-  int np2 = (int)((c->total + kRedChunk - 1) / kRedChunk);
-  hipLaunchKernelGGL(k_reduce_partial, dim3(np2), dim3(256), 0, c->stream, c->d_dmat,
-                     (const double*)nullptr, c->total, 1, part + kMaxChunks);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, part + kMaxChunks, np2, 1,
-                     part + 2 * kMaxChunks + 1);
-  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, part + 2 * kMaxChunks, 2 * sizeof(double),
-                                hipMemcpyDeviceToHost, c->stream));
+  RedBatch B;
+  B.n = 2;
+  B.j[0] = RedJob{c->d_pvec, nullptr, (long)c->m, 1, 0};
+  B.j[1] = RedJob{c->d_dmat, nullptr, c->total, 1, 1};
+  double* res = c->d_red + (size_t)kMaxRed * kMaxChunks;
+  if (int e = reduce_batch(c, B, res)) return e;
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, res, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
   *normP = c->h_red[0];
   *normD = c->h_red[1];
@@ -300,8 +341,10 @@ int residuals(Ctx* c, double* normP, double* normD) {
 
 int compute_dots(Ctx* c, int nq, const int* codes, double* out) {
   hipStream_t s = c->stream;
-  double* part = c->d_red;
-  double* res = c->d_red + 2 * kMaxChunks;
+  if (nq > 64) return -2;
+  double* res = c->d_red + (size_t)kMaxRed * kMaxChunks;
+  RedBatch B;
+  B.n = 0;
   for (int k = 0; k < nq; ++k) {
     const double *x = nullptr, *y = nullptr;
     long len = c->total;
@@ -326,12 +369,13 @@ int compute_dots(Ctx* c, int nq, const int* codes, double* out) {
         return -2;
     }
     if (!x || !y) return -3;
-    int np = (int)((len + kRedChunk - 1) / kRedChunk);
-    if (np > kMaxChunks) return -4;
-    hipLaunchKernelGGL(k_reduce_partial, dim3(np > 0 ? np : 1), dim3(256), 0, s, x, y, len, 0,
-                       part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, part, np, 0, res + k);
+    B.j[B.n++] = RedJob{x, y, len, 0, k};
+    if (B.n == kMaxRed) {  // a full batch (the partial slots are reused by the next)
+      if (int e = reduce_batch(c, B, res)) return e;
+      B.n = 0;
+    }
   }
+  if (int e = reduce_batch(c, B, res)) return e;
   SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, res, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
   SDPA_HIP_CHECK(hipStreamSynchronize(s));
   for (int k = 0; k < nq; ++k) out[k] = c->h_red[k];
@@ -452,14 +496,12 @@ int inverse_chol(Ctx* c) {
 int lp_min_step(Ctx* c, const double* D, const double* L, double* out) {
   // min over LP coordinates of D * L * L (Jal::getMinEigen LP part, sdpa_jordan.cpp:101-106)
   if (c->nLP <= 0) { *out = 1.0e50; return 0; }
-  double* part = c->d_red;
-  int np = (c->nLP + kRedChunk - 1) / kRedChunk;
-  hipLaunchKernelGGL(k_reduce_partial, dim3(np), dim3(256), 0, c->stream, D + c->lpOff,
-                     L + c->lpOff, (long)c->nLP, 2, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, part, np, 2,
-                     part + 2 * kMaxChunks);
-  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, part + 2 * kMaxChunks, sizeof(double),
-                                hipMemcpyDeviceToHost, c->stream));
+  RedBatch B;
+  B.n = 1;
+  B.j[0] = RedJob{D + c->lpOff, L + c->lpOff, (long)c->nLP, 2, 0};
+  double* res = c->d_red + (size_t)kMaxRed * kMaxChunks;
+  if (int e = reduce_batch(c, B, res)) return e;
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
   *out = c->h_red[0];
   return 0;

@@ -754,7 +754,7 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   if (int e = dalloc(&c->d_B, (size_t)m * m)) return e;
   if (int e = dalloc(&c->d_Binv, (size_t)((m + 63) / 64) * 4096)) return e;
   if (int e = dalloc(&c->d_Winv, (size_t)((m + 255) / 256) * 65536)) return e;
-  if (int e = dalloc(&c->d_red, 3 * 4096 + 64)) return e;
+  if (int e = dalloc(&c->d_red, (size_t)kMaxRed * 4096 + 64)) return e;
   if (int e = dalloc(&c->d_eig, 2 * (size_t)(nS + 1))) return e;
   SDPA_HIP_CHECK(hipMalloc(&c->d_info, sizeof(int) * (2 * nS + 2)));
   if (2 * nS + 2 > 64) {
