@@ -140,10 +140,6 @@ int chol_solve_f32(hipStream_t s, DenseWork* w, const float* L, int ld, int m, c
 bool chol_solve_f32_supported(int m, int ld);
 
 struct DenseWork {
-  GemmItem* d_items = nullptr;  // device array of descriptors
-  int cap_items = 0;
-  MatDesc* d_mats = nullptr;
-  int cap_mats = 0;
   double* d_dinv = nullptr;  // inverses of 64x64 diagonal blocks
   size_t cap_dinv = 0;
   double* d_vec = nullptr;   // m-vector scratch of the Schur solves
@@ -191,10 +187,19 @@ struct DenseWork {
   int ensure_trailing_events(int n);
   int ensure_panel_events(int n);
   int collect_trailing();  // call once the factorization has completed
-  int ensure_items(int n);
   int ensure_vec(size_t n);
-  int ensure_mats(int n);
   int ensure_dinv(size_t n);
+  // Device copies of small descriptor arrays (MatDesc, GemmItem), kept
+  // across calls and keyed by their bytes: an array equal to one uploaded
+  // before is not copied again (the solver's loop passes the same ones every
+  // iteration, and a copy from pageable memory holds the host until the
+  // stream reaches it).  *out: the device copy.
+  struct DescCopy {
+    std::vector<unsigned char> bytes;
+    void* d = nullptr;
+  };
+  std::vector<DescCopy> desc_pool;
+  int desc_copy(hipStream_t s, const void* h, size_t bytes, const void** out);
   void release();
 };
 

@@ -159,20 +159,30 @@ int gemm_batched(hipStream_t s, const GemmItem* d_items, int nitems, int maxM, i
   return 0;
 }
 
-int DenseWork::ensure_items(int n) {
-  if (n <= cap_items) return 0;
-  if (d_items) SDPA_HIP_CHECK(hipFree(d_items));
-  cap_items = n < 64 ? 64 : n;
-  SDPA_HIP_CHECK(hipMalloc(&d_items, sizeof(GemmItem) * cap_items));
+constexpr size_t kDescPoolMax = 64;  // copies kept before the pool starts over
+
+int DenseWork::desc_copy(hipStream_t s, const void* h, size_t bytes, const void** out) {
+  for (const DescCopy& e : desc_pool)
+    if (e.bytes.size() == bytes && std::memcmp(e.bytes.data(), h, bytes) == 0) {
+      *out = e.d;
+      return 0;
+    }
+  if (desc_pool.size() >= kDescPoolMax) {
+    // launches on either stream may still read the old copies
+    SDPA_HIP_CHECK(hipStreamSynchronize(s));
+    if (side) SDPA_HIP_CHECK(hipStreamSynchronize(side));
+    for (DescCopy& e : desc_pool) SDPA_HIP_CHECK(hipFree(e.d));
+    desc_pool.clear();
+  }
+  DescCopy e;
+  e.bytes.assign(static_cast<const unsigned char*>(h), static_cast<const unsigned char*>(h) + bytes);
+  SDPA_HIP_CHECK(hipMalloc(&e.d, bytes));
+  SDPA_HIP_CHECK(hipMemcpyAsync(e.d, h, bytes, hipMemcpyHostToDevice, s));
+  *out = e.d;
+  desc_pool.push_back(std::move(e));
   return 0;
 }
-int DenseWork::ensure_mats(int n) {
-  if (n <= cap_mats) return 0;
-  if (d_mats) SDPA_HIP_CHECK(hipFree(d_mats));
-  cap_mats = n < 64 ? 64 : n;
-  SDPA_HIP_CHECK(hipMalloc(&d_mats, sizeof(MatDesc) * cap_mats));
-  return 0;
-}
+
 int DenseWork::ensure_dinv(size_t n) {
   if (n <= cap_dinv) return 0;
   if (d_dinv) SDPA_HIP_CHECK(hipFree(d_dinv));
@@ -221,6 +231,8 @@ int DenseWork::ensure_vec(size_t n) {
 }
 void DenseWork::release() {
   if (side) (void)hipStreamSynchronize(side);
+  for (DescCopy& e : desc_pool) (void)hipFree(e.d);
+  desc_pool.clear();
   for (auto e : evt0) (void)hipEventDestroy(e);
   for (auto e : evt1) (void)hipEventDestroy(e);
   for (auto e : evq0) (void)hipEventDestroy(e);
@@ -250,21 +262,20 @@ void DenseWork::release() {
   d_ll = nullptr; cap_ll = 0; ll_max_nb = ll_max_nb32 = -1; ll_epoch = 0;
   if (d_ptag) (void)hipFree(d_ptag);
   d_ptag = nullptr; ptag_epoch = 0;
-  if (d_items) (void)hipFree(d_items);
-  if (d_mats) (void)hipFree(d_mats);
   if (d_dinv) (void)hipFree(d_dinv);
-  d_items = nullptr; d_mats = nullptr; d_dinv = nullptr;
-  cap_items = cap_mats = 0; cap_dinv = 0;
+  d_dinv = nullptr;
+  cap_dinv = 0;
 }
 
 int gemm(hipStream_t s, DenseWork* w, const double* A, int lda, const double* B, int ldb,
          double* C, int ldc, int M, int N, int K, int transA, int transB, double alpha,
          double beta, int lower_only) {
   if (M <= 0 || N <= 0) return 0;
-  if (int e = w->ensure_items(1)) return e;
   GemmItem it{A, B, C, M, N, K, lda, ldb, ldc};
-  SDPA_HIP_CHECK(hipMemcpyAsync(w->d_items, &it, sizeof(it), hipMemcpyHostToDevice, s));
-  return gemm_batched(s, w->d_items, 1, M, N, transA, transB, alpha, beta, lower_only);
+  const void* di = nullptr;
+  if (int e = w->desc_copy(s, &it, sizeof(it), &di)) return e;
+  return gemm_batched(s, static_cast<const GemmItem*>(di), 1, M, N, transA, transB, alpha, beta,
+                      lower_only);
 }
 
 // ---------------------------------------------------------------------------
@@ -1502,14 +1513,14 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
   int maxn = 0;
   for (int b = 0; b < nitems; ++b) maxn = h_items[b].n > maxn ? h_items[b].n : maxn;
   const int nslot = (maxn + TB - 1) / TB;
-  if (int e = w->ensure_mats(nitems)) return e;
   double* dinv = dinv_out;
   if (!dinv) {
     if (int e = w->ensure_dinv((size_t)nitems * nslot * TB * TB)) return e;
     dinv = w->d_dinv;
   }
-  SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats, h_items, sizeof(MatDesc) * nitems,
-                                hipMemcpyHostToDevice, s));
+  const void* dm = nullptr;
+  if (int e = w->desc_copy(s, h_items, sizeof(MatDesc) * nitems, &dm)) return e;
+  const MatDesc* d_mats = static_cast<const MatDesc*>(dm);
   SDPA_HIP_CHECK(hipMemsetAsync(d_info, 0, sizeof(int) * nitems, s));
   // outer panels of PW = 256 columns; inside, 64-column steps whose updates
   // stay within the panel; then the trailing update with K = 256.
@@ -1517,8 +1528,8 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
   const int np = (maxn + PW - 1) / PW;
   if (nitems > 1 || np <= 2) {
     for (int K0 = 0; K0 < maxn; K0 += PW) {
-      factor_panel(s, w->d_mats, nitems, maxn, K0, PW, mode, d_info, dinv, nslot);
-      launch_syrk(s, w->d_mats, d_info, nitems, maxn, K0, PW, K0 + PW, maxn);
+      factor_panel(s, d_mats, nitems, maxn, K0, PW, mode, d_info, dinv, nslot);
+      launch_syrk(s, d_mats, d_info, nitems, maxn, K0, PW, K0 + PW, maxn);
     }
   } else {
     // Look-ahead.  Panel p's trailing update is split by columns: panel
@@ -1561,7 +1572,7 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
       // the side stream, once panel p is factored: the panel-after-next's
       // columns, then the rest
       SDPA_HIP_CHECK(hipStreamWaitEvent(w->side, w->evp[p], 0));
-      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, K0 + 3 * PW,
+      launch_syrk(w->side, d_mats, d_info, 1, maxn, K0, PW, K0 + 2 * PW, K0 + 3 * PW,
                   w->side_waves);
       SDPA_HIP_CHECK(hipEventRecord(w->evn[p], w->side));
       const int nrest = maxn - (K0 + 3 * PW);
@@ -1571,7 +1582,7 @@ int potrf_batched(hipStream_t s, DenseWork* w, const MatDesc* h_items, int nitem
         if (int e = w->ensure_trailing_events(w->trail_pending + 1)) return e;
         SDPA_HIP_CHECK(hipEventRecord(w->evt0[w->trail_pending], w->side));
       }
-      launch_syrk(w->side, w->d_mats, d_info, 1, maxn, K0, PW, K0 + 3 * PW, maxn, w->side_waves);
+      launch_syrk(w->side, d_mats, d_info, 1, maxn, K0, PW, K0 + 3 * PW, maxn, w->side_waves);
       if (timed) {
         SDPA_HIP_CHECK(hipEventRecord(w->evt1[w->trail_pending], w->side));
         // algorithmic flops: 2 K per element of the lower triangle updated
@@ -1988,23 +1999,24 @@ int trtri_batched(hipStream_t s, DenseWork* w, const MatDesc* h_L, const MatDesc
   int maxn = 0;
   for (int b = 0; b < nitems; ++b) maxn = h_L[b].n > maxn ? h_L[b].n : maxn;
   const int nblk = (maxn + TB - 1) / TB;
-  if (int e = w->ensure_mats(2 * nitems)) return e;
-  SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats, h_L, sizeof(MatDesc) * nitems, hipMemcpyHostToDevice, s));
-  SDPA_HIP_CHECK(hipMemcpyAsync(w->d_mats + nitems, h_W, sizeof(MatDesc) * nitems,
-                                hipMemcpyHostToDevice, s));
+  std::vector<MatDesc> lw(h_L, h_L + nitems);
+  lw.insert(lw.end(), h_W, h_W + nitems);
+  const void* dm = nullptr;
+  if (int e = w->desc_copy(s, lw.data(), sizeof(MatDesc) * 2 * nitems, &dm)) return e;
+  const MatDesc* d_mats = static_cast<const MatDesc*>(dm);
   const double* dv = dinv_pre;
   if (dv) {
-    hipLaunchKernelGGL(k_diag_place, dim3(nblk, nitems), dim3(256), 0, s, w->d_mats,
-                       w->d_mats + nitems, nblk, dv);
+    hipLaunchKernelGGL(k_diag_place, dim3(nblk, nitems), dim3(256), 0, s, d_mats,
+                       d_mats + nitems, nblk, dv);
   } else {
     if (int e = w->ensure_dinv((size_t)nitems * nblk * TB * TB)) return e;
-    hipLaunchKernelGGL(k_diag_inv, dim3(nblk, nitems), dim3(64), 0, s, w->d_mats,
-                       w->d_mats + nitems, nblk, w->d_dinv);
+    hipLaunchKernelGGL(k_diag_inv, dim3(nblk, nitems), dim3(64), 0, s, d_mats,
+                       d_mats + nitems, nblk, w->d_dinv);
     dv = w->d_dinv;
   }
   for (int I = 1; I < nblk; ++I)
-    hipLaunchKernelGGL(k_trtri_row, dim3(I, nitems), dim3(256), 0, s, w->d_mats,
-                       w->d_mats + nitems, I, nblk, dv);
+    hipLaunchKernelGGL(k_trtri_row, dim3(I, nitems), dim3(256), 0, s, d_mats,
+                       d_mats + nitems, I, nblk, dv);
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -209,12 +209,12 @@ int min_eig(Ctx* c, double* lamX, double* lamZ) {
           return e;
       }
     } else {
-      if (int e = c->dw.ensure_items(4 * nb)) return e;
-      SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * 4 * nb,
-                                    hipMemcpyHostToDevice, c->stream));
-      if (int e = gemm_batched(c->stream, c->dw.d_items, 2 * nb, maxn, maxn, 0, 1, 1.0, 0.0, 0))
+      const void* di = nullptr;
+      if (int e = c->dw.desc_copy(c->stream, it.data(), sizeof(GemmItem) * 4 * nb, &di)) return e;
+      const GemmItem* d_items = static_cast<const GemmItem*>(di);
+      if (int e = gemm_batched(c->stream, d_items, 2 * nb, maxn, maxn, 0, 1, 1.0, 0.0, 0))
         return e;
-      if (int e = gemm_batched(c->stream, c->dw.d_items + 2 * nb, 2 * nb, maxn, maxn, 0, 0, 1.0,
+      if (int e = gemm_batched(c->stream, d_items + 2 * nb, 2 * nb, maxn, maxn, 0, 0, 1.0,
                                0.0, 0))
         return e;
     }

@@ -287,12 +287,12 @@ int triple_product(Ctx* c, const double* A, const double* B, const double* Cm, d
           return e;
       }
     } else {
-      if (int e = c->dw.ensure_items(2 * nb)) return e;
-      SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * 2 * nb,
-                                    hipMemcpyHostToDevice, c->stream));
-      if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
+      const void* di = nullptr;
+      if (int e = c->dw.desc_copy(c->stream, it.data(), sizeof(GemmItem) * 2 * nb, &di)) return e;
+      const GemmItem* d_items = static_cast<const GemmItem*>(di);
+      if (int e = gemm_batched(c->stream, d_items, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
         return e;
-      if (int e = gemm_batched(c->stream, c->dw.d_items + nb, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
+      if (int e = gemm_batched(c->stream, d_items + nb, nb, maxn, maxn, 0, 0, 1.0, 0.0, 0))
         return e;
     }
   }
@@ -473,10 +473,10 @@ int inverse_chol(Ctx* c) {
       if (int e = dgemm(c->stream, 1, 0, g.M, g.N, g.K, 1.0, g.A, g.lda, 0, g.B, g.ldb, 0, 0.0, g.C, g.ldc, 0, 1))
         return e;
     } else {
-      if (int e = c->dw.ensure_items(nb)) return e;
-      SDPA_HIP_CHECK(hipMemcpyAsync(c->dw.d_items, it.data(), sizeof(GemmItem) * nb,
-                                    hipMemcpyHostToDevice, c->stream));
-      if (int e = gemm_batched(c->stream, c->dw.d_items, nb, maxn, maxn, 1, 0, 1.0, 0.0, 0))
+      const void* di = nullptr;
+      if (int e = c->dw.desc_copy(c->stream, it.data(), sizeof(GemmItem) * nb, &di)) return e;
+      if (int e = gemm_batched(c->stream, static_cast<const GemmItem*>(di), nb, maxn, maxn, 1, 0,
+                               1.0, 0.0, 0))
         return e;
     }
   }
