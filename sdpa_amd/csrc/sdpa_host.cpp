@@ -1021,16 +1021,13 @@ void SDPA::solve() {
       thetaP = std::fabs((1.0 - alphaP) * thetaP);
       thetaD = std::fabs((1.0 - alphaD) * thetaD);
     }
-    double o[4];
+    double o[5];
     {
-      int codes[4] = {D_XZ, D_CX, D_BY, isInitPoint_ ? D_X0Z : D_TRZ};
-      check(sdpa_gpu_dots(g, 4, codes, o), g, "dots");
+      int codes[5] = {D_XZ, D_CX, D_BY, isInitPoint_ ? D_X0Z : D_TRZ,
+                      isInitPoint_ ? D_XZ0 : D_TRX};
+      check(sdpa_gpu_dots(g, 5, codes, o), g, "dots");
     }
-    double o2;
-    {
-      int code = isInitPoint_ ? D_XZ0 : D_TRX;
-      check(sdpa_gpu_dots(g, 1, &code, &o2), g, "dots");
-    }
+    const double o2 = o[4];
     muCur = o[0] / nDim;
     check(sdpa_gpu_residuals(g, &normP, &normD), g, "residuals");
     thetaP = (initNormP > P.epsilonDash * 1.0e-2) ? normP / initNormP : 0.0;
