@@ -132,6 +132,10 @@ struct Ctx {
   int* d_info = nullptr;   // factorization info (one per batch item)
   double* d_red = nullptr;  // reduction partials (kMaxRed x 4096), then the results (64)
   double* h_red = nullptr;  // pinned host mirror of reduction results
+  // pinned staging ring of the loader's dense-pool upload (kStageBufs
+  // buffers of kStageBytes; each one's last copy recorded in ev_stage)
+  void* h_stage[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_stage[4] = {nullptr, nullptr, nullptr, nullptr};
   int* h_info = nullptr;    // pinned
   double* d_eig = nullptr;  // per-item eigen results
   // mixed precision (made on first use): the FP32 factor of S B S, the

@@ -71,6 +71,34 @@ void par_chunks(int nt, long n, const F& f) {
   for (auto& x : th) x.join();
 }
 
+// One column of the dense pool: A_k of the block (n x n, both triangles) from
+// its entries (SparseMatrix::changeToDense, sdpa_struct.cpp:510-555: a later
+// duplicate entry overwrites an earlier one).
+void fill_dense_column(double* A, int k, int n, const std::vector<int>& estart,
+                       const std::vector<int>& ecnt, const int* er, const int* ec,
+                       const double* ev) {
+  std::fill(A, A + (size_t)n * n, 0.0);
+  for (int e = estart[k]; e < estart[k] + ecnt[k]; ++e) {
+    const int r = er[e], cc = ec[e];
+    A[r + (size_t)n * cc] = ev[e];
+    if (r != cc) A[cc + (size_t)n * r] = ev[e];
+  }
+}
+
+constexpr int kStageBufs = 4;  // at most the size of Ctx::h_stage
+constexpr size_t kStageBytes = (size_t)32 << 20;
+
+// the pinned staging ring (allocated once per context)
+int ensure_stage(Ctx* c) {
+  if (c->h_stage[0]) return 0;
+  for (int b = 0; b < kStageBufs; ++b) {
+    SDPA_HIP_CHECK(hipHostMalloc(&c->h_stage[b], kStageBytes, hipHostMallocDefault));
+    SDPA_HIP_CHECK(hipEventCreateWithFlags(&c->ev_stage[b], hipEventDisableTiming));
+    SDPA_HIP_CHECK(hipEventRecord(c->ev_stage[b], c->stream));
+  }
+  return 0;
+}
+
 int dalloc(double** p, size_t n) {
   if (*p) SDPA_HIP_CHECK(hipFree(*p));
   *p = nullptr;
@@ -170,6 +198,10 @@ int sdpa_gpu_destroy(sdpa_gpu_ctx* h) {
   c->dw.release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   if (c->h_red) (void)hipHostFree(c->h_red);
+  for (int b = 0; b < kStageBufs; ++b) {
+    if (c->h_stage[b]) (void)hipHostFree(c->h_stage[b]);
+    if (c->ev_stage[b]) (void)hipEventDestroy(c->ev_stage[b]);
+  }
   if (c->h_info) (void)hipHostFree(c->h_info);
   (void)hipStreamDestroy(c->stream);
   delete h;
@@ -483,22 +515,38 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
       // sdpa_struct.cpp:510-555: a later duplicate entry overwrites an
       // earlier one), the columns filled by host threads
       const size_t nn2 = (size_t)n * n;
-      std::unique_ptr<double[]> pool(new double[nn2 * T.ndense]);
-      double* pl = pool.get();
-      const int ntp = std::min(T.ndense, loader_threads((long)(nn2 * T.ndense)));
-      par_chunks(ntp, T.ndense, [&](int, long p0, long p1) {
-        for (long p = p0; p < p1; ++p) {
-          const int k = dl[p];
-          double* A = pl + nn2 * p;
-          std::fill(A, A + nn2, 0.0);
-          for (int e = estart[k]; e < estart[k] + ecnt[k]; ++e) {
-            const int r = h_er[e], cc = h_ec[e];
-            A[r + (size_t)n * cc] = h_ev[e];
-            if (r != cc) A[cc + (size_t)n * r] = h_ev[e];
-          }
+      if (T.d_dense) SDPA_HIP_CHECK(hipFree(T.d_dense));
+      T.d_dense = nullptr;
+      SDPA_HIP_CHECK(hipMalloc((void**)&T.d_dense, sizeof(double) * nn2 * T.ndense));
+      // columns filled into the pinned staging ring chunk by chunk, each
+      // chunk's copy running while the next one is filled
+      if (int e = ensure_stage(c)) return e;
+      const long per = std::max<long>(1L, (long)(kStageBytes / (sizeof(double) * nn2)));
+      if (nn2 * sizeof(double) > kStageBytes) {
+        // a column larger than a staging buffer: one pageable column at a time
+        std::vector<double> A(nn2);
+        for (int p = 0; p < T.ndense; ++p) {
+          fill_dense_column(A.data(), dl[p], n, estart, ecnt, h_er, h_ec, h_ev);
+          SDPA_HIP_CHECK(hipMemcpy(T.d_dense + nn2 * p, A.data(), sizeof(double) * nn2,
+                                   hipMemcpyHostToDevice));
         }
-      });
-      if (int e = upload_raw(c, &T.d_dense, pl, nn2 * T.ndense)) return e;
+      } else {
+        int buf = 0;
+        for (long p0 = 0; p0 < T.ndense; p0 += per, buf = (buf + 1) % kStageBufs) {
+          const long p1 = std::min<long>(T.ndense, p0 + per);
+          SDPA_HIP_CHECK(hipEventSynchronize(c->ev_stage[buf]));  // its previous copy is done
+          double* st = static_cast<double*>(c->h_stage[buf]);
+          const int ntp = (int)std::min<long>(p1 - p0, loader_threads((long)(nn2 * (p1 - p0))));
+          par_chunks(ntp, p1 - p0, [&](int, long q0, long q1) {
+            for (long q = q0; q < q1; ++q)
+              fill_dense_column(st + nn2 * q, dl[p0 + q], n, estart, ecnt, h_er, h_ec, h_ev);
+          });
+          SDPA_HIP_CHECK(hipMemcpyAsync(T.d_dense + nn2 * p0, st, sizeof(double) * nn2 * (p1 - p0),
+                                        hipMemcpyHostToDevice, c->stream));
+          SDPA_HIP_CHECK(hipEventRecord(c->ev_stage[buf], c->stream));
+        }
+        SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+      }
     }
   }
   if (int e = arena.commit(&c->d_blkarena)) return e;
