@@ -134,6 +134,21 @@ int sdpa_gpu_dots(sdpa_gpu_ctx* ctx, int nq, const int* codes, double* out);
 /* Solutions::update's step (sdpa_dataset.cpp:151-158). */
 int sdpa_gpu_update(sdpa_gpu_ctx* ctx, double alphaP, double alphaD);
 
+/* Two calls' work with one host round trip (no counterpart in the reference;
+ * for loops that want fewer synchronizations):
+ * sdpa_gpu_min_eig_dots = sdpa_gpu_min_eig then sdpa_gpu_dots (nq >= 0 codes)
+ * (StepLength::MehrotraCorrector's estimates and the inner products of its
+ * rules, sdpa_parts.cpp:493-625, called at sdpa_solve.cpp:155);
+ * sdpa_gpu_update_end = sdpa_gpu_inv_chol, sdpa_gpu_dots and
+ * sdpa_gpu_residuals after an update (Solutions::update's computeInverse,
+ * sdpa_solve.cpp:167, then mu, the residuals and SolveInfo::update,
+ * sdpa_solve.cpp:179-190): returns 1 (outputs not set) if an X/Z
+ * factorization fails; the inverses are then not meaningful either. */
+int sdpa_gpu_min_eig_dots(sdpa_gpu_ctx* ctx, double* lamX, double* lamZ, int nq, const int* codes,
+                          double* out);
+int sdpa_gpu_update_end(sdpa_gpu_ctx* ctx, int nq, const int* codes, double* out,
+                        double* normPrimalVec, double* normDualMat);
+
 /* Buffer access for tests and callers that keep their own copies.
  * which: 0 X, 1 Z, 2 y, 3 Zinv, 4 invCholX, 5 invCholZ, 6 Dx, 7 Dz, 8 dy,
  *        9 gVec, 10 bMat (m*m), 11 primal residual, 12 dual residual.

@@ -78,6 +78,10 @@ def lib():
         L.sdpa_gpu_make_direction.argtypes = [V, C.c_int]
         L.sdpa_gpu_dots.argtypes = [V, C.c_int, _ip, _dp]
         L.sdpa_gpu_update.argtypes = [V, C.c_double, C.c_double]
+        L.sdpa_gpu_min_eig_dots.argtypes = [V, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.c_int, _ip, _dp]
+        L.sdpa_gpu_update_end.argtypes = [V, C.c_int, _ip, _dp, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
         L.sdpa_gpu_get_buffer.argtypes = [V, C.c_int, _dp, C.c_long]
         L.sdpa_gpu_set_buffer.argtypes = [V, C.c_int, _dp, C.c_long]
         L.sdpa_gpu_timing.argtypes = [V, C.c_int, V]
@@ -238,6 +242,27 @@ class GpuContext:
 
     def update(self, ap, ad):
         return self._chk(lib().sdpa_gpu_update(self.h, ap, ad), "update")
+
+    def min_eig_dots(self, *names):
+        """min_eig() and dots(*names) with one host round trip:
+        (lamX, lamZ, products)."""
+        codes = np.array([DOT[n] for n in names], np.int32)
+        out = np.zeros(max(1, len(codes)))
+        a, b = C.c_double(), C.c_double()
+        self._chk(lib().sdpa_gpu_min_eig_dots(self.h, C.byref(a), C.byref(b), len(codes), codes,
+                                              out), "min_eig_dots")
+        return a.value, b.value, out[: len(codes)]
+
+    def update_end(self, *names):
+        """inv_chol(), dots(*names) and residuals() with one host round trip:
+        (status, products, normP, normD); status 1 if an X/Z factorization
+        failed (the other outputs are then not set)."""
+        codes = np.array([DOT[n] for n in names], np.int32)
+        out = np.zeros(max(1, len(codes)))
+        a, b = C.c_double(), C.c_double()
+        r = self._chk(lib().sdpa_gpu_update_end(self.h, len(codes), codes, out, C.byref(a),
+                                                C.byref(b)), "update_end")
+        return r, out[: len(codes)], a.value, b.value
 
     def timing(self, enable=-1):
         out = np.zeros(10)

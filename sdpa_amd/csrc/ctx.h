@@ -132,6 +132,7 @@ struct Ctx {
   int* d_info = nullptr;   // factorization info (one per batch item)
   double* d_red = nullptr;  // reduction partials (kMaxRed x 4096), then the results (64)
   double* h_red = nullptr;  // pinned host mirror of reduction results
+  double* h_eig = nullptr;  // pinned: the step-length estimates per SDP block (2 nSDP)
   // pinned staging ring of the loader's dense-pool upload (kStageBufs
   // buffers of kStageBytes; each one's last copy recorded in ev_stage)
   void* h_stage[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -186,7 +187,18 @@ int inverse_chol(Ctx* c);
 int sum_yA(Ctx* c, const double* d_yv, double* d_out);  // out = sum_i y_i A_i (packed)
 int a_dot(Ctx* c, const double* d_D, double* d_out);    // out_i = A_i . D
 int triple_product(Ctx* c, const double* A, const double* B, const double* C, double* R);
-int lp_min_step(Ctx* c, const double* D, const double* L, double* out);
+// launches whose scalar results are read after one later synchronization:
+// result slots of d_red's result area (kRedResults doubles, mirrored in h_red)
+constexpr int kRedResults = 64;
+constexpr int kSlotResid = 40;  // residual max norms (2)
+constexpr int kSlotLpMin = 44;  // LP minima of the step-length estimates (2)
+int dots_enqueue(Ctx* c, int nq, const int* codes, int slot0);
+int residuals_enqueue(Ctx* c, int slot);
+int lp_min_enqueue(Ctx* c, int slot);
+int inverse_chol_enqueue(Ctx* c);
+int inverse_chol_status(const Ctx* c);
+int min_eig_enqueue(Ctx* c, int lp_slot);
+void min_eig_finish(const Ctx* c, int lp_slot, double* lamX, double* lamZ);
 // eig.hip
 int min_eig(Ctx* c, double* lamX, double* lamZ);
 // gemm.hip: C_b = alpha op(A_b) op(B_b) + beta C_b for b < batch (FP64 MFMA,

@@ -180,11 +180,10 @@ __global__ __launch_bounds__(NT) void k_lanczos(const double* __restrict__ Ms, l
   if (tid == 0) out[which * nblk + q] = mn - fabs(error * beta);
 }
 
-int lp_min_step(Ctx* c, const double* D, const double* L, double* out);
-
-int min_eig(Ctx* c, double* lamX, double* lamZ) {
+// The launches of min_eig and the copies of their results (per SDP block to
+// c->h_eig, the LP minima to c->h_red[lp_slot..]); no synchronization.
+int min_eig_enqueue(Ctx* c, int lp_slot) {
   const int nb = c->nSDP;
-  double mX = 1.0e50, mZ = 1.0e50;
   if (nb > 0) {
     for (int q = 0; q < nb; ++q)
       if (c->sdpN[q] > kMaxN) return -7;
@@ -232,24 +231,35 @@ int min_eig(Ctx* c, double* lamX, double* lamZ) {
       hipLaunchKernelGGL((k_lanczos<kLzThreads, 0>), dim3(nb, 2), dim3(kLzThreads), 0, c->stream,
                          c->d_w1, c->lpOff, c->d_sdpOff, c->d_sdpN, nb, c->d_eig);
     SDPA_HIP_CHECK(hipGetLastError());
-    std::vector<double> h(2 * nb);
-    SDPA_HIP_CHECK(hipMemcpyAsync(h.data(), c->d_eig, sizeof(double) * 2 * nb,
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->h_eig, c->d_eig, sizeof(double) * 2 * nb,
                                   hipMemcpyDeviceToHost, c->stream));
-    SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
-    for (int q = 0; q < nb; ++q) {
-      mX = h[q] < mX ? h[q] : mX;
-      mZ = h[nb + q] < mZ ? h[nb + q] : mZ;
-    }
+  }
+  if (c->nLP > 0)
+    if (int e = lp_min_enqueue(c, lp_slot)) return e;
+  return 0;
+}
+
+// after the stream has passed min_eig_enqueue: the minima over the blocks
+void min_eig_finish(const Ctx* c, int lp_slot, double* lamX, double* lamZ) {
+  const int nb = c->nSDP;
+  double mX = 1.0e50, mZ = 1.0e50;
+  for (int q = 0; q < nb; ++q) {
+    mX = c->h_eig[q] < mX ? c->h_eig[q] : mX;
+    mZ = c->h_eig[nb + q] < mZ ? c->h_eig[nb + q] : mZ;
   }
   if (c->nLP > 0) {
-    double a, b;
-    if (int e = lp_min_step(c, c->d_Dx, c->d_invCholX, &a)) return e;
-    if (int e = lp_min_step(c, c->d_Dz, c->d_invCholZ, &b)) return e;
+    const double a = c->h_red[lp_slot], b = c->h_red[lp_slot + 1];
     mX = a < mX ? a : mX;
     mZ = b < mZ ? b : mZ;
   }
   *lamX = mX;
   *lamZ = mZ;
+}
+
+int min_eig(Ctx* c, double* lamX, double* lamZ) {
+  if (int e = min_eig_enqueue(c, kSlotLpMin)) return e;
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  min_eig_finish(c, kSlotLpMin, lamX, lamZ);
   return 0;
 }
 

@@ -317,31 +317,38 @@ static int triple(Ctx* c, const double* A, const double* B, const double* Cm, do
   return 0;
 }
 
-int residuals(Ctx* c, double* normP, double* normD) {
-  // p = b - A(X) ; D = C - sum y A - Z
+// p = b - A(X), D = C - sum y A - Z and their max norms into the result slots
+// slot, slot + 1, copied to c->h_red[slot..]; no synchronization
+int residuals_enqueue(Ctx* c, int slot) {
   if (int e = a_dot(c, c->d_X, c->d_pvec)) return e;
   if (int e = axpbypcz(c, c->m, -1.0, c->d_pvec, 1.0, c->d_b, 0.0, nullptr, c->d_pvec)) return e;
   if (int e = sum_yA(c, c->d_y, c->d_dmat)) return e;
   if (int e = axpbypcz(c, c->total, -1.0, c->d_dmat, 1.0, c->d_C, -1.0, c->d_Z, c->d_dmat))
     return e;
-  // max norms
   RedBatch B;
   B.n = 2;
-  B.j[0] = RedJob{c->d_pvec, nullptr, (long)c->m, 1, 0};
-  B.j[1] = RedJob{c->d_dmat, nullptr, c->total, 1, 1};
+  B.j[0] = RedJob{c->d_pvec, nullptr, (long)c->m, 1, slot};
+  B.j[1] = RedJob{c->d_dmat, nullptr, c->total, 1, slot + 1};
   double* res = c->d_red + (size_t)kMaxRed * kMaxChunks;
   if (int e = reduce_batch(c, B, res)) return e;
-  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, res, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                                c->stream));
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red + slot, res + slot, 2 * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+int residuals(Ctx* c, double* normP, double* normD) {
+  if (int e = residuals_enqueue(c, 0)) return e;
   SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
   *normP = c->h_red[0];
   *normD = c->h_red[1];
   return 0;
 }
 
-int compute_dots(Ctx* c, int nq, const int* codes, double* out) {
+// the inner products of codes[0..nq) into the result slots slot0.., copied to
+// c->h_red[slot0..]; no synchronization
+int dots_enqueue(Ctx* c, int nq, const int* codes, int slot0) {
   hipStream_t s = c->stream;
-  if (nq > 64) return -2;
+  if (nq <= 0 || slot0 < 0 || slot0 + nq > kRedResults) return -2;
   double* res = c->d_red + (size_t)kMaxRed * kMaxChunks;
   RedBatch B;
   B.n = 0;
@@ -362,22 +369,28 @@ int compute_dots(Ctx* c, int nq, const int* codes, double* out) {
       case DOT_TRX:
       case DOT_TRZ: {
         hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, codes[k] == DOT_TRX ? c->d_X : c->d_Z,
-                           c->d_sdpOff, c->d_sdpN, c->nSDP, c->lpOff, c->nLP, res + k);
+                           c->d_sdpOff, c->d_sdpN, c->nSDP, c->lpOff, c->nLP, res + slot0 + k);
         continue;
       }
       default:
         return -2;
     }
     if (!x || !y) return -3;
-    B.j[B.n++] = RedJob{x, y, len, 0, k};
+    B.j[B.n++] = RedJob{x, y, len, 0, slot0 + k};
     if (B.n == kMaxRed) {  // a full batch (the partial slots are reused by the next)
       if (int e = reduce_batch(c, B, res)) return e;
       B.n = 0;
     }
   }
   if (int e = reduce_batch(c, B, res)) return e;
-  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, res, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
-  SDPA_HIP_CHECK(hipStreamSynchronize(s));
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red + slot0, res + slot0, sizeof(double) * nq,
+                                hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+int compute_dots(Ctx* c, int nq, const int* codes, double* out) {
+  if (int e = dots_enqueue(c, nq, codes, 0)) return e;
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
   for (int k = 0; k < nq; ++k) out[k] = c->h_red[k];
   return 0;
 }
@@ -431,10 +444,13 @@ int update_point(Ctx* c, double alphaP, double alphaD) {
   return axpbypcz(c, c->total, 1.0, c->d_Z, alphaD, c->d_Dz, 0.0, nullptr, c->d_Z);
 }
 
-int inverse_chol(Ctx* c) {
+// The launches of inverse_chol and the copy of its status words (one per
+// X/Z block, then the LP flag) to c->h_info; no synchronization.  The
+// inverses are formed whether or not a factorization failed (on a failed
+// one they hold no meaning; the status says so).
+int inverse_chol_enqueue(Ctx* c) {
   // L = chol(X), invCholX = L^-1 ; L = chol(Z), invCholZ = L^-1, Zinv = L^-T L^-1
   const int nb = c->nSDP;
-  int fail = 0;
   if (nb > 0) {
     SDPA_HIP_CHECK(hipMemcpyAsync(c->d_LX, c->d_X, sizeof(double) * c->lpOff,
                                   hipMemcpyDeviceToDevice, c->stream));
@@ -450,12 +466,6 @@ int inverse_chol(Ctx* c) {
       W[nb + q] = MatDesc{c->d_invCholZ + o, n, n};
     }
     if (int e = potrf_batched(c->stream, &c->dw, L.data(), 2 * nb, 0, c->d_info)) return e;
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, c->d_info, sizeof(int) * 2 * nb,
-                                  hipMemcpyDeviceToHost, c->stream));
-    SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
-    for (int b = 0; b < 2 * nb; ++b)
-      if (c->h_info[b] != 0) fail = 1;
-    if (fail) return 1;
     // reuse the 64x64 diagonal inverses the factorization left in dw.d_dinv
     if (int e = trtri_batched(c->stream, &c->dw, L.data(), W.data(), 2 * nb, c->dw.d_dinv))
       return e;
@@ -485,25 +495,41 @@ int inverse_chol(Ctx* c) {
     SDPA_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
     hipLaunchKernelGGL(k_lp_inverse, grid1(c->nLP), dim3(256), 0, c->stream, c->d_X, c->d_Z,
                        c->d_invCholX, c->d_invCholZ, c->d_Zinv, c->lpOff, c->nLP, bad);
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
-    if (c->h_info[0]) return 1;
   }
+  const int nstat = 2 * nb + (c->nLP > 0 ? 1 : 0);
+  if (nstat > 0)
+    SDPA_HIP_CHECK(hipMemcpyAsync(c->h_info, c->d_info, sizeof(int) * nstat,
+                                  hipMemcpyDeviceToHost, c->stream));
   SDPA_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-int lp_min_step(Ctx* c, const double* D, const double* L, double* out) {
-  // min over LP coordinates of D * L * L (Jal::getMinEigen LP part, sdpa_jordan.cpp:101-106)
-  if (c->nLP <= 0) { *out = 1.0e50; return 0; }
+// after the stream has passed inverse_chol_enqueue: 1 if a factorization failed
+int inverse_chol_status(const Ctx* c) {
+  const int nstat = 2 * c->nSDP + (c->nLP > 0 ? 1 : 0);
+  for (int b = 0; b < nstat; ++b)
+    if (c->h_info[b] != 0) return 1;
+  return 0;
+}
+
+int inverse_chol(Ctx* c) {
+  if (int e = inverse_chol_enqueue(c)) return e;
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return inverse_chol_status(c);
+}
+
+// min over the LP coordinates of Dx * L_X^2 and Dz * L_Z^2 (Jal::getMinEigen's LP
+// part, sdpa_jordan.cpp:101-106) into the result slots slot, slot + 1, copied
+// to c->h_red[slot..]; no synchronization (nLP > 0)
+int lp_min_enqueue(Ctx* c, int slot) {
   RedBatch B;
-  B.n = 1;
-  B.j[0] = RedJob{D + c->lpOff, L + c->lpOff, (long)c->nLP, 2, 0};
+  B.n = 2;
+  B.j[0] = RedJob{c->d_Dx + c->lpOff, c->d_invCholX + c->lpOff, (long)c->nLP, 2, slot};
+  B.j[1] = RedJob{c->d_Dz + c->lpOff, c->d_invCholZ + c->lpOff, (long)c->nLP, 2, slot + 1};
   double* res = c->d_red + (size_t)kMaxRed * kMaxChunks;
   if (int e = reduce_batch(c, B, res)) return e;
-  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red, res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
-  *out = c->h_red[0];
+  SDPA_HIP_CHECK(hipMemcpyAsync(c->h_red + slot, res + slot, 2 * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
   return 0;
 }
 

@@ -198,6 +198,7 @@ int sdpa_gpu_destroy(sdpa_gpu_ctx* h) {
   c->dw.release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   if (c->h_red) (void)hipHostFree(c->h_red);
+  if (c->h_eig) (void)hipHostFree(c->h_eig);
   for (int b = 0; b < kStageBufs; ++b) {
     if (c->h_stage[b]) (void)hipHostFree(c->h_stage[b]);
     if (c->ev_stage[b]) (void)hipEventDestroy(c->ev_stage[b]);
@@ -804,6 +805,10 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
   if (int e = dalloc(&c->d_Winv, (size_t)((m + 255) / 256) * 65536)) return e;
   if (int e = dalloc(&c->d_red, (size_t)kMaxRed * 4096 + 64)) return e;
   if (int e = dalloc(&c->d_eig, 2 * (size_t)(nS + 1))) return e;
+  if (c->h_eig) (void)hipHostFree(c->h_eig);
+  c->h_eig = nullptr;
+  SDPA_HIP_CHECK(hipHostMalloc((void**)&c->h_eig, sizeof(double) * 2 * (nS + 1),
+                               hipHostMallocDefault));
   SDPA_HIP_CHECK(hipMalloc(&c->d_info, sizeof(int) * (2 * nS + 2)));
   if (2 * nS + 2 > 64) {
     (void)hipHostFree(c->h_info);
@@ -1076,18 +1081,65 @@ int sdpa_gpu_make_direction(sdpa_gpu_ctx* h, int infeasible) {
   return r;
 }
 
+// after a synchronization: the failure flag of the last cooperative Schur
+// solve (copied to pinned memory behind it) is current
+static int coop_solve_check(sdpa_gpu_ctx* h) {
+  if (h->c.dw.h_err && *h->c.dw.h_err) {
+    if (int e = h->c.dw.clear_err(h->c.stream)) return e;
+    return fail(&h->c, -60, "a cooperative Schur solve gave up waiting (work-groups not co-resident)");
+  }
+  return 0;
+}
+
+int sdpa_gpu_min_eig_dots(sdpa_gpu_ctx* h, double* lamX, double* lamZ, int nq, const int* codes,
+                          double* out) {
+  if (!h || !h->c.loaded) return -1;
+  if (nq < 0 || nq > 32) return -2;
+  Ctx* c = &h->c;
+  tic(c);
+  if (int e = min_eig_enqueue(c, kSlotLpMin)) return e;
+  c->t_eig += toc(c);
+  tic(c);
+  if (nq > 0)
+    if (int e = dots_enqueue(c, nq, codes, 0)) return e;
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->t_dots += toc(c);
+  min_eig_finish(c, kSlotLpMin, lamX, lamZ);
+  for (int k = 0; k < nq; ++k) out[k] = c->h_red[k];
+  return coop_solve_check(h);
+}
+
+int sdpa_gpu_update_end(sdpa_gpu_ctx* h, int nq, const int* codes, double* out, double* nP,
+                        double* nD) {
+  if (!h || !h->c.loaded) return -1;
+  if (nq < 0 || nq > 32) return -2;
+  Ctx* c = &h->c;
+  tic(c);
+  if (int e = inverse_chol_enqueue(c)) return e;
+  c->t_inv += toc(c);
+  tic(c);
+  if (nq > 0)
+    if (int e = dots_enqueue(c, nq, codes, 0)) return e;
+  c->t_dots += toc(c);
+  tic(c);
+  if (int e = residuals_enqueue(c, kSlotResid)) return e;
+  SDPA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->t_res += toc(c);
+  if (inverse_chol_status(c)) return 1;
+  for (int k = 0; k < nq; ++k) out[k] = c->h_red[k];
+  *nP = c->h_red[kSlotResid];
+  *nD = c->h_red[kSlotResid + 1];
+  return coop_solve_check(h);
+}
+
 int sdpa_gpu_dots(sdpa_gpu_ctx* h, int nq, const int* codes, double* out) {
   if (!h || !h->c.loaded) return -1;
   if (nq <= 0 || nq > 32) return -2;
   tic(&h->c);
   const int r = compute_dots(&h->c, nq, codes, out);
   h->c.t_dots += toc(&h->c);
-  // compute_dots synchronizes the stream: the failure flag of the last
-  // cooperative Schur solve (copied to pinned memory behind it) is current
-  if (r == 0 && h->c.dw.h_err && *h->c.dw.h_err) {
-    if (int e = h->c.dw.clear_err(h->c.stream)) return e;
-    return fail(&h->c, -60, "a cooperative Schur solve gave up waiting (work-groups not co-resident)");
-  }
+  // compute_dots synchronizes the stream
+  if (r == 0) return coop_solve_check(h);
   return r;
 }
 

@@ -962,16 +962,17 @@ void SDPA::solve() {
     // StepLength::MehrotraCorrector
     const clk::time_point tStepC = clk::now();
     double lamX, lamZ;
-    check(sdpa_gpu_min_eig(g, &lamX, &lamZ), g, "min_eig");
+    {
+      // the eigenvalue estimates and the inner products of the rules below
+      // with one host round trip
+      int codes[7] = {D_CDX, D_BDY, D_XDZ, D_DXZ, D_DXDZ, D_BY, D_CX};
+      check(sdpa_gpu_min_eig_dots(g, &lamX, &lamZ, 7, codes, q), g, "min_eig_dots");
+    }
     const double alphaBD = 100.0;
     alphaP = (-lamX > 1.0 / alphaBD) ? -1.0 / lamX : alphaBD;
     alphaD = (-lamZ > 1.0 / alphaBD) ? -1.0 / lamZ : alphaBD;
     alphaP *= P.gammaStar;
     alphaD *= P.gammaStar;
-    {
-      int codes[7] = {D_CDX, D_BDY, D_XDZ, D_DXZ, D_DXDZ, D_BY, D_CX};
-      check(sdpa_gpu_dots(g, 7, codes, q), g, "dots");
-    }
     if (phase == noINFO || phase == dFEAS) {
       if (alphaP > 1.0) alphaP = 1.0;
     } else if (q[0] > 0.0) {
@@ -1014,22 +1015,22 @@ void SDPA::solve() {
     // Solutions::update (sdpa_dataset.cpp:145-171)
     check(sdpa_gpu_update(g, alphaP, alphaD), g, "update");
     if (alphaP < 1.0e-4 && alphaD < 1.0e-4) { failure_ = 2; break; }
-    int inv = sdpa_gpu_inv_chol(g);
-    check(inv, g, "inv_chol");
-    if (inv > 0) { failure_ = 3; break; }
-    if (!affine) {
-      thetaP = std::fabs((1.0 - alphaP) * thetaP);
-      thetaD = std::fabs((1.0 - alphaD) * thetaD);
-    }
+    // computeInverse, the inner products and the residual norms with one
+    // host round trip
     double o[5];
     {
       int codes[5] = {D_XZ, D_CX, D_BY, isInitPoint_ ? D_X0Z : D_TRZ,
                       isInitPoint_ ? D_XZ0 : D_TRX};
-      check(sdpa_gpu_dots(g, 5, codes, o), g, "dots");
+      const int inv = sdpa_gpu_update_end(g, 5, codes, o, &normP, &normD);
+      check(inv, g, "update_end");
+      if (inv > 0) { failure_ = 3; break; }
+    }
+    if (!affine) {
+      thetaP = std::fabs((1.0 - alphaP) * thetaP);
+      thetaD = std::fabs((1.0 - alphaD) * thetaD);
     }
     const double o2 = o[4];
     muCur = o[0] / nDim;
-    check(sdpa_gpu_residuals(g, &normP, &normD), g, "residuals");
     thetaP = (initNormP > P.epsilonDash * 1.0e-2) ? normP / initNormP : 0.0;
     thetaD = (initNormD > P.epsilonDash * 1.0e-2) ? normD / initNormD : 0.0;
     // SolveInfo::update (sdpa_parts.cpp:914-1060)
