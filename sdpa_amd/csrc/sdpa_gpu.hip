@@ -595,19 +595,41 @@ int sdpa_gpu_load_problem(sdpa_gpu_ctx* h, int m, int nBlock, const int* blockSt
         pts.push_back(PT{cp1[e], k, cv[e]});
         if (cp2[e] != cp1[e]) pts.push_back(PT{cp2[e], k, cv[e]});
       }
-    std::stable_sort(pts.begin(), pts.end(), [](const PT& a, const PT& b) { return a.p < b.p; });
+    // grouped by position, in (constraint, entry) order within a position
     std::vector<long> ppos, pptr(1, 0);
     std::vector<int> pcon;
     std::vector<double> pval;
-    for (size_t t = 0; t < pts.size(); ++t) {
-      if (t == 0 || pts[t].p != pts[t - 1].p) {
-        if (t > 0) pptr.push_back((long)t);
-        ppos.push_back(pts[t].p);
+    const size_t npt = pts.size();
+    if ((size_t)c->total <= 8 * npt + ((size_t)1 << 20) && npt < ((size_t)1 << 31)) {
+      // a counting sort over the packed positions (stable)
+      std::vector<int> start((size_t)c->total + 1, 0);
+      for (const PT& x : pts) ++start[x.p + 1];
+      for (long q = 0; q < c->total; ++q) start[q + 1] += start[q];
+      std::vector<int> at(start.begin(), start.end() - 1);
+      pcon.resize(npt);
+      pval.resize(npt);
+      for (const PT& x : pts) {
+        const int w = at[x.p]++;
+        pcon[w] = x.con;
+        pval[w] = x.v;
       }
-      pcon.push_back(pts[t].con);
-      pval.push_back(pts[t].v);
+      for (long q = 0; q < c->total; ++q)
+        if (start[q + 1] > start[q]) {
+          ppos.push_back(q);
+          pptr.push_back(start[q + 1]);
+        }
+    } else {
+      std::stable_sort(pts.begin(), pts.end(), [](const PT& a, const PT& b) { return a.p < b.p; });
+      for (size_t t = 0; t < npt; ++t) {
+        if (t == 0 || pts[t].p != pts[t - 1].p) {
+          if (t > 0) pptr.push_back((long)t);
+          ppos.push_back(pts[t].p);
+        }
+        pcon.push_back(pts[t].con);
+        pval.push_back(pts[t].v);
+      }
+      if (npt) pptr.push_back((long)npt);
     }
-    if (!pts.empty()) pptr.push_back((long)pts.size());
     c->npos = (long)ppos.size();
     if (int e = upload(c, &c->d_ppos, ppos)) return e;
     if (int e = upload(c, &c->d_pptr, pptr)) return e;
