@@ -95,34 +95,47 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmItem* __restrict__ items
   v4d acc[2][2];
   acc_zero(acc);
   const int tid = threadIdx.x;
-  for (int k0 = 0; k0 < it.K; k0 += BK) {
-    // A tile: op(A)[bm + r][k0 + k], r < 64, k < 16  -> As[r + 64*k]
+  // this thread's elements of the A tile, op(A)[bm + r][k0 + k] (r < 64,
+  // k < 16) -> As[r + 64 k], and of the B tile, op(B)[k0 + k][bn + c] ->
+  // Bs[k + 16 c]; the next step's are loaded before the current step's
+  // product, so the loads overlap it
+  auto load_tiles = [&](int k0, double (&pa)[4], double (&pb)[4]) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      int idx = tid + 256 * e;  // 0..1023
+      const int idx = tid + 256 * e;  // 0..1023
       int r, k;
       if (TA == 0) { r = idx & 63; k = idx >> 6; }
       else { k = idx & 15; r = idx >> 4; }
-      int gr = bm + r, gk = k0 + k;
-      double v = 0.0;
-      if (gr < it.M && gk < it.K)
-        v = TA == 0 ? it.A[gr + (size_t)it.lda * gk] : it.A[gk + (size_t)it.lda * gr];
-      As[r + 64 * k] = v;
+      const int gr = bm + r, gk = k0 + k;
+      pa[e] = (gr < it.M && gk < it.K)
+                  ? (TA == 0 ? it.A[gr + (size_t)it.lda * gk] : it.A[gk + (size_t)it.lda * gr])
+                  : 0.0;
     }
-    // B tile: op(B)[k0 + k][bn + c] -> Bs[k + 16*c]
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      int idx = tid + 256 * e;
+      const int idx = tid + 256 * e;
       int k, c;
       if (TBt == 0) { k = idx & 15; c = idx >> 4; }
       else { c = idx & 63; k = idx >> 6; }
-      int gk = k0 + k, gc = bn + c;
-      double v = 0.0;
-      if (gk < it.K && gc < it.N)
-        v = TBt == 0 ? it.B[gk + (size_t)it.ldb * gc] : it.B[gc + (size_t)it.ldb * gk];
-      Bs[k + 16 * c] = v;
+      const int gk = k0 + k, gc = bn + c;
+      pb[e] = (gk < it.K && gc < it.N)
+                  ? (TBt == 0 ? it.B[gk + (size_t)it.ldb * gc] : it.B[gc + (size_t)it.ldb * gk])
+                  : 0.0;
+    }
+  };
+  double pa[4], pb[4];
+  load_tiles(0, pa, pb);
+  for (int k0 = 0; k0 < it.K; k0 += BK) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      if (TA == 0) As[(idx & 63) + 64 * (idx >> 6)] = pa[e];
+      else As[(idx >> 4) + 64 * (idx & 15)] = pa[e];
+      if (TBt == 0) Bs[(idx & 15) + 16 * (idx >> 4)] = pb[e];
+      else Bs[(idx >> 6) + 16 * (idx & 63)] = pb[e];
     }
     __syncthreads();
+    if (k0 + BK < it.K) load_tiles(k0 + BK, pa, pb);
     mma_tile(acc, As, 64, Bs, 16, BK);
     __syncthreads();
   }
