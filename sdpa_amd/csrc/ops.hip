@@ -128,10 +128,21 @@ __global__ __launch_bounds__(256) void k_reduce_partial(RedBatch B, double* __re
   const double* __restrict__ x = J.x;
   const double* __restrict__ y = J.y;
   double s = (op == 2) ? 1.0e50 : 0.0;
-  for (long t = base + threadIdx.x; t < base + kRedChunk && t < J.len; t += 256) {
-    if (op == 0) s += x[t] * y[t];
-    else if (op == 1) s = fmax(s, fabs(x[t]));
-    else s = fmin(s, x[t] * y[t] * y[t]);
+  // the thread's kRedChunk / 256 elements loaded at once, then taken in order
+  constexpr int kPer = kRedChunk / 256;
+  double xv[kPer], yv[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const long t = base + threadIdx.x + 256 * q;
+    xv[q] = t < J.len ? x[t] : 0.0;
+    yv[q] = (op != 1 && t < J.len) ? y[t] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    if (base + threadIdx.x + 256 * q >= J.len) break;
+    if (op == 0) s += xv[q] * yv[q];
+    else if (op == 1) s = fmax(s, fabs(xv[q]));
+    else s = fmin(s, xv[q] * yv[q] * yv[q]);
   }
   __shared__ double red[256];
   red[threadIdx.x] = s;
