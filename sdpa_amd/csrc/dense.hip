@@ -12,6 +12,7 @@
 #include "common.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include <algorithm>
@@ -718,10 +719,67 @@ __device__ __forceinline__ void syrk_tile(const MatDesc& M, int k0, int kw, int 
       }
 }
 
+// Tile g of the lower-triangular tile grid (R row tiles, Cn column tiles, a
+// column tile as wide as r row tiles, so column j's tiles are rows i >= r j)
+// in a blocked order: bands of sbc columns, each band a triangle at the
+// diagonal (rows [r c0, r c0 + r bw)) and then full rows below; inside the
+// triangle column by column, below it in row bands of r sbc rows, column by
+// column inside a row band.  Consecutive tiles then share a few row and
+// column slices of the panel.  Returns false past the last tile.
+__host__ __device__ inline bool blocked_tile(int g, int R, int Cn, int r, int sbc, int& i,
+                                             int& j) {
+  for (int c0 = 0; c0 < Cn; c0 += sbc) {
+    const int bw = min(sbc, Cn - c0), rs = r * c0;
+    if (rs >= R) return false;
+    const int rd = min(R, rs + r * bw);
+    // the triangle: column jj holds rows [rs + r jj, rd)
+    int tri = 0;
+    if (rd - rs == r * bw) {
+      tri = r * bw * (bw + 1) / 2;
+    } else {
+      for (int jj = 0; jj < bw; ++jj) tri += max(0, rd - rs - r * jj);
+    }
+    if (g < tri) {
+      for (int jj = 0; jj < bw; ++jj) {
+        const int cnt = max(0, rd - rs - r * jj);
+        if (g < cnt) {
+          i = rs + r * jj + g;
+          j = c0 + jj;
+          return true;
+        }
+        g -= cnt;
+      }
+    }
+    g -= tri;
+    const int below = (R - rd) * bw;
+    if (g < below) {
+      const int sbr = r * sbc;
+      const int band = g / (sbr * bw), off = g - band * sbr * bw;
+      const int r0 = rd + band * sbr, nr = min(sbr, R - r0);
+      j = c0 + off / nr;
+      i = r0 + off % nr;
+      return true;
+    }
+    g -= below;
+  }
+  return false;
+}
+
+// Column band of the blocked tile order (0: column by column, the tiles of
+// a bounded grid); SDPA_SYRK_BAND overrides it.
+static int syrk_band() {
+  static const int band = [] {
+    const char* e = getenv("SDPA_SYRK_BAND");
+    const int v = e ? atoi(e) : 8;
+    return v >= 0 && v <= 64 ? v : 8;
+  }();
+  return band;
+}
+
 template <int TM, int TN, int D>
 __global__ __launch_bounds__(64) void k_syrk_update(const MatDesc* __restrict__ mats, int k0,
                                                     int kw, int cb, int ce,
-                                                    const int* __restrict__ info) {
+                                                    const int* __restrict__ info, int sbc) {
   constexpr int BMt = 16 * TM, BNt = 16 * TN;
   const int b = blockIdx.y;
   const MatDesc M = mats[b];
@@ -730,6 +788,19 @@ __global__ __launch_bounds__(64) void k_syrk_update(const MatDesc* __restrict__ 
   const int cend = min(ce, M.n) - cb;
   if (rows <= 0 || cend <= 0) return;
   const int R = (rows + BMt - 1) / BMt, Cn = (cend + BNt - 1) / BNt;
+  if (sbc > 0) {
+    // one matrix, one tile per work-group: work-groups go to the 8 XCDs in
+    // turn (work-group w to XCD w % 8), so XCD x is given the x-th eighth of
+    // the blocked order, in its own dispatch order, and its L2 holds the
+    // panel slices its concurrent tiles share
+    static_assert(BNt % BMt == 0, "column tile a multiple of the row tile");
+    const int per = gridDim.x >> 3;
+    const int g = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    int i = 0, j = 0;
+    if (!blocked_tile(g, R, Cn, BNt / BMt, sbc, i, j)) return;
+    syrk_tile<TM, TN, D>(M, k0, kw, cb, rows, cend, i * BMt, j * BNt);
+    return;
+  }
   // a bounded grid walks the tiles (persistent waves); the tiles are
   // enumerated column by column, rows from the diagonal down
   for (int tile = blockIdx.x;; tile += gridDim.x) {
@@ -797,8 +868,11 @@ static void launch_syrk_t(hipStream_t s, const MatDesc* d_mats, const int* d_inf
   int ntiles = 0;
   for (int j = 0; j < Cn; ++j) ntiles += R - (j * BNt) / BMt;
   if (maxwaves > 0 && nitems == 1) ntiles = std::min(ntiles, maxwaves);
+  // one matrix and one tile per work-group: the XCD-aware blocked tile order
+  const int sbc = (nitems == 1 && maxwaves <= 0) ? syrk_band() : 0;
+  if (sbc) ntiles = (ntiles + 7) / 8 * 8;
   hipLaunchKernelGGL((k_syrk_update<TM, TN, D>), dim3(ntiles, nitems), dim3(64), 0, s, d_mats, k0,
-                     kw, cb, ce, d_info);
+                     kw, cb, ce, d_info, sbc);
 }
 
 // Tile shapes measured on MI355X (FP64): 32x64 tiles with 3 k-steps in flight
@@ -1453,8 +1527,14 @@ int DenseWork::ensure_side(int npanels) {
     SDPA_HIP_CHECK(hipGetDevice(&dev));
     SDPA_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (ncu >= 128) {
+      // SDPA_SIDE_CUS overrides the side stream's CU count (1 .. ncu - 1)
+      int nside = ncu * 3 / 4;
+      if (const char* env = getenv("SDPA_SIDE_CUS")) {
+        const int v = atoi(env);
+        if (v >= 1 && v < ncu) nside = v;
+      }
       std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
-      for (int i = 0; i < ncu * 3 / 4; ++i) mask[i >> 5] |= 1u << (i & 31);
+      for (int i = 0; i < nside; ++i) mask[i >> 5] |= 1u << (i & 31);
       if (hipExtStreamCreateWithCUMask(&side, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
         (void)hipGetLastError();
         side = nullptr;
@@ -1777,12 +1857,21 @@ template <int TM, int TN, int D>
 __global__ __launch_bounds__(64) void k_syrk32_update(float* __restrict__ A, int ld, int n, int k0,
                                                       int kw, int cb, int ce,
                                                       const int* __restrict__ info,
-                                                      double* __restrict__ W64) {
+                                                      double* __restrict__ W64, int sbc) {
   constexpr int BMt = 16 * TM, BNt = 16 * TN;
   if (info[0] != 0) return;
   const int rows = n - cb, cend = min(ce, n) - cb;
   if (rows <= 0 || cend <= 0) return;
   const int R = (rows + BMt - 1) / BMt, Cn = (cend + BNt - 1) / BNt;
+  if (sbc > 0) {  // the XCD-aware blocked order, as in k_syrk_update
+    static_assert(BNt % BMt == 0, "column tile a multiple of the row tile");
+    const int per = gridDim.x >> 3;
+    const int g = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    int i = 0, j = 0;
+    if (!blocked_tile(g, R, Cn, BNt / BMt, sbc, i, j)) return;
+    syrk32_tile<TM, TN, D>(A, ld, k0, kw, cb, rows, cend, i * BMt, j * BNt, W64);
+    return;
+  }
   for (int tile = blockIdx.x;; tile += gridDim.x) {
     int g = tile, i = 0, j = 0;
     for (; j < Cn; ++j) {
@@ -1805,8 +1894,10 @@ static void launch_syrk32_t(hipStream_t s, float* A, int ld, int n, int k0, int 
   int ntiles = 0;
   for (int j = 0; j < Cn; ++j) ntiles += R - (j * BNt) / BMt;
   if (maxwaves > 0) ntiles = std::min(ntiles, maxwaves);
+  const int sbc = maxwaves <= 0 ? syrk_band() : 0;
+  if (sbc) ntiles = (ntiles + 7) / 8 * 8;
   hipLaunchKernelGGL((k_syrk32_update<TM, TN, D>), dim3(ntiles), dim3(64), 0, s, A, ld, n, k0, kw,
-                     cb, ce, d_info, W64);
+                     cb, ce, d_info, W64, sbc);
 }
 
 // W64 (optional): also write the updated columns, widened, to the FP64
