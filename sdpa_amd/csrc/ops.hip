@@ -11,6 +11,7 @@
 // Reductions are two-pass with a fixed order, so results are reproducible.
 #include "ctx.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace sdpa_amd {
@@ -85,6 +86,52 @@ __global__ void k_axpbypcz(long len, double a, const double* __restrict__ A, dou
   if (B) r += B[t] * b;
   if (Cv) r += Cv[t] * cc;
   out[t] = r;
+}
+
+// Two independent k_axpbypcz operations in one launch: operation 0 over
+// the first len0 threads, operation 1 over the next len1.
+struct Axpb {
+  long len;
+  double a, b, cc;
+  const double* A;
+  const double* B;
+  const double* Cv;
+  double* out;
+};
+
+__global__ void k_axpbypcz2(Axpb o0, Axpb o1) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool second = t >= o0.len;
+  const Axpb& o = second ? o1 : o0;
+  if (second) t -= o0.len;
+  if (t >= o.len) return;
+  double r = o.a * o.A[t];
+  if (o.B) r += o.B[t] * o.b;
+  if (o.Cv) r += o.Cv[t] * o.cc;
+  o.out[t] = r;
+}
+
+// The new point in one launch (sdpa_solve.cpp:167-190): X += alphaP Dx,
+// Z += alphaD Dz over the packed blocks (len total) and y += alphaD dy
+// (len m), each element as k_axpbypcz forms it.
+__global__ void k_update_point(long total, long m, double ap, double ad, double* __restrict__ X,
+                               const double* __restrict__ Dx, double* __restrict__ Z,
+                               const double* __restrict__ Dz, double* __restrict__ y,
+                               const double* __restrict__ dy) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < total) {
+    double rx = 1.0 * X[t];
+    rx += Dx[t] * ap;
+    double rz = 1.0 * Z[t];
+    rz += Dz[t] * ad;
+    X[t] = rx;
+    Z[t] = rz;
+  }
+  if (t < m) {
+    double ry = 1.0 * y[t];
+    ry += dy[t] * ad;
+    y[t] = ry;
+  }
 }
 
 // Lal::getSymmetrize per SDP block: a_ij = a_ji = (a_ij + a_ji) * 0.5
@@ -332,10 +379,16 @@ static int triple(Ctx* c, const double* A, const double* B, const double* Cm, do
 // slot, slot + 1, copied to c->h_red[slot..]; no synchronization
 int residuals_enqueue(Ctx* c, int slot) {
   if (int e = a_dot(c, c->d_X, c->d_pvec)) return e;
-  if (int e = axpbypcz(c, c->m, -1.0, c->d_pvec, 1.0, c->d_b, 0.0, nullptr, c->d_pvec)) return e;
   if (int e = sum_yA(c, c->d_y, c->d_dmat)) return e;
-  if (int e = axpbypcz(c, c->total, -1.0, c->d_dmat, 1.0, c->d_C, -1.0, c->d_Z, c->d_dmat))
-    return e;
+  {
+    // p = b - p and D = C - D - Z, one launch
+    const Axpb o0{(long)c->m, -1.0, 1.0, 0.0, c->d_pvec, c->d_b, nullptr, c->d_pvec};
+    const Axpb o1{(long)c->total, -1.0, 1.0, -1.0, c->d_dmat, c->d_C, c->d_Z, c->d_dmat};
+    const long len = o0.len + o1.len;
+    if (len > 0)
+      hipLaunchKernelGGL(k_axpbypcz2, grid1(len), dim3(256), 0, c->stream, o0, o1);
+    SDPA_HIP_CHECK(hipGetLastError());
+  }
   RedBatch B;
   B.n = 2;
   B.j[0] = RedJob{c->d_pvec, nullptr, (long)c->m, 1, slot};
@@ -450,9 +503,13 @@ int make_direction(Ctx* c, int infeasible) {
 }
 
 int update_point(Ctx* c, double alphaP, double alphaD) {
-  if (int e = axpbypcz(c, c->total, 1.0, c->d_X, alphaP, c->d_Dx, 0.0, nullptr, c->d_X)) return e;
-  if (int e = axpbypcz(c, c->m, 1.0, c->d_y, alphaD, c->d_dy, 0.0, nullptr, c->d_y)) return e;
-  return axpbypcz(c, c->total, 1.0, c->d_Z, alphaD, c->d_Dz, 0.0, nullptr, c->d_Z);
+  const long len = std::max<long>(c->total, c->m);
+  if (len <= 0) return 0;
+  hipLaunchKernelGGL(k_update_point, grid1(len), dim3(256), 0, c->stream, (long)c->total,
+                     (long)c->m, alphaP, alphaD, c->d_X, c->d_Dx, c->d_Z, c->d_Dz, c->d_y,
+                     c->d_dy);
+  SDPA_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 // The launches of inverse_chol and the copy of its status words (one per
