@@ -460,21 +460,36 @@ int compute_dots(Ctx* c, int nq, const int* codes, double* out) {
 }
 
 int make_gvec(Ctx* c, int corrector, double target, int infeasible) {
-  // r_zinv = target * Zinv - X  (- Dx Dz Zinv in the corrector)
-  if (int e = axpbypcz(c, c->total, target, c->d_Zinv, -1.0, c->d_X, 0.0, nullptr, c->d_rzinv))
-    return e;
-  if (corrector) {
+  // r_zinv = target * Zinv - X  (- Dx Dz Zinv in the corrector), and in the
+  // infeasible phase w1 = r_zinv - X D Zinv; the products first, then the
+  // element-wise steps in one launch (each element formed in the same order
+  // of operations as by separate steps)
+  if (corrector)
     if (int e = triple(c, c->d_Dx, c->d_Dz, c->d_Zinv, c->d_w1)) return e;
-    if (int e = axpbypcz(c, c->total, 1.0, c->d_rzinv, -1.0, c->d_w1, 0.0, nullptr, c->d_rzinv))
-      return e;
-  }
+  if (infeasible && !corrector)
+    if (int e = triple(c, c->d_X, c->d_dmat, c->d_Zinv, c->d_xrdzinv)) return e;
+  const Axpb r{c->total, target, -1.0, -1.0, c->d_Zinv, c->d_X, corrector ? c->d_w1 : nullptr,
+               c->d_rzinv};
+  Axpb w{0, 0.0, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr};
   const double* D = c->d_rzinv;
   if (infeasible) {
-    if (!corrector)
-      if (int e = triple(c, c->d_X, c->d_dmat, c->d_Zinv, c->d_xrdzinv)) return e;
-    if (int e = axpbypcz(c, c->total, 1.0, c->d_rzinv, -1.0, c->d_xrdzinv, 0.0, nullptr, c->d_w1))
-      return e;
+    if (corrector) {
+      // needs the finished r_zinv: its own step
+      if (int e = axpbypcz(c, c->total, target, c->d_Zinv, -1.0, c->d_X, -1.0, c->d_w1,
+                           c->d_rzinv))
+        return e;
+      if (int e = axpbypcz(c, c->total, 1.0, c->d_rzinv, -1.0, c->d_xrdzinv, 0.0, nullptr,
+                           c->d_w1))
+        return e;
+    } else {
+      w = Axpb{c->total, target, -1.0, -1.0, c->d_Zinv, c->d_X, c->d_xrdzinv, c->d_w1};
+    }
     D = c->d_w1;
+  }
+  if (!(infeasible && corrector)) {
+    const long len = r.len + w.len;
+    if (len > 0) hipLaunchKernelGGL(k_axpbypcz2, grid1(len), dim3(256), 0, c->stream, r, w);
+    SDPA_HIP_CHECK(hipGetLastError());
   }
   // g = -A(D) + p
   if (int e = a_dot(c, D, c->d_g)) return e;
