@@ -130,9 +130,10 @@ int trtri_batched(hipStream_t s, DenseWork* w, const MatDesc* h_L, const MatDesc
 int diag_block_inverses(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
                         double* Winv, const double* dinv64 = nullptr);
 // x := L^-T L^-1 x (two triangular solves with the lower factor L, m x m),
-// using the block inverses from diag_block_inverses.
+// using the block inverses from diag_block_inverses; rhs (optional): the
+// right-hand side read from there instead of x, left unchanged.
 int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, const double* Winv,
-               double* x);
+               double* x, const double* rhs = nullptr);
 // The same with a float factor (mixed precision); returns 1 if the
 // pipelined kernels do not apply to this size (m <= 512 included).
 int chol_solve_f32(hipStream_t s, DenseWork* w, const float* L, int ld, int m, const double* Winv,

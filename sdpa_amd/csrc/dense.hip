@@ -2852,7 +2852,7 @@ __global__ __launch_bounds__(kLLThreads) void k_trsv_bwd_ll(LLParams P) {
 // with its LDS) and the caller falls back
 template <typename T>
 static int chol_solve_ll(hipStream_t s, DenseWork* w, const T* L, int ld, int m,
-                         const double* Winv, double* x) {
+                         const double* Winv, double* x, const double* rhs = nullptr) {
   const int ns = (m + 63) / 64, nb = (m + SB - 1) / SB;
   const size_t lds = (size_t)nb * SB * sizeof(double);
   if (nb > 48 || (size_t)ld * m * sizeof(T) >= (size_t)1 << 31) return 1;
@@ -2885,7 +2885,7 @@ static int chol_solve_ll(hipStream_t s, DenseWork* w, const T* L, int ld, int m,
   double* t = w->d_vec;
   // forward: t = L^-1 x
   if (++w->ll_epoch == 0) ++w->ll_epoch;
-  P.rhs = x; P.out = t; P.epoch = w->ll_epoch;
+  P.rhs = rhs ? rhs : x; P.out = t; P.epoch = w->ll_epoch;
   void* args[] = {&P};
   SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_fwd_ll<T>, dim3(ns), dim3(kLLThreads),
                                             args, lds, s));
@@ -2903,7 +2903,7 @@ static int chol_solve_ll(hipStream_t s, DenseWork* w, const T* L, int ld, int m,
 // x := L^-T L^-1 x by the two cooperative launches; returns 1 when the
 // device cannot launch them cooperatively (the caller falls back)
 static int chol_solve_coop(hipStream_t s, DenseWork* w, const double* L, int ld, int m,
-                           const double* Winv, double* x) {
+                           const double* Winv, double* x, const double* rhs = nullptr) {
   if (w->coop_state == 0) {
     int dev = 0, coop = 0;
     SDPA_HIP_CHECK(hipGetDevice(&dev));
@@ -2919,7 +2919,7 @@ static int chol_solve_coop(hipStream_t s, DenseWork* w, const double* L, int ld,
   double* t = w->d_vec;
   double* y = w->d_vec + m;
   // forward: t = L^-1 x
-  P.rhs = x; P.y = y; P.out = t; P.epoch = ++w->epoch;
+  P.rhs = rhs ? rhs : x; P.y = y; P.out = t; P.epoch = ++w->epoch;
   void* args[] = {&P};
   SDPA_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_trsv_fwd, dim3(ns), dim3(256), args, 0, s));
   // backward: x = L^-T t
@@ -2932,20 +2932,27 @@ static int chol_solve_coop(hipStream_t s, DenseWork* w, const double* L, int ld,
 }
 
 int chol_solve(hipStream_t s, DenseWork* w, const double* L, int ld, int m, const double* Winv,
-               double* x) {
+               double* x, const double* rhs) {
   if (m <= 0) return 0;
+  if (rhs == x) rhs = nullptr;
   if (m > 2 * SB) {
-    int r = chol_solve_ll<double>(s, w, L, ld, m, Winv, x);
+    int r = chol_solve_ll<double>(s, w, L, ld, m, Winv, x, rhs);
     if (r <= 0) return r;
-    r = chol_solve_coop(s, w, L, ld, m, Winv, x);
+    r = chol_solve_coop(s, w, L, ld, m, Winv, x, rhs);
     if (r <= 0) return r;
   }
   if (int e = w->ensure_vec((size_t)m)) return e;
   double* t = w->d_vec;
   const int nb = (m + SB - 1) / SB;
+  // one block: the forward step only reads the right-hand side; more: the
+  // forward updates consume it in x
+  if (rhs && nb > 1) {
+    SDPA_HIP_CHECK(hipMemcpyAsync(x, rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+    rhs = nullptr;
+  }
   // forward: t = L^-1 x   (x is consumed as the right-hand side)
   for (int b = 0; b < nb; ++b) {
-    hipLaunchKernelGGL(k_block_gemv, dim3(8), dim3(256), 0, s, Winv, m, b, 0, x, t);
+    hipLaunchKernelGGL(k_block_gemv, dim3(8), dim3(256), 0, s, Winv, m, b, 0, rhs ? rhs : x, t);
     const int beyond = m - (b + 1) * SB;
     if (beyond > 0)
       hipLaunchKernelGGL(k_fwd_update, dim3((beyond + 63) / 64), dim3(256), 0, s, L, ld, m, b, t, x);

@@ -111,6 +111,17 @@ __global__ void k_axpbypcz2(Axpb o0, Axpb o1) {
   o.out[t] = r;
 }
 
+// a0 -> b0 and a1 -> b1 (len doubles each) in one launch
+__global__ void k_copy2(long len, const double* __restrict__ a0, double* __restrict__ b0,
+                        const double* __restrict__ a1, double* __restrict__ b1) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < len) {
+    b0[t] = a0[t];
+  } else if (t < 2 * len) {
+    b1[t - len] = a1[t - len];
+  }
+}
+
 // The new point in one launch (sdpa_solve.cpp:167-190): X += alphaP Dx,
 // Z += alphaD Dz over the packed blocks (len total) and y += alphaD dy
 // (len m), each element as k_axpbypcz forms it.
@@ -535,10 +546,9 @@ int inverse_chol_enqueue(Ctx* c) {
   // L = chol(X), invCholX = L^-1 ; L = chol(Z), invCholZ = L^-1, Zinv = L^-T L^-1
   const int nb = c->nSDP;
   if (nb > 0) {
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->d_LX, c->d_X, sizeof(double) * c->lpOff,
-                                  hipMemcpyDeviceToDevice, c->stream));
-    SDPA_HIP_CHECK(hipMemcpyAsync(c->d_LZ, c->d_Z, sizeof(double) * c->lpOff,
-                                  hipMemcpyDeviceToDevice, c->stream));
+    if (c->lpOff > 0)
+      hipLaunchKernelGGL(k_copy2, grid1(2 * (long)c->lpOff), dim3(256), 0, c->stream,
+                         (long)c->lpOff, c->d_X, c->d_LX, c->d_Z, c->d_LZ);
     std::vector<MatDesc> L(2 * nb), W(2 * nb);
     for (int q = 0; q < nb; ++q) {
       const int n = c->sdpN[q];

@@ -1059,9 +1059,7 @@ int sdpa_gpu_solve_bmat(sdpa_gpu_ctx* h, int refine) {
     c->n_solve++;
     return r;
   }
-  SDPA_HIP_CHECK(hipMemcpyAsync(c->d_dy, c->d_g, sizeof(double) * c->m, hipMemcpyDeviceToDevice,
-                                c->stream));
-  int r = chol_solve(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv, c->d_dy);
+  int r = chol_solve(c->stream, &c->dw, c->d_B, c->m, c->m, c->d_Winv, c->d_dy, c->d_g);
   c->t_solve += toc(c);
   c->n_solve++;
   return r;
